@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Headline benchmark: D3Q19 BGK periodic box 256^3, FP64, tau=0.8, Guo body
+force F=(1e-6,0,0), rho=1 and seeded uniform noise u in [-1e-3,1e-3], one
+lattice per GPU.  A step is one mfa_step launch (pull + moments + BGK + Guo,
+periodic wrap inline) over the whole 256^3 box with inputs resident in HBM.
+
+Prints one JSON line with MLUPS, the roofline of the step kernel (HBM bytes
+per lattice update = 2*19*8 = 304, one read and one write of every
+population) and the CPU baseline (the oracle, timed on this host's cores on a
+bounded number of steps of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS & %HBM roofline, D3Q19 256^3; time-to-steady warm vs plain @1e-8"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--n", type=int, default=256)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="target duration of the bounded CPU baseline sample")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="also run a plain device copy of known size (counter calibration)")
+    p.add_argument("--seed", type=int, default=2024)
+    return p.parse_args()
+
+
+def cpu_baseline(n, steps_hint, seconds):
+    """Time the oracle (the reference's algorithm in C, OpenMP over z) on the
+    same workload for a bounded number of steps."""
+    import numpy as np
+    from oracle import oracle as O
+
+    L = O.Lattice("D3Q19")
+    nn = [n, n, n]
+    F = [1e-6, 0.0, 0.0]
+    rng = np.random.default_rng(2024)
+    rho = np.ones((n, n, n))
+    vel = rng.uniform(-1e-3, 1e-3, (3, n, n, n))
+    f, _, _ = O.init_equilibrium(L, rho, vel, nn, F)
+    del rho, vel
+    res = O.resolve_bc(3, [O.BC["Periodic"]] * 6)
+    O.fill_periodic(L, f, nn, [1, 1, 1])
+    fb = np.empty_like(f)
+    cores = min(os.cpu_count() or 1, 16)
+    t1 = O.time_run(L, f, fb, nn, 1.0 / 0.8, F, [1, 1, 1], res, 1, cores)
+    steps = max(2, min(steps_hint, int(seconds / max(t1, 1e-3))))
+    t = O.time_run(L, f, fb, nn, 1.0 / 0.8, F, [1, 1, 1], res, steps, cores)
+    mlups = n ** 3 * steps / t / 1e6
+    return {"value": round(mlups, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"D3Q19 {n}^3 FP64 forced periodic, {steps} steps after 1 warm-up step, "
+                      f"{t:.2f} s, OpenMP over z on {cores} threads "
+                      f"(= {mlups * 304 / 1e3:.1f} GB/s algorithmic)"}
+
+
+def traffic_from_profiles(n):
+    path = os.path.join(ROOT, "profiles", f"step_d3q19_{n}_f64_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            return json.load(fh).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU halo exchange "
+                                   "is designed but not built (use --gpus 1)"}))
+        sys.exit(2)
+
+    import torch
+    from muphfasa_amd import abi
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    N3 = n ** 3
+    lattice = abi.LATTICE["D3Q19"]
+    F = [1e-6, 0.0, 0.0]
+    tau = 0.8
+    omega = 1.0 / tau
+    g = abi.resolve_bc(lattice, [abi.BC["Periodic"]] * 6)
+
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed)
+    rho = torch.ones((1, n, n, n), dtype=torch.float64, device=dev)
+    vel = (torch.rand((3, n, n, n), dtype=torch.float64, device=dev, generator=gen) * 2 - 1) * 1e-3
+    fa_t = torch.empty((19, n, n, n), dtype=torch.float64, device=dev)
+    fb_t = torch.empty_like(fa_t)
+    fa, fb = abi.field(fa_t, [n] * 3, 0, 3), abi.field(fb_t, [n] * 3, 0, 3)
+    stream = torch.cuda.current_stream()
+    abi.init_equilibrium(abi.field(rho, [n] * 3, 0, 3), abi.field(vel, [n] * 3, 0, 3), fa,
+                         lattice, F, stream)
+    del rho, vel
+
+    out = torch.zeros(4, dtype=torch.float64, device=dev)
+    abi.reduce(fa, lattice, abi.RED["mass"], out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    mass0 = out[0].item()
+
+    cur, nxt = fa, fb
+    for _ in range(args.warmup):
+        abi.step(cur, nxt, g, omega, F, stream)
+        cur, nxt = nxt, cur
+
+    torch.cuda.synchronize()
+    t_begin = time.perf_counter()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        abi.step(cur, nxt, g, omega, F, stream)
+        cur, nxt = nxt, cur
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_begin
+    ms = ev0.elapsed_time(ev1)
+    ms_per_step = ms / args.steps
+
+    abi.reduce(cur, lattice, abi.RED["mass"], out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    mass1 = out[0].item()
+
+    mlups = N3 / (ms_per_step * 1e-3) / 1e6
+    bytes_per_lu = 2 * 19 * 8
+    achieved = N3 * bytes_per_lu / (ms_per_step * 1e-3) / 1e9
+    traffic = traffic_from_profiles(n)
+
+    if args.copy_probe:
+        src = torch.empty(N3 * 19, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        for _ in range(3):
+            dst.copy_(src)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        reps = 20
+        for _ in range(reps):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        copy_gbs = 2 * src.numel() * 8 / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+        print(json.dumps({"copy_probe_bytes_per_launch": 2 * src.numel() * 8,
+                          "copy_probe_GBs": round(copy_gbs, 1)}), file=sys.stderr, flush=True)
+        del src, dst
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        del fa_t, fb_t
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(n, 50, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (rho=1, u seeded uniform noise in [-1e-3,1e-3])",
+        "config": {"workload": f"D3Q19 BGK periodic {n}^3, FP64 populations, tau=0.8, "
+                               f"Guo force (1e-6,0,0), {args.steps} timed steps",
+                   "lattice": "D3Q19", "cells": [n, n, n], "tau": tau, "force": F,
+                   "storage": "SoA [q][z][y][x], no halo (periodic wrap inline)"},
+        "hbm_roofline_pct": round(100.0 * achieved / HBM_PEAK_GBS, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "k_step<D3Q19,forced,fused>",
+                     "algorithmic_bytes_per_launch": N3 * bytes_per_lu},
+        "mass_rel_drift": abs(mass1 - mass0) / mass0,
+        "wall_s_timed": round(wall, 4),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

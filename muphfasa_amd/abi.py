@@ -1,0 +1,174 @@
+"""ctypes binding of the step engine's C ABI (include/mfa_lbm.h).
+
+This is the Python-side equivalent of the binding a C++ or Fortran caller
+would write: plain structs and pointers, no torch types in the ABI.  torch is
+used only to own device memory and to name the HIP stream.  Loading fails
+loudly when the built library is missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmfa_lbm.so")
+
+LATTICE = {"D2Q9": 0, "D3Q15": 1, "D3Q19": 2, "D3Q27": 3}
+LATTICE_DQ = {0: (2, 9), 1: (3, 15), 2: (3, 19), 3: (3, 27)}
+BC = {"MovingWall": 0, "NoSlip": 1, "Inflow": 2, "Outflow": 3, "PressureOutlet": 4,
+      "Periodic": 5}
+DTYPE = {"f64": 0, "f32": 1, "f16dev": 2, "bf16dev": 3}
+RED = {"mass": 0, "momentum": 1, "comp_sum": 2}
+
+EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_pull_collide",
+           "mfa_fill_periodic", "mfa_apply_bc", "mfa_moments", "mfa_init_equilibrium",
+           "mfa_reduce", "mfa_residual"]
+
+
+class MfaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mfa status {code}: {msg}")
+        self.code = code
+
+
+class Field(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("n", ctypes.c_int * 3), ("ngrow", ctypes.c_int),
+                ("sy", ctypes.c_longlong), ("sz", ctypes.c_longlong), ("sq", ctypes.c_longlong),
+                ("ncomp", ctypes.c_int), ("dtype", ctypes.c_int)]
+
+
+class GeomBC(ctypes.Structure):
+    _fields_ = [("lattice", ctypes.c_int), ("periodic", ctypes.c_int * 3),
+                ("resolved_type", ctypes.c_int * 27), ("resolved_rho", ctypes.c_double * 27),
+                ("resolved_u", (ctypes.c_double * 3) * 27)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libmfa_lbm.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m muphfasa_amd.build` "
+                "(there is no CPU fallback for the step engine)")
+        L = ctypes.CDLL(LIB_PATH)
+        F = ctypes.POINTER(Field)
+        G = ctypes.POINTER(GeomBC)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.mfa_version.restype = ctypes.c_char_p
+        L.mfa_last_error.restype = ctypes.c_char_p
+        L.mfa_resolve_bc.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp, dp, G]
+        L.mfa_step.argtypes = [F, F, G, ctypes.c_double, dp, vp]
+        L.mfa_pull_collide.argtypes = [F, F, ctypes.c_int, ctypes.c_double, dp, vp]
+        L.mfa_fill_periodic.argtypes = [F, G, vp]
+        L.mfa_apply_bc.argtypes = [F, G, vp]
+        L.mfa_moments.argtypes = [F, F, F, G, dp, ctypes.c_int, vp]
+        L.mfa_init_equilibrium.argtypes = [F, F, F, ctypes.c_int, dp, vp]
+        L.mfa_reduce.argtypes = [F, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.mfa_residual.argtypes = [F, F, vp, vp]
+        for name in EXPORTS[2:]:
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != 0:
+        raise MfaError(rc, lib().mfa_last_error().decode())
+
+
+def _dbl(vals, n=3):
+    if vals is None:
+        return None
+    a = (ctypes.c_double * n)()
+    for i, v in enumerate(list(vals)[:n]):
+        a[i] = float(v)
+    return a
+
+
+def resolve_bc(lattice: int, face_type, face_rho=None, face_u=None) -> GeomBC:
+    """BCManager::finalize on the host (C ABI mfa_resolve_bc)."""
+    D = LATTICE_DQ[lattice][0]
+    ft = list(face_type) + [BC["Periodic"]] * (6 - len(face_type))
+    fta = (ctypes.c_int * 6)(*ft)
+    fr = _dbl(face_rho if face_rho is not None else [1.0] * 6, 6)
+    fu = (ctypes.c_double * 18)()
+    if face_u is not None:
+        for f in range(6):
+            for d in range(3):
+                fu[3 * f + d] = float(face_u[f][d])
+    g = GeomBC()
+    check(lib().mfa_resolve_bc(lattice, fta, fr, fu, ctypes.byref(g)))
+    return g
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def field(t, n, ngrow: int, lattice_dim: int, dtype: int = 0) -> Field:
+    """mfa_field over a contiguous tensor of shape (ncomp, e2, e1, e0) whose
+    extents are n_d + 2*ngrow in the lattice's dimensions (z has no ghost in
+    2-D) or larger (padding)."""
+    assert t.is_contiguous()
+    ncomp, e2, e1, e0 = t.shape
+    n = list(n) + [1] * (3 - len(n))
+    gz = ngrow if lattice_dim == 3 else 0
+    assert e0 >= n[0] + 2 * ngrow and e1 >= n[1] + 2 * ngrow and e2 >= n[2] + 2 * gz
+    f = Field()
+    esz = t.element_size()
+    off = ngrow + ngrow * e0 + gz * e0 * e1
+    f.data = t.data_ptr() + off * esz
+    for d in range(3):
+        f.n[d] = n[d]
+    f.ngrow = ngrow
+    f.sy, f.sz, f.sq = e0, e0 * e1, e0 * e1 * e2
+    f.ncomp = ncomp
+    f.dtype = dtype
+    return f
+
+
+def step(fin: Field, fout: Field, g: GeomBC, omega: float, force=None, stream=None):
+    check(lib().mfa_step(ctypes.byref(fin), ctypes.byref(fout), ctypes.byref(g), omega,
+                         _dbl(force), stream_ptr(stream)))
+
+
+def pull_collide(fin: Field, fout: Field, lattice: int, omega: float, force=None, stream=None):
+    check(lib().mfa_pull_collide(ctypes.byref(fin), ctypes.byref(fout), lattice, omega,
+                                 _dbl(force), stream_ptr(stream)))
+
+
+def fill_periodic(f: Field, g: GeomBC, stream=None):
+    check(lib().mfa_fill_periodic(ctypes.byref(f), ctypes.byref(g), stream_ptr(stream)))
+
+
+def apply_bc(f: Field, g: GeomBC, stream=None):
+    check(lib().mfa_apply_bc(ctypes.byref(f), ctypes.byref(g), stream_ptr(stream)))
+
+
+def moments(f: Field, rho: Field, vel: Field, g: GeomBC, force=None, streamed=True,
+            stream=None):
+    check(lib().mfa_moments(ctypes.byref(f), ctypes.byref(rho), ctypes.byref(vel),
+                            ctypes.byref(g), _dbl(force), 1 if streamed else 0,
+                            stream_ptr(stream)))
+
+
+def init_equilibrium(rho: Field, vel: Field, f: Field, lattice: int, force=None, stream=None):
+    check(lib().mfa_init_equilibrium(ctypes.byref(rho), ctypes.byref(vel), ctypes.byref(f),
+                                     lattice, _dbl(force), stream_ptr(stream)))
+
+
+def reduce(f: Field, lattice: int, what: int, out_ptr: int, stream=None):
+    check(lib().mfa_reduce(ctypes.byref(f), lattice, what, ctypes.c_void_p(out_ptr),
+                           stream_ptr(stream)))
+
+
+def residual(a: Field, b: Field, out_ptr: int, stream=None):
+    check(lib().mfa_residual(ctypes.byref(a), ctypes.byref(b), ctypes.c_void_p(out_ptr),
+                             stream_ptr(stream)))

@@ -1,0 +1,930 @@
+// mfa_lbm.hip -- gfx950 kernels and C ABI of the LBM step engine.
+//
+// Layout: structure of arrays, one population per component plane, x fastest
+// (see include/mfa_lbm.h).  A wavefront covers 64 consecutive x cells, so
+// every population load and store of a wavefront is one contiguous 512-byte
+// (FP64) run; pulls with c_x = +-1 are shifted by one element and share
+// cache lines with the neighbouring wavefront of the same block.
+//
+// mfa_step is the hot path: a single pass that reads each population of fin
+// once and writes each population of fout once (2*Q*8 bytes per cell in FP64),
+// with periodic wrap and wall ghost values computed inline, so no halo pass
+// and no extra launch per step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <type_traits>
+
+#include "lattice.hpp"
+
+namespace mfa {
+
+// ---------------------------------------------------------------- errors ---
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define MFA_HIP_CHECK(expr)                                                         \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return fail(MFA_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------- kernel arguments --
+struct Geo {
+    int n[3];
+    int per[3];
+    int walls;            // some dimension is non-periodic
+    signed char rtype[27];
+    double rrho[27];
+    double ru[27][3];
+};
+
+struct View {             // one field, addressed relative to cell (0,0,0)
+    long long sy, sz, sq;
+};
+
+static View view_of(const mfa_field& f) { return View{f.sy, f.sz, f.sq}; }
+
+// ------------------------------------------------------------- pull stage --
+// Coordinates of the pull source for each lattice velocity component value
+// (index c+1): source = x - c.  Fused mode wraps periodic dimensions and
+// clamps non-periodic ones (the clamped load is replaced by the wall value).
+struct Src {
+    long long o[3][3];    // per dimension, per (c+1): element offset
+};
+
+template <int D, bool HALO>
+__device__ __forceinline__ Src make_src(int x, int y, int z, const Geo& g, const View& v)
+{
+    Src s;
+    const int p[3] = {x, y, z};
+    const long long st[3] = {1, v.sy, v.sz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci) {
+            const int c = ci - 1;
+            int q = p[d] - c;
+            if (d >= D) q = p[d];
+            else if (!HALO) {
+                if (q < 0) q = g.per[d] ? g.n[d] - 1 : 0;
+                else if (q >= g.n[d]) q = g.per[d] ? 0 : g.n[d] - 1;
+            }
+            s.o[d][ci] = (long long)q * st[d];
+        }
+    }
+    return s;
+}
+
+// Ghost value f(g, q) for g = x - c_q outside a non-periodic face, evaluated
+// from fin's valid cells as the reference's BCManager would have written it
+// (src/LBM/BoundaryConditions.H:124-189 and the src/LBM/BCTypes/ headers).
+template <class L, int q>
+__device__ __forceinline__ double wall_value(const double* __restrict__ fin, long long self,
+                                             const View& v, const Geo& g, int idx)
+{
+    const int t = g.rtype[idx];
+    if (t == MFA_BC_MOVING_WALL || t == MFA_BC_NO_SLIP || t == MFA_BC_INFLOW) {
+        constexpr int m = L::opp(q);
+        return fin[m * v.sq + self] +
+               bb_contrib<L, m>(g.rrho[idx], g.ru[idx][0], g.ru[idx][1], g.ru[idx][2]);
+    }
+    if (t == MFA_BC_OUTFLOW) return fin[q * v.sq + self];
+    if (t == MFA_BC_PRESSURE_OUTLET) {
+        // src/LBM/BCTypes/PressureOutletKernel.H:29-64
+        double rho = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < L::Q; ++m) {
+            const double fm = fin[m * v.sq + self];
+            rho += fm;
+            j0 += L::c(m, 0) * fm;
+            j1 += L::c(m, 1) * fm;
+            if constexpr (L::D == 3) j2 += L::c(m, 2) * fm;
+        }
+        const double inv_rho = 1.0 / rho;
+        const double cu = cdot<L, q>(j0, j1, j2) * inv_rho;
+        double uu;
+        if constexpr (L::D == 2) uu = (j0 * j0 + j1 * j1) * inv_rho * inv_rho;
+        else uu = (j0 * j0 + j1 * j1 + j2 * j2) * inv_rho * inv_rho;
+        const double feq_even =
+            L::w(q) * g.rrho[idx] * (1.0 + 0.5 * cu * cu * 3.0 * 3.0 - 0.5 * uu * 3.0);
+        constexpr int m = L::opp(q);
+        return -fin[m * v.sq + self] + 2.0 * feq_even;
+    }
+    return __builtin_nan("");
+}
+
+template <class L, int q>
+__device__ __forceinline__ void fix_wall(double& fq, const double* __restrict__ fin, long long self,
+                                         const View& v, const Geo& g, const int lo[3],
+                                         const int hi[3])
+{
+    int code[3] = {0, 0, 0};
+    bool wall = false;
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+        const int c = L::c(q, d);
+        if (c == 1 && lo[d]) { code[d] = -1; wall = true; }
+        if (c == -1 && hi[d]) { code[d] = 1; wall = true; }
+    }
+    if (wall) {
+        const int idx = (code[0] + 1) + 3 * (code[1] + 1) + 9 * (code[2] + 1) * (L::D == 3);
+        fq = wall_value<L, q>(fin, self, v, g, idx);
+    }
+}
+
+template <class L, int q>
+struct Unroll {
+    template <class Fn>
+    static __device__ __forceinline__ void run(Fn&& fn)
+    {
+        fn(std::integral_constant<int, q>{});
+        if constexpr (q + 1 < L::Q) Unroll<L, q + 1>::run(fn);
+    }
+};
+
+template <class L, class Fn>
+__device__ __forceinline__ void for_q(Fn&& fn) { Unroll<L, 0>::run(fn); }
+
+// Pull the Q populations of cell (x,y,z).  mode: 0 fused (inline wrap and
+// walls), 1 halo (reads ghost cells), 2 unstreamed (the cell's own values).
+template <class L, int MODE>
+__device__ __forceinline__ void pull(double (&f)[L::Q], const double* __restrict__ fin, int x, int y,
+                                     int z, const Geo& g, const View& v)
+{
+    const long long self = x + y * v.sy + z * v.sz;
+    if constexpr (MODE == 2) {
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = fin[q * v.sq + self];
+        });
+        return;
+    } else {
+        const Src s = make_src<L::D, MODE == 1>(x, y, z, g, v);
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const long long off = s.o[0][L::c(q, 0) + 1] + s.o[1][L::c(q, 1) + 1] +
+                                  s.o[2][L::c(q, 2) + 1];
+            f[q] = fin[q * v.sq + off];
+        });
+        if constexpr (MODE == 0) {
+            if (g.walls) {
+                const int p[3] = {x, y, z};
+                int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+                bool edge = false;
+#pragma unroll
+                for (int d = 0; d < L::D; ++d) {
+                    lo[d] = !g.per[d] && p[d] == 0;
+                    hi[d] = !g.per[d] && p[d] == g.n[d] - 1;
+                    edge = edge || lo[d] || hi[d];
+                }
+                if (edge) {
+                    for_q<L>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        fix_wall<L, q>(f[q], fin, self, v, g, lo, hi);
+                    });
+                }
+            }
+        }
+    }
+}
+
+// rho and u of Q populations: src/LBM/Macroscopic.H:36-45,115-127,186-201.
+template <class L, bool FORCED>
+__device__ __forceinline__ void moments(const double (&f)[L::Q], const double F[3], double& rho,
+                                        double& u0, double& u1, double& u2)
+{
+    rho = 0.;
+    double j0 = 0., j1 = 0., j2 = 0.;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        rho += f[q];
+    });
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (L::c(q, 0) != 0) j0 = (L::c(q, 0) > 0) ? j0 + f[q] : j0 - f[q];
+        if constexpr (L::c(q, 1) != 0) j1 = (L::c(q, 1) > 0) ? j1 + f[q] : j1 - f[q];
+        if constexpr (L::D == 3 && L::c(q, 2) != 0) j2 = (L::c(q, 2) > 0) ? j2 + f[q] : j2 - f[q];
+    });
+    if constexpr (FORCED) {
+        u0 = (j0 + 0.5 * F[0]) / rho;
+        u1 = (j1 + 0.5 * F[1]) / rho;
+        u2 = (L::D == 3) ? (j2 + 0.5 * F[2]) / rho : 0.0;
+    } else {
+        u0 = j0 / rho;
+        u1 = j1 / rho;
+        u2 = (L::D == 3) ? j2 / rho : 0.0;
+    }
+}
+
+// ----------------------------------------------------------- step kernel --
+// One thread per cell; block = (BX, 256/BX); grid.z = z.
+template <class L, bool FORCED, int MODE>
+__global__ __launch_bounds__(256) void k_step(const double* __restrict__ fin,
+                                              double* __restrict__ fout, const Geo g,
+                                              const View vi, const View vo, const double omega,
+                                              const double F0, const double F1, const double F2)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+
+    double f[L::Q];
+    pull<L, MODE>(f, fin, x, y, z, g, vi);
+
+    const double F[3] = {F0, F1, F2};
+    double rho, u0, u1, u2;
+    moments<L, FORCED>(f, F, rho, u0, u1, u2);
+    const double usq = usq_of<L>(u0, u1, u2);
+    const double omw = 1 - omega;
+    double pref = 0.0, uF = 0.0;
+    if constexpr (FORCED) {
+        pref = 1.0 - 0.5 * omega;
+        uF = (L::D == 2) ? u0 * F0 + u1 * F1 : u0 * F0 + u1 * F1 + u2 * F2;
+    }
+    const long long self = x + y * vo.sy + z * vo.sz;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const double fe = feq<L, q>(rho, u0, u1, u2, usq);
+        double out = omw * f[q] + omega * fe;
+        if constexpr (FORCED) out = out + guo<L, q>(u0, u1, u2, F0, F1, F2, pref, uF);
+        __builtin_nontemporal_store(out, &fout[q * vo.sq + self]);
+    });
+}
+
+// -------------------------------------------------------- moments kernel --
+template <class L, bool FORCED, int MODE>
+__global__ __launch_bounds__(256) void k_moments(const double* __restrict__ fin,
+                                                 double* __restrict__ rho_out,
+                                                 double* __restrict__ vel_out, const Geo g,
+                                                 const View vi, const View vr, const View vv,
+                                                 const double F0, const double F1,
+                                                 const double F2)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+    double f[L::Q];
+    pull<L, MODE>(f, fin, x, y, z, g, vi);
+    const double F[3] = {F0, F1, F2};
+    double rho, u0, u1, u2;
+    moments<L, FORCED>(f, F, rho, u0, u1, u2);
+    rho_out[x + y * vr.sy + z * vr.sz] = rho;
+    const long long sv = x + y * vv.sy + z * vv.sz;
+    vel_out[sv] = u0;
+    vel_out[sv + vv.sq] = u1;
+    if constexpr (L::D == 3) vel_out[sv + 2 * vv.sq] = u2;
+}
+
+// ----------------------------------------------------------- init kernel --
+// setInitialConditions: src/Simulation/SimulationSetup.cpp:200-219,
+// src/LBM/Macroscopic.cpp:164-181, src/LBM/Equilibrium.cpp:32-67.
+template <class L, bool FORCED>
+__global__ __launch_bounds__(256) void k_init(double* __restrict__ rho_io, double* __restrict__ vel_io,
+                                              double* __restrict__ fout, const Geo g, const View vr,
+                                              const View vv, const View vo, const double F0,
+                                              const double F1, const double F2)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+    const long long cr = x + y * vr.sy + z * vr.sz;
+    const long long cv = x + y * vv.sy + z * vv.sz;
+    const double rho = rho_io[cr];
+    double u0 = vel_io[cv], u1 = vel_io[cv + vv.sq];
+    double u2 = (L::D == 3) ? vel_io[cv + 2 * vv.sq] : 0.0;
+    if constexpr (FORCED) {
+        u0 -= 0.5 * F0 / rho;
+        u1 -= 0.5 * F1 / rho;
+        if constexpr (L::D == 3) u2 -= 0.5 * F2 / rho;
+    }
+    const double usq = usq_of<L>(u0, u1, u2);
+    double f[L::Q];
+    const long long self = x + y * vo.sy + z * vo.sz;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = feq<L, q>(rho, u0, u1, u2, usq);
+        fout[q * vo.sq + self] = f[q];
+    });
+    if constexpr (FORCED) {
+        const double F[3] = {F0, F1, F2};
+        double r2, v0, v1, v2;
+        moments<L, true>(f, F, r2, v0, v1, v2);
+        rho_io[cr] = r2;
+        vel_io[cv] = v0;
+        vel_io[cv + vv.sq] = v1;
+        if constexpr (L::D == 3) vel_io[cv + 2 * vv.sq] = v2;
+    }
+}
+
+// ------------------------------------------------------ ghost-fill kernels --
+// One thread per cell of the grown box; valid cells return at once.
+template <class L>
+__global__ __launch_bounds__(256) void k_fill_periodic(double* __restrict__ f, const Geo g,
+                                                       const View v)
+{
+    const int gx = L::D >= 1, gy = 1, gz = (L::D == 3);
+    const int x = blockIdx.x * blockDim.x + threadIdx.x - gx;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y - gy;
+    const int z = (int)blockIdx.z - gz;
+    if (x >= g.n[0] + gx || y >= g.n[1] + gy) return;
+    const int p[3] = {x, y, z};
+    int s[3];
+    bool out = false, ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        s[d] = p[d];
+        if (p[d] < 0 || p[d] >= g.n[d]) {
+            out = true;
+            if (!g.per[d]) ok = false;
+            s[d] = p[d] < 0 ? p[d] + g.n[d] : p[d] - g.n[d];
+        }
+    }
+    if (!out || !ok) return;
+    const long long dst = x + y * v.sy + z * v.sz;
+    const long long src = s[0] + s[1] * v.sy + s[2] * v.sz;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q * v.sq + dst] = f[q * v.sq + src];
+    });
+}
+
+template <class L>
+__global__ __launch_bounds__(256) void k_apply_bc(double* __restrict__ f, const Geo g, const View v)
+{
+    const int gz = (L::D == 3);
+    const int x = blockIdx.x * blockDim.x + threadIdx.x - 1;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y - 1;
+    const int z = (int)blockIdx.z - gz;
+    if (x >= g.n[0] + 1 || y >= g.n[1] + 1) return;
+    const int p[3] = {x, y, z};
+    int code[3] = {0, 0, 0};
+    bool wall = false;
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+        if (p[d] < 0) code[d] = -1;
+        else if (p[d] >= g.n[d]) code[d] = 1;
+        if (g.per[d]) code[d] = 0;
+        if (code[d]) wall = true;
+    }
+    if (!wall) return;
+    const int idx = (code[0] + 1) + 3 * (code[1] + 1) + (L::D == 3 ? 9 * (code[2] + 1) : 0);
+    if (g.rtype[idx] == MFA_BC_PERIODIC) return;
+    const long long ghost = x + y * v.sy + z * v.sz;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int sx = x + L::c(q, 0), sy = y + L::c(q, 1), sz = z + L::c(q, 2);
+        if (sx < 0 || sx >= g.n[0] || sy < 0 || sy >= g.n[1] || sz < 0 || sz >= g.n[2]) return;
+        const long long src = sx + sy * v.sy + sz * v.sz;
+        f[q * v.sq + ghost] = wall_value<L, q>(f, src, v, g, idx);
+    });
+}
+
+// ------------------------------------------------------------- reductions --
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+constexpr int kRedMaxOut = 27;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Deterministic block sum of NV values per thread into part[blockIdx][k].
+template <int NV>
+__device__ __forceinline__ void block_store(double (&acc)[NV], double* __restrict__ part)
+{
+    __shared__ double sh[kRedThreads / 64][NV > 0 ? NV : 1];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = wave_sum(acc[k]);
+        if (lane == 0) sh[wid][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kRedThreads / 64; ++w) s += sh[w][threadIdx.x];
+        part[blockIdx.x * kRedMaxOut + threadIdx.x] = s;
+    }
+}
+
+template <class L, int WHAT>
+__global__ __launch_bounds__(kRedThreads) void k_reduce_f(const double* __restrict__ f, const Geo g,
+                                                          const View v, int ncomp,
+                                                          double* __restrict__ part)
+{
+    constexpr int NV = (WHAT == MFA_RED_MASS) ? 1 : (WHAT == MFA_RED_MOMENTUM ? 3 : kRedMaxOut);
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(c % g.n[0]);
+        const long long r = c / g.n[0];
+        const int y = (int)(r % g.n[1]);
+        const int z = (int)(r / g.n[1]);
+        const long long self = x + y * v.sy + z * v.sz;
+        if constexpr (WHAT == MFA_RED_COMP_SUM) {
+            for (int k = 0; k < ncomp && k < kRedMaxOut; ++k) acc[k] += f[k * v.sq + self];
+        } else {
+            for_q<L>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const double fq = f[q * v.sq + self];
+                if constexpr (WHAT == MFA_RED_MASS) acc[0] += fq;
+                else {
+                    if constexpr (L::c(q, 0) != 0) acc[0] += L::c(q, 0) * fq;
+                    if constexpr (L::c(q, 1) != 0) acc[1] += L::c(q, 1) * fq;
+                    if constexpr (L::c(q, 2) != 0) acc[2] += L::c(q, 2) * fq;
+                }
+            });
+        }
+    }
+    block_store<NV>(acc, part);
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_residual(const double* __restrict__ a,
+                                                          const double* __restrict__ b, const Geo g,
+                                                          const View va, const View vb, int ncomp,
+                                                          double* __restrict__ part)
+{
+    double acc[2] = {0.0, 0.0};
+    const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(c % g.n[0]);
+        const long long r = c / g.n[0];
+        const int y = (int)(r % g.n[1]);
+        const int z = (int)(r / g.n[1]);
+        for (int k = 0; k < ncomp; ++k) {
+            const double av = a[x + y * va.sy + z * va.sz + k * va.sq];
+            const double bv = b[x + y * vb.sy + z * vb.sz + k * vb.sq];
+            const double d = av - bv;
+            acc[0] += d * d;
+            acc[1] += av * av;
+        }
+    }
+    block_store<2>(acc, part);
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double* __restrict__ part,
+                                                              int nblocks, int nv,
+                                                              double* __restrict__ out)
+{
+    // thread t sums blocks t, t+256, ... in order; then a fixed tree.
+    for (int k = 0; k < nv; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[b * kRedMaxOut + k];
+        __shared__ double sh[kRedThreads / 64];
+        const double w = wave_sum(s);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = w;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int i = 0; i < kRedThreads / 64; ++i) t += sh[i];
+            out[k] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------- helpers ---
+static int lattice_dim(int lat) { return lat == MFA_D2Q9 ? 2 : 3; }
+static int lattice_q(int lat)
+{
+    switch (lat) {
+    case MFA_D2Q9: return 9;
+    case MFA_D3Q15: return 15;
+    case MFA_D3Q19: return 19;
+    case MFA_D3Q27: return 27;
+    default: return -1;
+    }
+}
+
+static int check_field(const mfa_field* f, int ncomp_min, int need_grow, const char* what)
+{
+    if (!f || !f->data) return fail(MFA_ERR_INVALID, std::string(what) + ": null field");
+    if (f->n[0] < 1 || f->n[1] < 1 || f->n[2] < 1)
+        return fail(MFA_ERR_INVALID, std::string(what) + ": empty box");
+    if (f->dtype != MFA_F64)
+        return fail(MFA_ERR_UNSUPPORTED, std::string(what) + ": dtype not supported by this entry point");
+    if (f->ncomp < ncomp_min)
+        return fail(MFA_ERR_INVALID, std::string(what) + ": too few components");
+    if (f->ngrow < need_grow)
+        return fail(MFA_ERR_INVALID, std::string(what) + ": needs a ghost layer");
+    if (f->sy < f->n[0] || f->sz < f->sy * f->n[1] || (f->ncomp > 1 && f->sq < f->sz * f->n[2]))
+        return fail(MFA_ERR_INVALID, std::string(what) + ": strides smaller than the box");
+    return MFA_OK;
+}
+
+static Geo geo_of(const mfa_field& f, const mfa_geom_bc* gb, int lat)
+{
+    Geo g;
+    std::memset(&g, 0, sizeof(g));
+    const int D = lattice_dim(lat);
+    for (int d = 0; d < 3; ++d) {
+        g.n[d] = f.n[d];
+        g.per[d] = (d < D) ? (gb ? gb->periodic[d] != 0 : 1) : 1;
+    }
+    g.walls = 0;
+    for (int d = 0; d < D; ++d)
+        if (!g.per[d]) g.walls = 1;
+    for (int i = 0; i < 27; ++i) {
+        g.rtype[i] = gb ? (signed char)gb->resolved_type[i] : MFA_BC_PERIODIC;
+        g.rrho[i] = gb ? gb->resolved_rho[i] : 1.0;
+        for (int d = 0; d < 3; ++d) g.ru[i][d] = gb ? gb->resolved_u[i][d] : 0.0;
+    }
+    return g;
+}
+
+static void launch_shape(const int n[3], dim3& grid, dim3& block, int grow)
+{
+    const int nx = n[0] + 2 * grow;
+    const int bx = nx <= 64 ? 64 : (nx <= 128 ? 128 : 256);
+    const int by = 256 / bx;
+    block = dim3(bx, by, 1);
+    grid = dim3((nx + bx - 1) / bx, (n[1] + 2 * grow + by - 1) / by, 1);
+}
+
+template <class L>
+static int launch_step(const mfa_field* fin, mfa_field* fout, const Geo& g, double omega,
+                       const double* force, int mode, hipStream_t st)
+{
+    dim3 grid, block;
+    launch_shape(g.n, grid, block, 0);
+    grid.z = g.n[2];
+    const View vi = view_of(*fin), vo = view_of(*fout);
+    const double* a = (const double*)fin->data;
+    double* b = (double*)fout->data;
+    const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0,
+                 F2 = (force && L::D == 3) ? force[2] : 0.0;
+    if (mode == 0) {
+        if (force) hipLaunchKernelGGL((k_step<L, true, 0>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        else hipLaunchKernelGGL((k_step<L, false, 0>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
+    } else {
+        if (force) hipLaunchKernelGGL((k_step<L, true, 1>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        else hipLaunchKernelGGL((k_step<L, false, 1>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
+    }
+    MFA_HIP_CHECK(hipGetLastError());
+    return MFA_OK;
+}
+
+template <class Fn>
+static int with_lattice(int lat, Fn&& fn)
+{
+    switch (lat) {
+    case MFA_D2Q9: return fn(Lattice<MFA_D2Q9>{});
+    case MFA_D3Q15: return fn(Lattice<MFA_D3Q15>{});
+    case MFA_D3Q19: return fn(Lattice<MFA_D3Q19>{});
+    case MFA_D3Q27: return fn(Lattice<MFA_D3Q27>{});
+    default: return fail(MFA_ERR_INVALID, "unknown lattice id");
+    }
+}
+
+static int check_2d(const mfa_field* f, int lat)
+{
+    if (lattice_dim(lat) == 2 && f->n[2] != 1)
+        return fail(MFA_ERR_INVALID, "2-D lattice needs n[2] == 1");
+    return MFA_OK;
+}
+
+// Reduction workspace: one per device, allocated once, kept until exit.
+static std::mutex g_ws_mu;
+static double* g_ws[64] = {nullptr};
+
+static int workspace(double** out)
+{
+    int dev = 0;
+    MFA_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(MFA_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    if (!g_ws[dev]) {
+        void* p = nullptr;
+        MFA_HIP_CHECK(hipMalloc(&p, sizeof(double) * kRedBlocks * kRedMaxOut));
+        g_ws[dev] = (double*)p;
+    }
+    *out = g_ws[dev];
+    return MFA_OK;
+}
+
+}  // namespace mfa
+
+using namespace mfa;
+
+extern "C" {
+
+const char* mfa_version(void) { return "muphfasa-mi355x 0.1 (gfx950)"; }
+const char* mfa_last_error(void) { return g_err.c_str(); }
+
+int mfa_resolve_bc(int lattice, const int face_type[6], const double face_rho[6],
+                   const double face_u[18], mfa_geom_bc* out)
+{
+    // src/LBM/BoundaryConditions.cpp:37-156 and BoundaryConditions.H:82-92
+    if (!out || !face_type) return fail(MFA_ERR_INVALID, "mfa_resolve_bc: null argument");
+    const int D = lattice_q(lattice) < 0 ? -1 : lattice_dim(lattice);
+    if (D < 0) return fail(MFA_ERR_INVALID, "mfa_resolve_bc: unknown lattice");
+    auto rank = [](int t) {
+        switch (t) {
+        case MFA_BC_NO_SLIP: return 3;
+        case MFA_BC_MOVING_WALL: return 2;
+        case MFA_BC_INFLOW: return 1;
+        case MFA_BC_OUTFLOW:
+        case MFA_BC_PRESSURE_OUTLET: return 0;
+        default: return -1;
+        }
+    };
+    out->lattice = lattice;
+    for (int d = 0; d < 3; ++d) out->periodic[d] = (d < D) ? (face_type[2 * d] == MFA_BC_PERIODIC) : 1;
+    for (int i = 0; i < 27; ++i) {
+        out->resolved_type[i] = MFA_BC_PERIODIC;
+        out->resolved_rho[i] = 1.0;
+        out->resolved_u[i][0] = out->resolved_u[i][1] = out->resolved_u[i][2] = 0.0;
+    }
+    for (int f = 0; f < 2 * D; ++f) {
+        if (face_type[f] < 0 || face_type[f] > MFA_BC_PERIODIC)
+            return fail(MFA_ERR_INVALID, "mfa_resolve_bc: unknown boundary type");
+        if ((face_type[f] == MFA_BC_PERIODIC) != (face_type[f ^ 1] == MFA_BC_PERIODIC))
+            return fail(MFA_ERR_INVALID,
+                        "mfa_resolve_bc: a Periodic face needs a Periodic opposite face");
+    }
+    const int ncodes = (D == 2) ? 9 : 27;
+    for (int idx = 0; idx < ncodes; ++idx) {
+        int code[3] = {0, 0, 0}, tmp = idx;
+        bool all_zero = true;
+        for (int d = 0; d < D; ++d) {
+            code[d] = tmp % 3 - 1;
+            tmp /= 3;
+            if (code[d]) all_zero = false;
+        }
+        if (all_zero) continue;
+        int types[3], nf = 0;
+        double rhos[3], us[3][3];
+        for (int d = 0; d < D; ++d) {
+            if (!code[d]) continue;
+            const int fi = 2 * d + (code[d] < 0 ? 0 : 1);
+            if (face_type[fi] == MFA_BC_PERIODIC) continue;
+            types[nf] = face_type[fi];
+            rhos[nf] = face_rho ? face_rho[fi] : 1.0;
+            for (int c = 0; c < 3; ++c) us[nf][c] = face_u ? face_u[3 * fi + c] : 0.0;
+            ++nf;
+        }
+        if (!nf) continue;
+        int hin = 0, hout = 0, hp = 0, nmov = 0;
+        for (int i = 0; i < nf; ++i) {
+            hin |= types[i] == MFA_BC_INFLOW;
+            hout |= types[i] == MFA_BC_OUTFLOW;
+            hp |= types[i] == MFA_BC_PRESSURE_OUTLET;
+            nmov += types[i] == MFA_BC_MOVING_WALL;
+        }
+        if (hin + hout + hp > 1 || nmov > 1)
+            return fail(MFA_ERR_CONFLICT,
+                        "resolveBC: conflicting boundary conditions at same edge/corner");
+        int best = -1;
+        for (int i = 0; i < nf; ++i) best = rank(types[i]) > best ? rank(types[i]) : best;
+        int winner = 0;
+        for (int i = 0; i < nf; ++i)
+            if (rank(types[i]) == best) { winner = types[i]; break; }
+        double sr = 0, su[3] = {0, 0, 0};
+        int cnt = 0;
+        for (int i = 0; i < nf; ++i) {
+            if (types[i] != winner) continue;
+            sr += rhos[i];
+            for (int d = 0; d < D; ++d) su[d] += us[i][d];
+            ++cnt;
+        }
+        out->resolved_type[idx] = winner;
+        out->resolved_rho[idx] = sr / cnt;
+        for (int d = 0; d < D; ++d) out->resolved_u[idx][d] = su[d] / cnt;
+    }
+    return MFA_OK;
+}
+
+int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, double omega,
+             const double* force, void* stream)
+{
+    if (!g) return fail(MFA_ERR_INVALID, "mfa_step: null geometry");
+    const int lat = g->lattice;
+    const int Q = lattice_q(lat);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_step: unknown lattice");
+    int rc;
+    if ((rc = check_field(fin, Q, 0, "mfa_step fin")) || (rc = check_field(fout, Q, 0, "mfa_step fout")))
+        return rc;
+    if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
+        return fail(MFA_ERR_INVALID, "mfa_step: fin/fout boxes differ");
+    if (fin->data == fout->data) return fail(MFA_ERR_INVALID, "mfa_step: fin aliases fout");
+    if ((rc = check_2d(fin, lat))) return rc;
+    const Geo geo = geo_of(*fin, g, lat);
+    return with_lattice(lat, [&](auto L) {
+        return launch_step<decltype(L)>(fin, fout, geo, omega, force, 0, (hipStream_t)stream);
+    });
+}
+
+int mfa_pull_collide(const mfa_field* fin, mfa_field* fout, int lattice, double omega,
+                     const double* force, void* stream)
+{
+    const int Q = lattice_q(lattice);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_pull_collide: unknown lattice");
+    int rc;
+    if ((rc = check_field(fin, Q, 1, "mfa_pull_collide fin")) ||
+        (rc = check_field(fout, Q, 0, "mfa_pull_collide fout")))
+        return rc;
+    if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
+        return fail(MFA_ERR_INVALID, "mfa_pull_collide: fin/fout boxes differ");
+    if ((rc = check_2d(fin, lattice))) return rc;
+    const Geo geo = geo_of(*fin, nullptr, lattice);
+    return with_lattice(lattice, [&](auto L) {
+        return launch_step<decltype(L)>(fin, fout, geo, omega, force, 1, (hipStream_t)stream);
+    });
+}
+
+int mfa_fill_periodic(mfa_field* f, const mfa_geom_bc* g, void* stream)
+{
+    if (!g) return fail(MFA_ERR_INVALID, "mfa_fill_periodic: null geometry");
+    const int Q = lattice_q(g->lattice);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_fill_periodic: unknown lattice");
+    int rc;
+    if ((rc = check_field(f, Q, 1, "mfa_fill_periodic f"))) return rc;
+    if ((rc = check_2d(f, g->lattice))) return rc;
+    const Geo geo = geo_of(*f, g, g->lattice);
+    const View v = view_of(*f);
+    return with_lattice(g->lattice, [&](auto L) {
+        using LT = decltype(L);
+        dim3 grid, block;
+        launch_shape(geo.n, grid, block, 1);
+        grid.z = geo.n[2] + (LT::D == 3 ? 2 : 0);
+        hipLaunchKernelGGL(k_fill_periodic<LT>, grid, block, 0, (hipStream_t)stream, (double*)f->data, geo, v);
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_apply_bc(mfa_field* f, const mfa_geom_bc* g, void* stream)
+{
+    if (!g) return fail(MFA_ERR_INVALID, "mfa_apply_bc: null geometry");
+    const int Q = lattice_q(g->lattice);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_apply_bc: unknown lattice");
+    int rc;
+    if ((rc = check_field(f, Q, 1, "mfa_apply_bc f"))) return rc;
+    if ((rc = check_2d(f, g->lattice))) return rc;
+    const Geo geo = geo_of(*f, g, g->lattice);
+    if (!geo.walls) return MFA_OK;
+    const View v = view_of(*f);
+    return with_lattice(g->lattice, [&](auto L) {
+        using LT = decltype(L);
+        dim3 grid, block;
+        launch_shape(geo.n, grid, block, 1);
+        grid.z = geo.n[2] + (LT::D == 3 ? 2 : 0);
+        hipLaunchKernelGGL(k_apply_bc<LT>, grid, block, 0, (hipStream_t)stream, (double*)f->data, geo, v);
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_moments(const mfa_field* f, mfa_field* rho, mfa_field* vel, const mfa_geom_bc* g,
+                const double* force, int streamed, void* stream)
+{
+    if (!g) return fail(MFA_ERR_INVALID, "mfa_moments: null geometry");
+    const int lat = g->lattice;
+    const int Q = lattice_q(lat);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_moments: unknown lattice");
+    const int D = lattice_dim(lat);
+    int rc;
+    if ((rc = check_field(f, Q, 0, "mfa_moments f")) || (rc = check_field(rho, 1, 0, "mfa_moments rho")) ||
+        (rc = check_field(vel, D, 0, "mfa_moments vel")))
+        return rc;
+    if (std::memcmp(f->n, rho->n, sizeof(f->n)) || std::memcmp(f->n, vel->n, sizeof(f->n)))
+        return fail(MFA_ERR_INVALID, "mfa_moments: boxes differ");
+    if ((rc = check_2d(f, lat))) return rc;
+    const Geo geo = geo_of(*f, g, lat);
+    const View vi = view_of(*f), vr = view_of(*rho), vv = view_of(*vel);
+    const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0,
+                 F2 = (force && D == 3) ? force[2] : 0.0;
+    return with_lattice(lat, [&](auto L) {
+        using LT = decltype(L);
+        dim3 grid, block;
+        launch_shape(geo.n, grid, block, 0);
+        grid.z = geo.n[2];
+        hipStream_t st = (hipStream_t)stream;
+        const double* a = (const double*)f->data;
+        double* r = (double*)rho->data;
+        double* u = (double*)vel->data;
+        if (streamed) {
+            if (force) hipLaunchKernelGGL((k_moments<LT, true, 0>), grid, block, 0, st, a, r, u, geo, vi, vr, vv, F0, F1, F2);
+            else hipLaunchKernelGGL((k_moments<LT, false, 0>), grid, block, 0, st, a, r, u, geo, vi, vr, vv, F0, F1, F2);
+        } else {
+            if (force) hipLaunchKernelGGL((k_moments<LT, true, 2>), grid, block, 0, st, a, r, u, geo, vi, vr, vv, F0, F1, F2);
+            else hipLaunchKernelGGL((k_moments<LT, false, 2>), grid, block, 0, st, a, r, u, geo, vi, vr, vv, F0, F1, F2);
+        }
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_init_equilibrium(mfa_field* rho, mfa_field* vel, mfa_field* f, int lattice,
+                         const double* force, void* stream)
+{
+    const int Q = lattice_q(lattice);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_init_equilibrium: unknown lattice");
+    const int D = lattice_dim(lattice);
+    int rc;
+    if ((rc = check_field(f, Q, 0, "mfa_init_equilibrium f")) ||
+        (rc = check_field(rho, 1, 0, "mfa_init_equilibrium rho")) ||
+        (rc = check_field(vel, D, 0, "mfa_init_equilibrium vel")))
+        return rc;
+    if (std::memcmp(f->n, rho->n, sizeof(f->n)) || std::memcmp(f->n, vel->n, sizeof(f->n)))
+        return fail(MFA_ERR_INVALID, "mfa_init_equilibrium: boxes differ");
+    if ((rc = check_2d(f, lattice))) return rc;
+    const Geo geo = geo_of(*f, nullptr, lattice);
+    const View vr = view_of(*rho), vv = view_of(*vel), vo = view_of(*f);
+    const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0,
+                 F2 = (force && D == 3) ? force[2] : 0.0;
+    return with_lattice(lattice, [&](auto L) {
+        using LT = decltype(L);
+        dim3 grid, block;
+        launch_shape(geo.n, grid, block, 0);
+        grid.z = geo.n[2];
+        hipStream_t st = (hipStream_t)stream;
+        if (force) hipLaunchKernelGGL((k_init<LT, true>), grid, block, 0, st, (double*)rho->data, (double*)vel->data, (double*)f->data, geo, vr, vv, vo, F0, F1, F2);
+        else hipLaunchKernelGGL((k_init<LT, false>), grid, block, 0, st, (double*)rho->data, (double*)vel->data, (double*)f->data, geo, vr, vv, vo, F0, F1, F2);
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void* stream)
+{
+    const int Q = lattice_q(lattice);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_reduce: unknown lattice");
+    if (!out_dev) return fail(MFA_ERR_INVALID, "mfa_reduce: null output");
+    int rc;
+    const int need = (what == MFA_RED_COMP_SUM) ? 1 : Q;
+    if ((rc = check_field(f, need, 0, "mfa_reduce f"))) return rc;
+    if (what == MFA_RED_COMP_SUM && f->ncomp > kRedMaxOut)
+        return fail(MFA_ERR_INVALID, "mfa_reduce: more than 27 components");
+    double* ws = nullptr;
+    if ((rc = workspace(&ws))) return rc;
+    const Geo geo = geo_of(*f, nullptr, lattice);
+    const View v = view_of(*f);
+    hipStream_t st = (hipStream_t)stream;
+    int nv = 0;
+    rc = with_lattice(lattice, [&](auto L) {
+        using LT = decltype(L);
+        if (what == MFA_RED_MASS) {
+            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_MASS>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
+            nv = 1;
+        } else if (what == MFA_RED_MOMENTUM) {
+            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_MOMENTUM>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
+            nv = LT::D;
+        } else if (what == MFA_RED_COMP_SUM) {
+            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_COMP_SUM>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
+            nv = f->ncomp;
+        } else {
+            return fail(MFA_ERR_INVALID, "mfa_reduce: unknown reduction");
+        }
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    });
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, ws, kRedBlocks, nv, out_dev);
+    MFA_HIP_CHECK(hipGetLastError());
+    return MFA_OK;
+}
+
+int mfa_residual(const mfa_field* a, const mfa_field* b, double* out_dev, void* stream)
+{
+    int rc;
+    if ((rc = check_field(a, 1, 0, "mfa_residual a")) || (rc = check_field(b, 1, 0, "mfa_residual b")))
+        return rc;
+    if (std::memcmp(a->n, b->n, sizeof(a->n)) || a->ncomp != b->ncomp)
+        return fail(MFA_ERR_INVALID, "mfa_residual: shapes differ");
+    if (!out_dev) return fail(MFA_ERR_INVALID, "mfa_residual: null output");
+    double* ws = nullptr;
+    if ((rc = workspace(&ws))) return rc;
+    const Geo geo = geo_of(*a, nullptr, MFA_D3Q19);
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_residual, dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)a->data,
+                       (const double*)b->data, geo, view_of(*a), view_of(*b), a->ncomp, ws);
+    MFA_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, ws, kRedBlocks, 2, out_dev);
+    MFA_HIP_CHECK(hipGetLastError());
+    return MFA_OK;
+}
+
+}  // extern "C"

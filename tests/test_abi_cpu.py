@@ -1,0 +1,85 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol the
+header declares, validates arguments before touching the GPU, and its host
+boundary resolution (mfa_resolve_bc) equals the oracle's BCManager
+restatement for every face combination."""
+import ctypes
+import itertools
+import os
+import re
+
+import numpy as np
+import pytest
+
+from muphfasa_amd import abi
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_exports_match_header():
+    text = open(os.path.join(ROOT, "include", "mfa_lbm.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(mfa_\w+)\s*\(", text, re.M))
+    assert declared, "no declarations parsed"
+    L = abi.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert declared == set(abi.EXPORTS)
+
+
+def test_version():
+    assert b"gfx950" in abi.lib().mfa_version()
+
+
+NAMES = ["MovingWall", "NoSlip", "Inflow", "Outflow", "PressureOutlet"]
+
+
+@pytest.mark.parametrize("lattice", ["D2Q9", "D3Q19"])
+def test_resolve_bc_matches_oracle(lattice):
+    lid = abi.LATTICE[lattice]
+    D = abi.LATTICE_DQ[lid][0]
+    rng = np.random.default_rng(5)
+    choices = NAMES + ["Periodic"]
+    combos = 0
+    for trial in range(400):
+        ft = []
+        for d in range(D):
+            if rng.random() < 0.25:
+                ft += [abi.BC["Periodic"]] * 2
+            else:
+                ft += [abi.BC[rng.choice(NAMES)], abi.BC[rng.choice(NAMES)]]
+        fr = rng.uniform(0.9, 1.1, 6)
+        fu = rng.uniform(-0.1, 0.1, (6, 3))
+        try:
+            ref = O.resolve_bc(D, ft, fr, fu)
+        except ValueError:
+            with pytest.raises(abi.MfaError) as e:
+                abi.resolve_bc(lid, ft, fr, fu)
+            assert e.value.code == -4
+            continue
+        g = abi.resolve_bc(lid, ft, fr, fu)
+        assert list(g.resolved_type) == list(ref[0])
+        assert np.array_equal(np.array(g.resolved_rho), ref[1])
+        assert np.array_equal(np.array([list(r) for r in g.resolved_u]), ref[2])
+        assert list(g.periodic)[:D] == [int(ft[2 * d] == abi.BC["Periodic"]) for d in range(D)]
+        combos += 1
+    assert combos > 50
+
+
+def test_invalid_arguments_fail_before_launch():
+    g = abi.resolve_bc(2, [abi.BC["Periodic"]] * 6)
+    f = abi.Field()
+    with pytest.raises(abi.MfaError) as e:
+        abi.check(abi.lib().mfa_step(ctypes.byref(f), ctypes.byref(f), ctypes.byref(g), 1.0,
+                                     None, None))
+    assert e.value.code == -1
+    # a Periodic face without a Periodic opposite is rejected
+    with pytest.raises(abi.MfaError):
+        abi.resolve_bc(2, [abi.BC["Periodic"], abi.BC["NoSlip"]] + [abi.BC["Periodic"]] * 4)
+    # unsupported dtype is reported, not silently run
+    f.data = 16
+    f.n[0] = f.n[1] = f.n[2] = 4
+    f.sy, f.sz, f.sq, f.ncomp, f.dtype = 4, 16, 64, 19, 3
+    with pytest.raises(abi.MfaError) as e:
+        abi.check(abi.lib().mfa_pull_collide(ctypes.byref(f), ctypes.byref(f), 2, 1.0, None,
+                                             None))
+    assert e.value.code in (-1, -2)

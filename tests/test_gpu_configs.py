@@ -1,0 +1,122 @@
+"""Multi-step parity on the benchmark configurations.
+
+* D2Q9 lid cavity 128x128, Re=100 (tau=0.884), 10 000 steps: every
+  population, rho and u identical to the oracle run of the same steps.
+* D3Q19 periodic box with Guo force, seeded noise: identical to the oracle
+  at 32^3 x 200 steps; at the full 256^3 the size-independent invariants are
+  checked instead (FP64 mass conserved, momentum grows by exactly N^3 F per
+  step, to rounding).
+"""
+import numpy as np
+import pytest
+
+from muphfasa_amd import abi
+from oracle import oracle as O
+from tests.gpu_util import bc_tables, scalar_fields, to_dev, to_host, valid_only
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_gpu(g, ta, fa, tb, fb, omega, F, nsteps):
+    for _ in range(nsteps):
+        abi.step(fa, fb, g, omega, F)
+        ta, fa, tb, fb = tb, fb, ta, fa
+    return ta, fa
+
+
+def test_lid_cavity_128_10000_steps(dev):
+    import torch
+    L = O.Lattice("D2Q9")
+    N = 128
+    n = [N, N]
+    u_lid = 0.1
+    nu = 0.128                       # Re=100, u_lat=0.1, dx=1/128 -> tau=0.884
+    tau = 0.5 + nu * 3.0
+    omega = 1.0 / tau
+    g, ref, per = bc_tables(L, "lid")
+    # lid velocity of this config: (0.1, 0)
+    fu = np.zeros((6, 3))
+    fu[3, 0] = u_lid
+    ft = [abi.BC["NoSlip"]] * 3 + [abi.BC["MovingWall"]]
+    g = abi.resolve_bc(0, ft, [1.0] * 6, fu)
+    ref = O.resolve_bc(2, ft, np.ones(6), fu)
+    rho0 = np.ones((1, N, N))
+    vel0 = np.zeros((2, 1, N, N))
+    f0, _, _ = O.init_equilibrium(L, rho0, vel0, n)
+    O.apply_bc(L, f0, n, [0, 0, 0], ref)
+    print("oracle: 10000 steps of 128^2 D2Q9", flush=True)
+    fe = O.run(L, f0, n, omega, None, [0, 0], ref, 10000)
+    print("gpu: 10000 steps", flush=True)
+    ta, fa = valid_only(L, n, dev, f0)
+    tb, fb = valid_only(L, n, dev)
+    ta, fa = _run_gpu(g, ta, fa, tb, fb, omega, None, 10000)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(ta), O.valid(L, fe))
+    rho_e, vel_e = O.moments(L, fe, n)
+    r, fr, v, fv = scalar_fields(L, n, dev)
+    abi.moments(fa, fr, fv, g, None, True)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(r)[0], rho_e)
+    assert np.array_equal(to_host(v), vel_e)
+    assert np.abs(vel_e).max() > 0.01
+
+
+def _periodic_box_ic(n, seed):
+    rng = np.random.default_rng(seed)
+    nn = O.n3(n)
+    rho = np.ones(nn[::-1])
+    vel = rng.uniform(-1e-3, 1e-3, (3,) + tuple(nn[::-1]))
+    return rho, vel
+
+
+def test_periodic_forced_d3q19_32cubed_parity(dev):
+    import torch
+    L = O.Lattice("D3Q19")
+    n = [32, 32, 32]
+    F = [1e-6, 0.0, 0.0]
+    omega = 1.0 / 0.8
+    g, ref, per = bc_tables(L, "periodic")
+    rho, vel = _periodic_box_ic(n, 2024)
+    f0, _, _ = O.init_equilibrium(L, rho, vel, n, F)
+    fe = O.run(L, f0, n, omega, F, [1, 1, 1], ref, 200)
+    r, fr, v, fv = scalar_fields(L, n, dev, rho, vel)
+    ta, fa = valid_only(L, n, dev)
+    tb, fb = valid_only(L, n, dev)
+    abi.init_equilibrium(fr, fv, fa, 2, F)
+    ta, fa = _run_gpu(g, ta, fa, tb, fb, omega, F, 200)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(ta), O.valid(L, fe))
+
+
+def test_periodic_forced_d3q19_256cubed_invariants(dev):
+    import torch
+    N = 256
+    n = [N, N, N]
+    F = [1e-6, 0.0, 0.0]
+    omega = 1.0 / 0.8
+    g = abi.resolve_bc(2, [abi.BC["Periodic"]] * 6)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    rho = torch.ones((1, N, N, N), dtype=torch.float64, device=dev)
+    vel = (torch.rand((3, N, N, N), dtype=torch.float64, device=dev, generator=gen) * 2 - 1) * 1e-3
+    fr, fv = abi.field(rho, n, 0, 3), abi.field(vel, n, 0, 3)
+    ta = torch.empty((19, N, N, N), dtype=torch.float64, device=dev)
+    tb = torch.empty_like(ta)
+    fa, fb = abi.field(ta, n, 0, 3), abi.field(tb, n, 0, 3)
+    abi.init_equilibrium(fr, fv, fa, 2, F)
+    out = torch.zeros(8, dtype=torch.float64, device=dev)
+    abi.reduce(fa, 2, abi.RED["mass"], out.data_ptr())
+    abi.reduce(fa, 2, abi.RED["momentum"], out[1:].data_ptr())
+    torch.cuda.synchronize()
+    m0, p0 = out[0].item(), out[1:4].cpu().numpy().copy()
+    steps = 50
+    ta, fa = _run_gpu(g, ta, fa, tb, fb, omega, F, steps)
+    abi.reduce(fa, 2, abi.RED["mass"], out.data_ptr())
+    abi.reduce(fa, 2, abi.RED["momentum"], out[1:].data_ptr())
+    torch.cuda.synchronize()
+    m1, p1 = out[0].item(), out[1:4].cpu().numpy()
+    assert abs(m1 - m0) / m0 < 1e-13
+    gain = steps * N ** 3 * F[0]
+    assert abs((p1[0] - p0[0]) - gain) / gain < 1e-9
+    assert abs(p1[1] - p0[1]) < 1e-9 and abs(p1[2] - p0[2]) < 1e-9
+    assert torch.isfinite(ta).all().item()

@@ -1,0 +1,172 @@
+"""GPU parity: every C-ABI entry point against the CPU oracle on the same
+seeded inputs.  FP64 results are required to be identical to the oracle
+(np.array_equal; NaN sentinels must sit at the same ghost positions): both
+sides evaluate the reference's expressions in the same order without FMA."""
+import zlib
+
+import numpy as np
+import pytest
+
+from muphfasa_amd import abi
+from oracle import oracle as O
+from tests.gpu_util import (BC_SETS, bc_tables, ghosted, random_state, scalar_fields, to_dev,
+                            to_host, valid_only)
+
+pytestmark = pytest.mark.gpu
+
+LATS = ["D2Q9", "D3Q15", "D3Q19", "D3Q27"]
+SIZES = {2: [[17, 9], [64, 33], [1, 5]], 3: [[13, 7, 6], [5, 9, 11], [64, 3, 2]]}
+
+
+def _shape_ids():
+    out = []
+    for lat in LATS:
+        D = 2 if lat == "D2Q9" else 3
+        for si, n in enumerate(SIZES[D]):
+            for bc in BC_SETS:
+                if n[0] == 1 and bc != "periodic":
+                    continue
+                out.append((lat, si, bc))
+    return out
+
+
+def _oracle_step(L, f, n, per, ref, omega, F):
+    f = f.copy()
+    O.fill_periodic(L, f, n, per)
+    O.apply_bc(L, f, n, per, ref)
+    out = O.new_fab(L, n)
+    O.step(L, f, out, n, omega, F)
+    return f, out
+
+
+@pytest.mark.parametrize("lat,si,bc", _shape_ids())
+@pytest.mark.parametrize("forced", [False, True])
+def test_step_fused_and_halo_match_oracle(dev, lat, si, bc, forced):
+    import torch
+    L = O.Lattice(lat)
+    n = SIZES[L.D][si]
+    print(f"case lattice={lat} n={n} bc={bc} forced={forced}", flush=True)
+    g, ref, per = bc_tables(L, bc)
+    F = [2e-5, -1e-5, 3e-6][: L.D] if forced else None
+    omega = 1.0 / 0.8
+    f0 = random_state(L, n, seed=zlib.crc32(f"{lat}/{si}/{bc}".encode()))
+    filled, expect = _oracle_step(L, f0, n, per, ref, omega, F)
+
+    # fused: valid-only storage, wrap and wall values inline
+    tin, fin = valid_only(L, n, dev, f0)
+    tout, fout = valid_only(L, n, dev)
+    abi.step(fin, fout, g, omega, F)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(tout), O.valid(L, expect))
+
+    # halo: literal FillBoundary + PhysBCFunct + pullCollideFusedMF
+    th, fh = ghosted(L, n, dev, f0)
+    abi.fill_periodic(fh, g)
+    abi.apply_bc(fh, g)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(th), filled, equal_nan=True)
+    to2, fo2 = valid_only(L, n, dev)
+    abi.pull_collide(fh, fo2, abi.LATTICE[lat], omega, F)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(to2), O.valid(L, expect))
+
+
+@pytest.mark.parametrize("lat", LATS)
+@pytest.mark.parametrize("forced", [False, True])
+@pytest.mark.parametrize("streamed", [True, False])
+def test_moments_match_oracle(dev, lat, forced, streamed):
+    import torch
+    L = O.Lattice(lat)
+    n = SIZES[L.D][0]
+    g, ref, per = bc_tables(L, "lid")
+    F = [1e-4, 2e-5, -3e-5][: L.D] if forced else None
+    f0 = random_state(L, n, seed=11)
+    filled = f0.copy()
+    O.fill_periodic(L, filled, n, per)
+    O.apply_bc(L, filled, n, per, ref)
+    rho_e, vel_e = O.moments(L, filled, n, F, streamed=streamed)
+    tin, fin = valid_only(L, n, dev, f0)
+    r, fr, v, fv = scalar_fields(L, n, dev)
+    abi.moments(fin, fr, fv, g, F, streamed)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(r)[0], rho_e)
+    assert np.array_equal(to_host(v), vel_e)
+
+
+@pytest.mark.parametrize("lat", LATS)
+@pytest.mark.parametrize("forced", [False, True])
+def test_init_equilibrium_matches_oracle(dev, lat, forced):
+    import torch
+    L = O.Lattice(lat)
+    n = SIZES[L.D][1]
+    nn = O.n3(n)
+    rng = np.random.default_rng(21)
+    rho = 1.0 + 0.02 * rng.standard_normal(nn[::-1])
+    vel = 0.03 * rng.standard_normal((L.D,) + tuple(nn[::-1]))
+    F = [1e-3, -2e-4, 5e-4][: L.D] if forced else None
+    f_e, rho_e, vel_e = O.init_equilibrium(L, rho, vel, n, F)
+    r, fr, v, fv = scalar_fields(L, n, dev, rho, vel)
+    tf, ff = valid_only(L, n, dev)
+    abi.init_equilibrium(fr, fv, ff, abi.LATTICE[lat], F)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(tf), O.valid(L, f_e))
+    assert np.array_equal(to_host(r)[0], rho_e)
+    assert np.array_equal(to_host(v), vel_e)
+
+
+@pytest.mark.parametrize("lat", LATS)
+def test_reductions(dev, lat):
+    import math
+    import torch
+    L = O.Lattice(lat)
+    n = [37, 21, 9] if L.D == 3 else [129, 77]
+    f0 = random_state(L, n, seed=5)
+    tin, fin = valid_only(L, n, dev, f0)
+    out = torch.zeros(32, dtype=torch.float64, device=dev)
+    v = O.valid(L, f0)
+    abi.reduce(fin, abi.LATTICE[lat], abi.RED["mass"], out.data_ptr())
+    torch.cuda.synchronize()
+    exact = math.fsum(v.ravel())
+    assert abs(out[0].item() - exact) <= 1e-13 * abs(exact)
+    abi.reduce(fin, abi.LATTICE[lat], abi.RED["momentum"], out.data_ptr())
+    torch.cuda.synchronize()
+    for d in range(L.D):
+        exact = math.fsum((L.c[:, d][:, None, None, None] * v).ravel())
+        assert abs(out[d].item() - exact) <= 1e-13 * math.fsum(np.abs(v).ravel())
+    abi.reduce(fin, abi.LATTICE[lat], abi.RED["comp_sum"], out.data_ptr())
+    torch.cuda.synchronize()
+    for q in range(L.Q):
+        exact = math.fsum(v[q].ravel())
+        assert abs(out[q].item() - exact) <= 1e-13 * abs(exact)
+    # deterministic: same input, same bits
+    abi.reduce(fin, abi.LATTICE[lat], abi.RED["mass"], out.data_ptr())
+    a = out[0].item()
+    abi.reduce(fin, abi.LATTICE[lat], abi.RED["mass"], out.data_ptr())
+    torch.cuda.synchronize()
+    assert out[0].item() == a
+    # residual
+    t2 = tin.clone()
+    t2[0, 0, 0, 0] += 0.5
+    f2 = abi.field(t2, n, 0, L.D)
+    abi.residual(f2, fin, out.data_ptr())
+    torch.cuda.synchronize()
+    assert abs(out[0].item() - 0.25) < 1e-15
+    assert abs(out[1].item() - math.fsum((to_host(t2) ** 2).ravel())) < 1e-12 * out[1].item()
+
+
+def test_empty_and_mismatched_boxes_are_errors(dev):
+    import torch
+    L = O.Lattice("D3Q19")
+    g, ref, per = bc_tables(L, "periodic")
+    a = torch.zeros((19, 4, 4, 4), dtype=torch.float64, device=dev)
+    b = torch.zeros((19, 4, 4, 5), dtype=torch.float64, device=dev)
+    fa = abi.field(a, [4, 4, 4], 0, 3)
+    fb = abi.field(b, [5, 4, 4], 0, 3)
+    with pytest.raises(abi.MfaError):
+        abi.step(fa, fb, g, 1.0)
+    with pytest.raises(abi.MfaError):
+        abi.step(fa, fa, g, 1.0)         # in-place is rejected (pull needs two lattices)
+    fz = abi.field(a, [4, 4, 4], 0, 3)
+    fz.n[0] = 0
+    with pytest.raises(abi.MfaError):
+        abi.step(fz, fz, g, 1.0)
