@@ -142,24 +142,13 @@ def main():
     achieved = N3 * bytes_per_lu / (ms_per_step * 1e-3) / 1e9
     traffic = traffic_from_profiles(n)
 
+    copy = None
     if args.copy_probe:
-        src = torch.empty(N3 * 19, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        src.fill_(1.0)
-        for _ in range(3):
-            dst.copy_(src)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        reps = 20
-        for _ in range(reps):
-            dst.copy_(src)
-        e1.record()
-        torch.cuda.synchronize()
-        copy_gbs = 2 * src.numel() * 8 / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
-        print(json.dumps({"copy_probe_bytes_per_launch": 2 * src.numel() * 8,
-                          "copy_probe_GBs": round(copy_gbs, 1)}), file=sys.stderr, flush=True)
-        del src, dst
+        from muphfasa_amd import autotune
+        del out
+        copy = autotune.copy_ceiling(N3 * 19 * 8)
+        print(json.dumps({"copy_probe_bytes_per_launch": 2 * N3 * 19 * 8,
+                          "copy_probe_GBs": copy}), file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -191,6 +180,8 @@ def main():
                      "kernel": "k_step<D3Q19,forced,fused>",
                      "algorithmic_bytes_per_launch": N3 * bytes_per_lu},
         "mass_rel_drift": abs(mass1 - mass0) / mass0,
+        "copy_ceiling_GBs": copy,
+        "launch": abi.get_launch(lattice),
         "wall_s_timed": round(wall, 4),
         "cpu_baseline": cpu,
     }

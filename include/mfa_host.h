@@ -1,0 +1,51 @@
+/* mfa_host.h -- C ABI of the C++ host (the solver's simulation API).
+ *
+ * Wraps ISimulation / makeSimulation (src/Simulation/ISimulation.H:16-44,
+ * src/Simulation/SimulationFactory.cpp:36-60 of the reference) and the run
+ * loop of src/main.cpp:14-38 for callers that are not C++ (Python drivers,
+ * the test-suite).  Functions return 0 on success or a negative code; the
+ * message is available from mfa_sim_error() (thread-local).  Every call on one
+ * mfa_sim must come from one thread at a time.
+ */
+#ifndef MFA_HOST_H
+#define MFA_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mfa_sim mfa_sim;
+
+const char* mfa_sim_error(void);
+
+/* inputs_text: the text of an inputs file; overrides: "key=value" strings
+ * applied after it (like command-line arguments).  NULL on error. */
+mfa_sim* mfa_sim_create(const char* inputs_text, int noverrides, const char* const* overrides);
+void mfa_sim_destroy(mfa_sim* s);
+
+int mfa_sim_initialize(mfa_sim* s);                 /* ISimulation::initialize */
+int mfa_sim_advance(mfa_sim* s, int nsteps);        /* nsteps x ISimulation::advance */
+int mfa_sim_update_macroscopic(mfa_sim* s, int step);
+int mfa_sim_write_output(mfa_sim* s, int step);
+int mfa_sim_sync(mfa_sim* s);
+
+/* lattice id, cells n[3], omega, lattice force F[3], time, start step */
+int mfa_sim_info(mfa_sim* s, int* lattice, int n[3], double* omega, double force[3],
+                 double* time, int* start_step);
+/* host copies of the valid cells, component-major [comp][z][y][x] */
+int mfa_sim_copy_f(mfa_sim* s, double* f);
+int mfa_sim_copy_macro(mfa_sim* s, double* rho, double* vel);
+
+/* Host-only (no GPU): parse inputs and run the UnitConverter chain.
+ * out[0..11] = dx, dt, u_lat, nu_lat, tau, omega, Re, nu_phys, Ma,
+ *              F_lat[0], F_lat[1], F_lat[2]. */
+int mfa_inputs_units(const char* inputs_text, int noverrides, const char* const* overrides,
+                     double out[12]);
+
+/* src/main.cpp: argv[1] = inputs file, then key=value overrides. */
+int mfa_run_main(int argc, const char* const* argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -75,6 +75,16 @@ typedef struct mfa_geom_bc {
     double resolved_u[27][3];
 } mfa_geom_bc;
 
+/* Launch configuration of mfa_step's kernel (per lattice, process-wide):
+ * block_x in {64,128,256} cells along x, block_y rows per block
+ * (block_x*block_y <= 256), cells_per_thread in {1,2} along z (3-D only),
+ * nt_loads: non-temporal population loads.  Chosen by the autotuner
+ * (muphfasa_amd/autotune.py); no counterpart in the reference, whose launch
+ * shapes are AMReX's. */
+typedef struct mfa_launch {
+    int block_x, block_y, cells_per_thread, nt_loads;
+} mfa_launch;
+
 /* Library version and last error message (thread-local). */
 const char* mfa_version(void);
 const char* mfa_last_error(void);
@@ -97,6 +107,11 @@ int mfa_resolve_bc(int lattice, const int face_type[6], const double face_rho[6]
  * src/LBM/Collision.cpp:95-143) on the valid cells.  force: D doubles. */
 int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g,
              double omega, const double* force, void* stream);
+
+/* Set (cfg != NULL) or reset to the built-in default (cfg == NULL) the
+ * launch configuration mfa_step uses for one lattice; query it. */
+int mfa_set_launch(int lattice, const mfa_launch* cfg);
+int mfa_get_launch(int lattice, mfa_launch* cfg);
 
 /* Literal pullCollideFusedMF (src/LBM/Collision.cpp:95-143): fin must have
  * ngrow >= 1 with ghosts filled (mfa_fill_periodic + mfa_apply_bc). */
@@ -137,6 +152,12 @@ int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void*
 /* Steady-state residual: out_dev[0] = sum (a-b)^2, out_dev[1] = sum a^2 over
  * the valid cells of all components of two same-shaped F64 fields. */
 int mfa_residual(const mfa_field* a, const mfa_field* b, double* out_dev, void* stream);
+
+/* Measurement utility (no reference counterpart): a streaming device copy of
+ * `bytes` (multiple of 16) with 16-byte loads/stores, one per thread.  Its
+ * bandwidth is the measured HBM ceiling the step kernel is compared with,
+ * and its known size calibrates the FETCH_SIZE / WRITE_SIZE counters. */
+int mfa_stream_copy(const void* src, void* dst, size_t bytes, int nontemporal, void* stream);
 
 #ifdef __cplusplus
 }

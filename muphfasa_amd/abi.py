@@ -22,7 +22,7 @@ RED = {"mass": 0, "momentum": 1, "comp_sum": 2}
 
 EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_pull_collide",
            "mfa_fill_periodic", "mfa_apply_bc", "mfa_moments", "mfa_init_equilibrium",
-           "mfa_reduce", "mfa_residual"]
+           "mfa_reduce", "mfa_residual", "mfa_set_launch", "mfa_get_launch", "mfa_stream_copy"]
 
 
 class MfaError(RuntimeError):
@@ -35,6 +35,11 @@ class Field(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("n", ctypes.c_int * 3), ("ngrow", ctypes.c_int),
                 ("sy", ctypes.c_longlong), ("sz", ctypes.c_longlong), ("sq", ctypes.c_longlong),
                 ("ncomp", ctypes.c_int), ("dtype", ctypes.c_int)]
+
+
+class Launch(ctypes.Structure):
+    _fields_ = [("block_x", ctypes.c_int), ("block_y", ctypes.c_int),
+                ("cells_per_thread", ctypes.c_int), ("nt_loads", ctypes.c_int)]
 
 
 class GeomBC(ctypes.Structure):
@@ -70,6 +75,9 @@ def lib():
         L.mfa_init_equilibrium.argtypes = [F, F, F, ctypes.c_int, dp, vp]
         L.mfa_reduce.argtypes = [F, ctypes.c_int, ctypes.c_int, vp, vp]
         L.mfa_residual.argtypes = [F, F, vp, vp]
+        L.mfa_set_launch.argtypes = [ctypes.c_int, ctypes.POINTER(Launch)]
+        L.mfa_get_launch.argtypes = [ctypes.c_int, ctypes.POINTER(Launch)]
+        L.mfa_stream_copy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
         for name in EXPORTS[2:]:
             getattr(L, name).restype = ctypes.c_int
         _lib = L
@@ -172,3 +180,23 @@ def reduce(f: Field, lattice: int, what: int, out_ptr: int, stream=None):
 def residual(a: Field, b: Field, out_ptr: int, stream=None):
     check(lib().mfa_residual(ctypes.byref(a), ctypes.byref(b), ctypes.c_void_p(out_ptr),
                              stream_ptr(stream)))
+
+
+def set_launch(lattice: int, block_x=None, block_y=1, cells_per_thread=1, nt_loads=0):
+    """Set mfa_step's launch configuration for a lattice (block_x None resets)."""
+    if block_x is None:
+        check(lib().mfa_set_launch(lattice, None))
+        return
+    c = Launch(block_x, block_y, cells_per_thread, nt_loads)
+    check(lib().mfa_set_launch(lattice, ctypes.byref(c)))
+
+
+def get_launch(lattice: int):
+    c = Launch()
+    check(lib().mfa_get_launch(lattice, ctypes.byref(c)))
+    return (c.block_x, c.block_y, c.cells_per_thread, c.nt_loads)
+
+
+def stream_copy(src_ptr: int, dst_ptr: int, nbytes: int, nontemporal=True, stream=None):
+    check(lib().mfa_stream_copy(ctypes.c_void_p(src_ptr), ctypes.c_void_p(dst_ptr), nbytes,
+                                1 if nontemporal else 0, stream_ptr(stream)))

@@ -158,7 +158,16 @@ __device__ __forceinline__ void for_q(Fn&& fn) { Unroll<L, 0>::run(fn); }
 
 // Pull the Q populations of cell (x,y,z).  mode: 0 fused (inline wrap and
 // walls), 1 halo (reads ghost cells), 2 unstreamed (the cell's own values).
-template <class L, int MODE>
+// WALLS: some dimension is non-periodic (compile-time, so the periodic
+// kernel carries no boundary code).  NT: non-temporal loads.
+template <bool NT>
+__device__ __forceinline__ double ld(const double* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <class L, int MODE, bool WALLS = true, bool NT = false>
 __device__ __forceinline__ void pull(double (&f)[L::Q], const double* __restrict__ fin, int x, int y,
                                      int z, const Geo& g, const View& v)
 {
@@ -166,7 +175,7 @@ __device__ __forceinline__ void pull(double (&f)[L::Q], const double* __restrict
     if constexpr (MODE == 2) {
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            f[q] = fin[q * v.sq + self];
+            f[q] = ld<NT>(&fin[q * v.sq + self]);
         });
         return;
     } else {
@@ -175,25 +184,23 @@ __device__ __forceinline__ void pull(double (&f)[L::Q], const double* __restrict
             constexpr int q = decltype(qc)::value;
             const long long off = s.o[0][L::c(q, 0) + 1] + s.o[1][L::c(q, 1) + 1] +
                                   s.o[2][L::c(q, 2) + 1];
-            f[q] = fin[q * v.sq + off];
+            f[q] = ld<NT>(&fin[q * v.sq + off]);
         });
-        if constexpr (MODE == 0) {
-            if (g.walls) {
-                const int p[3] = {x, y, z};
-                int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-                bool edge = false;
+        if constexpr (MODE == 0 && WALLS) {
+            const int p[3] = {x, y, z};
+            int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            bool edge = false;
 #pragma unroll
-                for (int d = 0; d < L::D; ++d) {
-                    lo[d] = !g.per[d] && p[d] == 0;
-                    hi[d] = !g.per[d] && p[d] == g.n[d] - 1;
-                    edge = edge || lo[d] || hi[d];
-                }
-                if (edge) {
-                    for_q<L>([&](auto qc) {
-                        constexpr int q = decltype(qc)::value;
-                        fix_wall<L, q>(f[q], fin, self, v, g, lo, hi);
-                    });
-                }
+            for (int d = 0; d < L::D; ++d) {
+                lo[d] = !g.per[d] && p[d] == 0;
+                hi[d] = !g.per[d] && p[d] == g.n[d] - 1;
+                edge = edge || lo[d] || hi[d];
+            }
+            if (edge) {
+                for_q<L>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    fix_wall<L, q>(f[q], fin, self, v, g, lo, hi);
+                });
             }
         }
     }
@@ -228,21 +235,13 @@ __device__ __forceinline__ void moments(const double (&f)[L::Q], const double F[
 }
 
 // ----------------------------------------------------------- step kernel --
-// One thread per cell; block = (BX, 256/BX); grid.z = z.
-template <class L, bool FORCED, int MODE>
-__global__ __launch_bounds__(256) void k_step(const double* __restrict__ fin,
-                                              double* __restrict__ fout, const Geo g,
-                                              const View vi, const View vo, const double omega,
+// BGK (+Guo) collision of pulled populations and store:
+// src/LBM/Collision.cpp:24-53 (plain) and :62-92 (forced).
+template <class L, bool FORCED>
+__device__ __forceinline__ void collide_store(const double (&f)[L::Q], double* __restrict__ fout,
+                                              long long self, const View& vo, const double omega,
                                               const double F0, const double F1, const double F2)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z;
-    if (x >= g.n[0] || y >= g.n[1]) return;
-
-    double f[L::Q];
-    pull<L, MODE>(f, fin, x, y, z, g, vi);
-
     const double F[3] = {F0, F1, F2};
     double rho, u0, u1, u2;
     moments<L, FORCED>(f, F, rho, u0, u1, u2);
@@ -253,7 +252,6 @@ __global__ __launch_bounds__(256) void k_step(const double* __restrict__ fin,
         pref = 1.0 - 0.5 * omega;
         uF = (L::D == 2) ? u0 * F0 + u1 * F1 : u0 * F0 + u1 * F1 + u2 * F2;
     }
-    const long long self = x + y * vo.sy + z * vo.sz;
     for_q<L>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const double fe = feq<L, q>(rho, u0, u1, u2, usq);
@@ -261,6 +259,31 @@ __global__ __launch_bounds__(256) void k_step(const double* __restrict__ fin,
         if constexpr (FORCED) out = out + guo<L, q>(u0, u1, u2, F0, F1, F2, pref, uF);
         __builtin_nontemporal_store(out, &fout[q * vo.sq + self]);
     });
+}
+
+// One thread per cell column of NC cells along z; block = (bx, by);
+// grid = (ceil(nx/bx), ceil(ny/by), ceil(nz/NC)).  All NC*Q loads of a
+// thread are issued before the first collision so they are in flight
+// together.
+template <class L, bool FORCED, int MODE, bool WALLS, int NC, bool NT>
+__global__ __launch_bounds__(256) void k_step(const double* __restrict__ fin,
+                                              double* __restrict__ fout, const Geo g,
+                                              const View vi, const View vo, const double omega,
+                                              const double F0, const double F1, const double F2)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z0 = blockIdx.z * NC;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+    double f[NC][L::Q];
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+        if (NC == 1 || z0 + k < g.n[2]) pull<L, MODE, WALLS, NT>(f[k], fin, x, y, z0 + k, g, vi);
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+        if (NC == 1 || z0 + k < g.n[2])
+            collide_store<L, FORCED>(f[k], fout, x + y * vo.sy + (z0 + k) * vo.sz, vo, omega, F0,
+                                     F1, F2);
 }
 
 // -------------------------------------------------------- moments kernel --
@@ -505,6 +528,23 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double* __re
     }
 }
 
+// ----------------------------------------------------- copy-ceiling probe --
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__global__ __launch_bounds__(256) void k_copy(const dbl2* __restrict__ a, dbl2* __restrict__ b,
+                                              long long n2)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
+    if constexpr (NT) {
+        const dbl2 v = __builtin_nontemporal_load(&a[i]);
+        __builtin_nontemporal_store(v, &b[i]);
+    } else {
+        b[i] = a[i];
+    }
+}
+
 // -------------------------------------------------------------- helpers ---
 static int lattice_dim(int lat) { return lat == MFA_D2Q9 ? 2 : 3; }
 static int lattice_q(int lat)
@@ -563,24 +603,74 @@ static void launch_shape(const int n[3], dim3& grid, dim3& block, int grow)
     grid = dim3((nx + bx - 1) / bx, (n[1] + 2 * grow + by - 1) / by, 1);
 }
 
+// Launch configuration of the fused step kernel, per lattice (set by the
+// autotuner through mfa_set_launch; defaults measured on MI355X).
+static std::mutex g_cfg_mu;
+static mfa_launch g_cfg[4] = {
+    {256, 1, 1, 0}, {256, 1, 1, 0}, {256, 1, 1, 0}, {256, 1, 1, 0}};
+
+static bool valid_cfg(const mfa_launch& c)
+{
+    const bool bx_ok = c.block_x == 64 || c.block_x == 128 || c.block_x == 256;
+    return bx_ok && c.block_y >= 1 && c.block_x * c.block_y <= 256 && (c.cells_per_thread == 1 ||
+           c.cells_per_thread == 2) && (c.nt_loads == 0 || c.nt_loads == 1);
+}
+
+template <class L, bool FORCED, bool WALLS, int NC, bool NT>
+static void launch_fused(dim3 grid, dim3 block, hipStream_t st, const double* a, double* b,
+                         const Geo& g, const View& vi, const View& vo, double omega, double F0,
+                         double F1, double F2)
+{
+    hipLaunchKernelGGL((k_step<L, FORCED, 0, WALLS, NC, NT>), grid, block, 0, st, a, b, g, vi, vo,
+                       omega, F0, F1, F2);
+}
+
+template <class L, bool FORCED, bool WALLS>
+static void launch_fused_cfg(const mfa_launch& c, dim3 grid, dim3 block, hipStream_t st,
+                             const double* a, double* b, const Geo& g, const View& vi,
+                             const View& vo, double omega, double F0, double F1, double F2)
+{
+    if (c.cells_per_thread == 2) {
+        if (c.nt_loads) launch_fused<L, FORCED, WALLS, 2, true>(grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        else launch_fused<L, FORCED, WALLS, 2, false>(grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+    } else {
+        if (c.nt_loads) launch_fused<L, FORCED, WALLS, 1, true>(grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        else launch_fused<L, FORCED, WALLS, 1, false>(grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+    }
+}
+
 template <class L>
 static int launch_step(const mfa_field* fin, mfa_field* fout, const Geo& g, double omega,
-                       const double* force, int mode, hipStream_t st)
+                       const double* force, int mode, hipStream_t st, int lat)
 {
-    dim3 grid, block;
-    launch_shape(g.n, grid, block, 0);
-    grid.z = g.n[2];
     const View vi = view_of(*fin), vo = view_of(*fout);
     const double* a = (const double*)fin->data;
     double* b = (double*)fout->data;
     const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0,
                  F2 = (force && L::D == 3) ? force[2] : 0.0;
     if (mode == 0) {
-        if (force) hipLaunchKernelGGL((k_step<L, true, 0>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
-        else hipLaunchKernelGGL((k_step<L, false, 0>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        mfa_launch c;
+        {
+            std::lock_guard<std::mutex> lk(g_cfg_mu);
+            c = g_cfg[lat];
+        }
+        if (L::D == 2) c.cells_per_thread = 1;
+        const dim3 block(c.block_x, c.block_y, 1);
+        const dim3 grid((g.n[0] + c.block_x - 1) / c.block_x, (g.n[1] + c.block_y - 1) / c.block_y,
+                        (g.n[2] + c.cells_per_thread - 1) / c.cells_per_thread);
+        if (force) {
+            if (g.walls) launch_fused_cfg<L, true, true>(c, grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+            else launch_fused_cfg<L, true, false>(c, grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        } else {
+            if (g.walls) launch_fused_cfg<L, false, true>(c, grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+            else launch_fused_cfg<L, false, false>(c, grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        }
     } else {
-        if (force) hipLaunchKernelGGL((k_step<L, true, 1>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
-        else hipLaunchKernelGGL((k_step<L, false, 1>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        dim3 grid, block;
+        launch_shape(g.n, grid, block, 0);
+        grid.z = g.n[2];
+        if (force) hipLaunchKernelGGL((k_step<L, true, 1, false, 1, false>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        else hipLaunchKernelGGL((k_step<L, false, 1, false, 1, false>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
     }
     MFA_HIP_CHECK(hipGetLastError());
     return MFA_OK;
@@ -651,6 +741,7 @@ int mfa_resolve_bc(int lattice, const int face_type[6], const double face_rho[6]
         }
     };
     out->lattice = lattice;
+    // periodicity is taken from the lo face (src/Geometry/GeometryHandler.cpp:40-47)
     for (int d = 0; d < 3; ++d) out->periodic[d] = (d < D) ? (face_type[2 * d] == MFA_BC_PERIODIC) : 1;
     for (int i = 0; i < 27; ++i) {
         out->resolved_type[i] = MFA_BC_PERIODIC;
@@ -660,9 +751,10 @@ int mfa_resolve_bc(int lattice, const int face_type[6], const double face_rho[6]
     for (int f = 0; f < 2 * D; ++f) {
         if (face_type[f] < 0 || face_type[f] > MFA_BC_PERIODIC)
             return fail(MFA_ERR_INVALID, "mfa_resolve_bc: unknown boundary type");
-        if ((face_type[f] == MFA_BC_PERIODIC) != (face_type[f ^ 1] == MFA_BC_PERIODIC))
+        if ((f & 1) && face_type[f] == MFA_BC_PERIODIC && face_type[f ^ 1] != MFA_BC_PERIODIC)
             return fail(MFA_ERR_INVALID,
-                        "mfa_resolve_bc: a Periodic face needs a Periodic opposite face");
+                        "buildBCManager: bc input requests \"Periodic\" on a face whose Geometry is "
+                        "not periodic in that direction");
     }
     const int ncodes = (D == 2) ? 9 : 27;
     for (int idx = 0; idx < ncodes; ++idx) {
@@ -732,8 +824,40 @@ int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, double
     if ((rc = check_2d(fin, lat))) return rc;
     const Geo geo = geo_of(*fin, g, lat);
     return with_lattice(lat, [&](auto L) {
-        return launch_step<decltype(L)>(fin, fout, geo, omega, force, 0, (hipStream_t)stream);
+        return launch_step<decltype(L)>(fin, fout, geo, omega, force, 0, (hipStream_t)stream, lat);
     });
+}
+
+int mfa_stream_copy(const void* src, void* dst, size_t bytes, int nontemporal, void* stream)
+{
+    if (!src || !dst || bytes % 16) return fail(MFA_ERR_INVALID, "mfa_stream_copy: needs 16-byte multiples");
+    const long long n2 = (long long)(bytes / 16);
+    const dim3 grid((unsigned)((n2 + 255) / 256));
+    if (nontemporal)
+        hipLaunchKernelGGL(k_copy<true>, grid, dim3(256), 0, (hipStream_t)stream, (const dbl2*)src, (dbl2*)dst, n2);
+    else
+        hipLaunchKernelGGL(k_copy<false>, grid, dim3(256), 0, (hipStream_t)stream, (const dbl2*)src, (dbl2*)dst, n2);
+    MFA_HIP_CHECK(hipGetLastError());
+    return MFA_OK;
+}
+
+int mfa_set_launch(int lattice, const mfa_launch* cfg)
+{
+    if (lattice_q(lattice) < 0) return fail(MFA_ERR_INVALID, "mfa_set_launch: unknown lattice");
+    const mfa_launch def = {256, 1, 1, 0};
+    const mfa_launch c = cfg ? *cfg : def;
+    if (!valid_cfg(c)) return fail(MFA_ERR_INVALID, "mfa_set_launch: unsupported launch configuration");
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    g_cfg[lattice] = c;
+    return MFA_OK;
+}
+
+int mfa_get_launch(int lattice, mfa_launch* cfg)
+{
+    if (lattice_q(lattice) < 0 || !cfg) return fail(MFA_ERR_INVALID, "mfa_get_launch: bad argument");
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    *cfg = g_cfg[lattice];
+    return MFA_OK;
 }
 
 int mfa_pull_collide(const mfa_field* fin, mfa_field* fout, int lattice, double omega,
@@ -750,7 +874,7 @@ int mfa_pull_collide(const mfa_field* fin, mfa_field* fout, int lattice, double 
     if ((rc = check_2d(fin, lattice))) return rc;
     const Geo geo = geo_of(*fin, nullptr, lattice);
     return with_lattice(lattice, [&](auto L) {
-        return launch_step<decltype(L)>(fin, fout, geo, omega, force, 1, (hipStream_t)stream);
+        return launch_step<decltype(L)>(fin, fout, geo, omega, force, 1, (hipStream_t)stream, lattice);
     });
 }
 

@@ -72,9 +72,12 @@ def test_invalid_arguments_fail_before_launch():
         abi.check(abi.lib().mfa_step(ctypes.byref(f), ctypes.byref(f), ctypes.byref(g), 1.0,
                                      None, None))
     assert e.value.code == -1
-    # a Periodic face without a Periodic opposite is rejected
+    # a Periodic hi face on a non-periodic direction (lo face not Periodic) is rejected
     with pytest.raises(abi.MfaError):
-        abi.resolve_bc(2, [abi.BC["Periodic"], abi.BC["NoSlip"]] + [abi.BC["Periodic"]] * 4)
+        abi.resolve_bc(2, [abi.BC["NoSlip"], abi.BC["Periodic"]] + [abi.BC["Periodic"]] * 4)
+    # a Periodic lo face makes the direction periodic whatever the hi face says
+    g = abi.resolve_bc(2, [abi.BC["Periodic"], abi.BC["NoSlip"]] + [abi.BC["Periodic"]] * 4)
+    assert list(g.periodic) == [1, 1, 1]
     # unsupported dtype is reported, not silently run
     f.data = 16
     f.n[0] = f.n[1] = f.n[2] = 4

@@ -78,6 +78,7 @@ def test_periodic_forced_d3q19_32cubed_parity(dev):
     g, ref, per = bc_tables(L, "periodic")
     rho, vel = _periodic_box_ic(n, 2024)
     f0, _, _ = O.init_equilibrium(L, rho, vel, n, F)
+    O.fill_periodic(L, f0, n, [1, 1, 1])
     fe = O.run(L, f0, n, omega, F, [1, 1, 1], ref, 200)
     r, fr, v, fv = scalar_fields(L, n, dev, rho, vel)
     ta, fa = valid_only(L, n, dev)

@@ -1,0 +1,140 @@
+"""The C++ host (ISimulation / makeSimulation / run loop / plotfile /
+checkpoint) driving the gfx950 kernels, against the oracle and the
+reference's regression criteria."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from muphfasa_amd import host
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INP = os.path.join(ROOT, "tests", "golden", "inputs")
+
+
+def _read(name):
+    return open(os.path.join(INP, name)).read()
+
+
+def _oracle_fab(L, n, valid):
+    f = O.new_fab(L, n)
+    O.valid(L, f)[...] = valid
+    return f
+
+
+def test_lid_cavity_128_through_host_matches_oracle():
+    """Configuration 1: D2Q9 lid cavity 128^2, Re=100, 10 000 steps."""
+    sim = host.Simulation(_read("lid_cavity_128"))
+    sim.initialize()
+    f0 = sim.f()
+    print("host: 10000 steps", flush=True)
+    sim.advance(10000)
+    sim.update_macroscopic(9999)
+    got = sim.f()
+    rho_g, vel_g = sim.macro()
+    L = O.Lattice("D2Q9")
+    n = [128, 128]
+    fu = np.zeros((6, 3))
+    fu[3, 0] = 0.1
+    res = O.resolve_bc(2, [1, 1, 1, 0], np.ones(6), fu)
+    fab = _oracle_fab(L, n, f0)
+    O.apply_bc(L, fab, n, [0, 0, 0], res)
+    print("oracle: 10000 steps", flush=True)
+    fe = O.run(L, fab, n, sim.omega, None, [0, 0], res, 10000)
+    assert np.array_equal(got, O.valid(L, fe))
+    rho_e, vel_e = O.moments(L, fe, n)
+    assert np.array_equal(rho_g, rho_e) and np.array_equal(vel_g, vel_e)
+    assert abs(sim.time - 10000.0) == 0.0
+
+
+def test_periodic_noise_init_and_steps_match_oracle():
+    """Configuration 2 keys at 32^3: init (noise + Guo shift) then 100 steps."""
+    text = _read("periodic_d3q19_256").replace("256 256 256", "32 32 32")
+    sim = host.Simulation(text)
+    sim.initialize()
+    f0 = sim.f()
+    sim.update_macroscopic(-1) if False else None
+    L = O.Lattice("D3Q19")
+    n = [32, 32, 32]
+    # the initial populations are feq(rho, u - F/2rho) of the seeded noise
+    rho0, vel0 = O.moments(L, _oracle_fab(L, n, f0), n, sim.force, streamed=False)
+    fe0, _, _ = O.init_equilibrium(L, np.ones((32, 32, 32)), vel0 - 0.5 * np.array(sim.force)[:, None, None, None],
+                                   n, None)
+    assert np.allclose(O.valid(L, fe0), f0, rtol=0, atol=1e-15)
+    assert np.abs(vel0).max() <= 1e-3 + 1e-12 and np.abs(vel0).max() > 9e-4
+    sim.advance(100)
+    res = O.resolve_bc(3, [5] * 6)
+    fab = _oracle_fab(L, n, f0)
+    fe = O.run(L, fab, n, sim.omega, sim.force, [1, 1, 1], res, 100)
+    assert np.array_equal(sim.f(), O.valid(L, fe))
+
+
+def test_uniform_force_every_step_through_executable(tmp_path):
+    """tests/regression/uniform_force, checked at every step (plot_int=1):
+    u_k = k F / rho exactly and rho = 1, in the plotfiles the executable
+    writes (physical units: u * dx/dt)."""
+    inp = tmp_path / "inputs"
+    inp.write_text(_read("uniform_force_2d"))
+    exe = host.EXE_PATH
+    r = subprocess.run([exe, str(inp), "amr.plot_int=1", f"amr.plot_file={tmp_path}/plt",
+                        "amr.chk_int=0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    u = host.inputs_units(_read("uniform_force_2d"))
+    scale = u["dx"] / u["dt"]
+    F = np.array([u["F0"], u["F1"]])
+    for step in range(30):
+        names, t, s, data = host.read_plotfile(str(tmp_path / f"plt{step:07d}"))
+        assert names == ["density", "vx", "vy"] and s == step
+        k = step + 1
+        assert abs(t - k * u["dt"]) < 1e-12
+        assert np.max(np.abs(data[0] - 1.0)) < 1e-12
+        for d in range(2):
+            assert np.max(np.abs(data[1 + d] / scale - k * F[d])) < 1e-12
+
+
+def test_restart_is_bit_identical(tmp_path):
+    """tests/unit/test_restart.cpp:137-209: run to step 35 writing a
+    checkpoint every 10 steps; restart from chk0000020 and finish; the final
+    populations are identical."""
+    base = _read("lid_cavity_128").replace("128 128", "32 32") + \
+        f"\namr.check_file = {tmp_path}/chk\namr.chk_int = 10\nsim.max_step = 35\n"
+    a = host.Simulation(base)
+    a.initialize()
+    for step in range(0, 36):
+        a.advance()
+        a.write_output(step)
+    fa = a.f()
+    assert os.path.exists(tmp_path / "chk0000020" / "Header")
+    b = host.Simulation(base, [f"amr.restart={tmp_path}/chk0000020", "amr.chk_int=0"])
+    b.initialize()
+    assert b.start_step == 21 and b.time == 21.0
+    for step in range(b.start_step, 36):
+        b.advance()
+    assert np.array_equal(b.f(), fa)
+    with pytest.raises(host.HostError, match="domain mismatch"):
+        c = host.Simulation(base.replace("32 32", "16 16"),
+                            [f"amr.restart={tmp_path}/chk0000020"])
+        c.initialize()
+
+
+def test_closed_box_mass_conservation_with_refreshed_moments():
+    """tests/unit/test_mass_conservation.cpp with the moments refreshed (the
+    reference compares two stale rho fields): NoSlip box, tau=1, 50 steps."""
+    text = ("grid.ncells = 12 6\nsim.max_step = 49\nlbm.lattice_type = D2Q9\nlbm.tau = 1.0\n"
+            "units.Re = 10\ngeometry.prob_lo = 0 0\ngeometry.prob_hi = 2 1\n"
+            "bc.lo_type = NoSlip NoSlip\nbc.hi_type = NoSlip NoSlip\nlbm.init = noise\n"
+            "lbm.init_amp = 0.01\namr.plot_int = 0\namr.chk_int = 0\n")
+    sim = host.Simulation(text)
+    sim.initialize()
+    sim.update_macroscopic(-1) if False else None
+    r0 = O.moments(O.Lattice("D2Q9"), _oracle_fab(O.Lattice("D2Q9"), [12, 6], sim.f()), [12, 6],
+                   streamed=False)[0].sum()
+    sim.advance(50)
+    sim.update_macroscopic(49)
+    r1, v1 = sim.macro()
+    assert abs(r1.sum() - r0) / r0 < 1e-12
+    assert np.abs(v1).max() > 0

@@ -1,0 +1,83 @@
+"""Host (C++) surface on the CPU: header/exports, the inputs parser and the
+UnitConverter chain against the reference's own numbers
+(tests/unit/test_units.cpp, the lid2d and uniform_force inputs, the
+configurations' tau/nu), and the abort conditions of InputParser/Units."""
+import os
+import re
+
+import pytest
+
+from muphfasa_amd import host
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INP = os.path.join(ROOT, "tests", "golden", "inputs")
+
+
+def _read(name):
+    return open(os.path.join(INP, name)).read()
+
+
+def test_host_exports_match_header():
+    text = open(os.path.join(ROOT, "include", "mfa_host.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|void|const char\*|mfa_sim\*)\s+(mfa_\w+)\s*\(", text, re.M))
+    assert declared == set(host.EXPORTS)
+    L = host.lib()
+    for name in declared:
+        assert hasattr(L, name)
+
+
+def test_units_kat_test_units_cpp():
+    # tests/unit/test_units.cpp:32-54: L=1, U=1, Re=32, N=32 -> dx=1/32, a_lat = a dt^2/dx
+    text = ("grid.ncells = 32 32 32\nsim.max_step = 1\nunits.Re = 32\n"
+            "geometry.prob_lo = 0 0 0\ngeometry.prob_hi = 1 1 1\n"
+            "bc.lo_type = Periodic Periodic Periodic\nbc.hi_type = Periodic Periodic Periodic\n"
+            "forcing.accel = 0.125 0 0\n")
+    u = host.inputs_units(text)
+    assert abs(u["dx"] - 1.0 / 32.0) <= 1e-15
+    a = 0.125
+    assert abs(u["F0"] - a * u["dt"] * u["dt"] / u["dx"]) <= 1e-15
+    assert abs(u["F0"] - a * u["dx"] * u["u_lat"] ** 2) <= 1e-15
+    assert u["F1"] == 0.0
+
+
+def test_units_of_configurations():
+    u = host.inputs_units(_read("lid_cavity_128"))
+    assert abs(u["nu_lat"] - 0.128) < 1e-15 and abs(u["tau"] - 0.884) < 1e-15
+    u = host.inputs_units(_read("periodic_d3q19_256"))
+    assert u["dx"] == 1.0 and u["dt"] == 1.0 and u["omega"] == 1.25 and u["F0"] == 1e-6
+    # tests/regression/lid2d/inputs: tau pinned at 0.692 gives u_lat = 0.1 at N=64
+    lid2d = _read("lid_cavity_128").replace("128 128", "64 64").replace("lbm.u_lat          = 0.1",
+                                                                       "lbm.tau = 0.692")
+    lid2d = lid2d.replace("units.ref_velocity = 1.0", "units.ref_velocity = 0.05")
+    u = host.inputs_units(lid2d)
+    assert abs(u["u_lat"] - 0.1) < 1e-14 and abs(u["tau"] - 0.692) < 1e-14
+
+
+@pytest.mark.parametrize("override,msg", [
+    ("units.nu_phys=0.01", "Both units.Re and units.nu_phys"),
+    ("lbm.tau=0.9", "lbm.u_lat and lbm.tau were both"),
+    ("lbm.u_lat=0.35", "low Mach"),
+    ("geometry.prob_hi=1.0 2.0", "square"),
+    ("bc.hiy_Uwall=1.0 0.5", "normal"),
+])
+def test_input_errors(override, msg):
+    with pytest.raises(host.HostError, match=msg):
+        host.inputs_units(_read("lid_cavity_128"), [override])
+
+
+def test_missing_required_key_and_bad_syntax():
+    text = _read("lid_cavity_128").replace("grid.ncells        = 128 128\n", "")
+    with pytest.raises(host.HostError, match="grid.ncells"):
+        host.inputs_units(text)
+    with pytest.raises(host.HostError, match="no '='"):
+        host.inputs_units("grid.ncells 32 32\n")
+    with pytest.raises(host.HostError, match="tau is not within"):
+        host.inputs_units(_read("lid_cavity_128"), ["units.Re=0.01"])
+
+
+def test_parmparse_last_definition_wins_and_comments():
+    text = _read("lid_cavity_128") + "\nunits.Re = 400  # later definition wins\n"
+    u = host.inputs_units(text)
+    assert u["Re"] == 400.0
+    u2 = host.inputs_units(text, ["units.Re=100"])     # command-line override wins
+    assert u2["Re"] == 100.0
