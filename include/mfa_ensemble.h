@@ -1,0 +1,61 @@
+/* mfa_ensemble.h -- C ABI of the batched engine: hundreds of independent
+ * D2Q9 lattices of uneven sizes advanced by one kernel launch per time step
+ * (persistent tile loop), optionally captured in a hipGraph, with per-cell
+ * solid obstacles and per-lattice convergence retirement.
+ *
+ * No counterpart in the reference (it advances one lattice per process);
+ * every lattice uses the reference's per-cell step (the same kernels as
+ * mfa_step) and one shared resolved boundary table (mfa_resolve_bc).
+ * Populations of lattice l live at element offset pop_off[l] of one buffer,
+ * component-major [q][y][x]; per-cell arrays (masks: uint8, 1 = solid; rho;
+ * vel as two planes of total_cells) at cell_off[l].  The ensemble object
+ * owns only its small metadata; population and field buffers belong to the
+ * caller.  Status codes as in mfa_lbm.h; message from mfa_ensemble_error().
+ */
+#ifndef MFA_ENSEMBLE_H
+#define MFA_ENSEMBLE_H
+
+#include "mfa_lbm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mfa_ensemble mfa_ensemble;
+
+const char* mfa_ensemble_error(void);
+
+/* lattice: MFA_D2Q9; dtype: MFA_F64 or MFA_F32 (shifted deviations);
+ * tile_x*tile_y cells per block (multiple of 64, at most 256); masked: the
+ * lattices carry solid masks.  NULL on error. */
+mfa_ensemble* mfa_ensemble_create(int lattice, int dtype, int nlat, const int* nx, const int* ny,
+                                  const double* omega, const mfa_geom_bc* bc, int tile_x,
+                                  int tile_y, int masked);
+void mfa_ensemble_destroy(mfa_ensemble* e);
+int mfa_ensemble_info(const mfa_ensemble* e, long long* total_populations, long long* total_cells,
+                      long long* pop_offsets, long long* cell_offsets, int* ntiles);
+/* Active lattices (host array of nlat flags); retired lattices are skipped. */
+int mfa_ensemble_set_active(mfa_ensemble* e, const int* active);
+/* Number of persistent blocks (0 = one block per tile). */
+int mfa_ensemble_set_grid(mfa_ensemble* e, int blocks);
+/* f = feq(rho, u) of the given per-cell fields (NULL = rest state rho=1,u=0);
+ * solid cells get the rest equilibrium. */
+int mfa_ensemble_init(mfa_ensemble* e, void* f, const unsigned char* masks, const double* rho,
+                      const double* vel, void* stream);
+int mfa_ensemble_step(mfa_ensemble* e, const void* fin, void* fout, const unsigned char* masks,
+                      void* stream);
+/* nsteps steps alternating fa->fb->fa...; use_graph captures them in one
+ * hipGraph and replays it. */
+int mfa_ensemble_run(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks, int nsteps,
+                     int use_graph, void* stream);
+int mfa_ensemble_moments(mfa_ensemble* e, const void* f, const unsigned char* masks, double* rho,
+                         double* vel, void* stream);
+/* Host output per_lattice[2l] = sum |u - u_prev|^2, [2l+1] = sum |u|^2 over
+ * the fluid cells of active lattice l (synchronises the stream). */
+int mfa_ensemble_residual(mfa_ensemble* e, const double* vel, const double* vel_prev,
+                          const unsigned char* masks, double* per_lattice, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

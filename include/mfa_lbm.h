@@ -38,9 +38,11 @@ typedef enum mfa_status {
 /* lattice ids: src/LBM/LatticeDescriptor.H:266-269 */
 enum { MFA_D2Q9 = 0, MFA_D3Q15 = 1, MFA_D3Q19 = 2, MFA_D3Q27 = 3 };
 
-/* storage types.  F64: the reference's amrex::Real.  F32: single.
- * F16DEV / BF16DEV: half / bfloat16 of the shifted deviation f_q - w_q
- * (collision always runs in FP32 for the reduced types). */
+/* storage types.  F64: the reference's amrex::Real (exact reference
+ * arithmetic).  F32 / F16DEV / BF16DEV: FP32 / IEEE half / bfloat16 of the
+ * shifted deviation d_q = f_q - w_q, collided in FP32 (mfa_step,
+ * mfa_moments, mfa_init_equilibrium, mfa_reduce mass/momentum and
+ * mfa_convert accept them; the halo entry points are FP64 only). */
 enum { MFA_F64 = 0, MFA_F32 = 1, MFA_F16DEV = 2, MFA_BF16DEV = 3 };
 
 /* boundary types, numbered as the reference's lbmBCtype
@@ -152,6 +154,12 @@ int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void*
 /* Steady-state residual: out_dev[0] = sum (a-b)^2, out_dev[1] = sum a^2 over
  * the valid cells of all components of two same-shaped F64 fields. */
 int mfa_residual(const mfa_field* a, const mfa_field* b, double* out_dev, void* stream);
+
+/* Convert populations between FP64 and a reduced storage type (valid
+ * cells): FP64 -> d = T(f - w_q) with round-to-nearest-even, or back
+ * f = w_q + d.  Exactly one of src/dst is FP64.  No reference counterpart
+ * (the reference stores FP64 only). */
+int mfa_convert(const mfa_field* src, mfa_field* dst, int lattice, void* stream);
 
 /* Measurement utility (no reference counterpart): a streaming device copy of
  * `bytes` (multiple of 16) with 16-byte loads/stores, one per thread.  Its

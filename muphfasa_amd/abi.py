@@ -22,7 +22,8 @@ RED = {"mass": 0, "momentum": 1, "comp_sum": 2}
 
 EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_pull_collide",
            "mfa_fill_periodic", "mfa_apply_bc", "mfa_moments", "mfa_init_equilibrium",
-           "mfa_reduce", "mfa_residual", "mfa_set_launch", "mfa_get_launch", "mfa_stream_copy"]
+           "mfa_reduce", "mfa_residual", "mfa_set_launch", "mfa_get_launch", "mfa_stream_copy",
+           "mfa_convert"]
 
 
 class MfaError(RuntimeError):
@@ -78,6 +79,7 @@ def lib():
         L.mfa_set_launch.argtypes = [ctypes.c_int, ctypes.POINTER(Launch)]
         L.mfa_get_launch.argtypes = [ctypes.c_int, ctypes.POINTER(Launch)]
         L.mfa_stream_copy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
+        L.mfa_convert.argtypes = [F, F, ctypes.c_int, vp]
         for name in EXPORTS[2:]:
             getattr(L, name).restype = ctypes.c_int
         _lib = L
@@ -200,3 +202,7 @@ def get_launch(lattice: int):
 def stream_copy(src_ptr: int, dst_ptr: int, nbytes: int, nontemporal=True, stream=None):
     check(lib().mfa_stream_copy(ctypes.c_void_p(src_ptr), ctypes.c_void_p(dst_ptr), nbytes,
                                 1 if nontemporal else 0, stream_ptr(stream)))
+
+
+def convert(src: Field, dst: Field, lattice: int, stream=None):
+    check(lib().mfa_convert(ctypes.byref(src), ctypes.byref(dst), lattice, stream_ptr(stream)))

@@ -1,0 +1,258 @@
+// kernels_common.hpp -- device-side building blocks shared by the FP64 and
+// reduced-precision kernels of the step engine: kernel argument structs,
+// pull-source addressing (periodic wrap / clamp), wall ghost values and the
+// compile-time loop over populations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <type_traits>
+
+#include "lattice.hpp"
+
+namespace mfa {
+
+int fail(int code, const std::string& msg);
+
+#define MFA_HIP_CHECK(expr)                                                         \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return fail(MFA_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------- kernel arguments --
+struct Geo {
+    int n[3];
+    int per[3];
+    int walls;            // some dimension is non-periodic
+    signed char rtype[27];
+    double rrho[27];
+    double ru[27][3];
+};
+
+// Geometry seen through pointers to boundary tables held in device memory
+// (used by the batched engine, whose lattices share one boundary table).
+struct GeoRef {
+    int n[3];
+    int per[3];
+    int walls;
+    const signed char* rtype;
+    const double* rrho;
+    const double (*ru)[3];
+};
+
+struct View {             // one field, addressed relative to cell (0,0,0)
+    long long sy, sz, sq;
+};
+
+static View view_of(const mfa_field& f) { return View{f.sy, f.sz, f.sq}; }
+
+// ------------------------------------------------------------- pull stage --
+// Coordinates of the pull source for each lattice velocity component value
+// (index c+1): source = x - c.  Fused mode wraps periodic dimensions and
+// clamps non-periodic ones (the clamped load is replaced by the wall value).
+struct Src {
+    long long o[3][3];    // per dimension, per (c+1): element offset
+};
+
+template <int D, bool HALO, class G>
+__device__ __forceinline__ Src make_src(int x, int y, int z, const G& g, const View& v)
+{
+    Src s;
+    const int p[3] = {x, y, z};
+    const long long st[3] = {1, v.sy, v.sz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci) {
+            const int c = ci - 1;
+            int q = p[d] - c;
+            if (d >= D) q = p[d];
+            else if (!HALO) {
+                if (q < 0) q = g.per[d] ? g.n[d] - 1 : 0;
+                else if (q >= g.n[d]) q = g.per[d] ? 0 : g.n[d] - 1;
+            }
+            s.o[d][ci] = (long long)q * st[d];
+        }
+    }
+    return s;
+}
+
+// Ghost value f(g, q) for g = x - c_q outside a non-periodic face, evaluated
+// from fin's valid cells as the reference's BCManager would have written it
+// (src/LBM/BoundaryConditions.H:124-189 and the src/LBM/BCTypes/ headers).
+template <class L, int q, class G>
+__device__ __forceinline__ double wall_value(const double* __restrict__ fin, long long self,
+                                             const View& v, const G& g, int idx)
+{
+    const int t = g.rtype[idx];
+    if (t == MFA_BC_MOVING_WALL || t == MFA_BC_NO_SLIP || t == MFA_BC_INFLOW) {
+        constexpr int m = L::opp(q);
+        return fin[m * v.sq + self] +
+               bb_contrib<L, m>(g.rrho[idx], g.ru[idx][0], g.ru[idx][1], g.ru[idx][2]);
+    }
+    if (t == MFA_BC_OUTFLOW) return fin[q * v.sq + self];
+    if (t == MFA_BC_PRESSURE_OUTLET) {
+        // src/LBM/BCTypes/PressureOutletKernel.H:29-64
+        double rho = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < L::Q; ++m) {
+            const double fm = fin[m * v.sq + self];
+            rho += fm;
+            j0 += L::c(m, 0) * fm;
+            j1 += L::c(m, 1) * fm;
+            if constexpr (L::D == 3) j2 += L::c(m, 2) * fm;
+        }
+        const double inv_rho = 1.0 / rho;
+        const double cu = cdot<L, q>(j0, j1, j2) * inv_rho;
+        double uu;
+        if constexpr (L::D == 2) uu = (j0 * j0 + j1 * j1) * inv_rho * inv_rho;
+        else uu = (j0 * j0 + j1 * j1 + j2 * j2) * inv_rho * inv_rho;
+        const double feq_even =
+            L::w(q) * g.rrho[idx] * (1.0 + 0.5 * cu * cu * 3.0 * 3.0 - 0.5 * uu * 3.0);
+        constexpr int m = L::opp(q);
+        return -fin[m * v.sq + self] + 2.0 * feq_even;
+    }
+    return __builtin_nan("");
+}
+
+template <class L, int q, class G>
+__device__ __forceinline__ void fix_wall(double& fq, const double* __restrict__ fin, long long self,
+                                         const View& v, const G& g, const int lo[3],
+                                         const int hi[3])
+{
+    int code[3] = {0, 0, 0};
+    bool wall = false;
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+        const int c = L::c(q, d);
+        if (c == 1 && lo[d]) { code[d] = -1; wall = true; }
+        if (c == -1 && hi[d]) { code[d] = 1; wall = true; }
+    }
+    if (wall) {
+        const int idx = (code[0] + 1) + 3 * (code[1] + 1) + 9 * (code[2] + 1) * (L::D == 3);
+        fq = wall_value<L, q>(fin, self, v, g, idx);
+    }
+}
+
+template <class L, int q>
+struct Unroll {
+    template <class Fn>
+    static __device__ __forceinline__ void run(Fn&& fn)
+    {
+        fn(std::integral_constant<int, q>{});
+        if constexpr (q + 1 < L::Q) Unroll<L, q + 1>::run(fn);
+    }
+};
+
+template <class L, class Fn>
+__device__ __forceinline__ void for_q(Fn&& fn) { Unroll<L, 0>::run(fn); }
+
+// Pull the Q populations of cell (x,y,z).  mode: 0 fused (inline wrap and
+// walls), 1 halo (reads ghost cells), 2 unstreamed (the cell's own values).
+// WALLS: some dimension is non-periodic (compile-time, so the periodic
+// kernel carries no boundary code).  NT: non-temporal loads.
+template <bool NT>
+__device__ __forceinline__ double ld(const double* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <class L, int MODE, bool WALLS = true, bool NT = false, class G = Geo>
+__device__ __forceinline__ void pull(double (&f)[L::Q], const double* __restrict__ fin, int x, int y,
+                                     int z, const G& g, const View& v)
+{
+    const long long self = x + y * v.sy + z * v.sz;
+    if constexpr (MODE == 2) {
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = ld<NT>(&fin[q * v.sq + self]);
+        });
+        return;
+    } else {
+        const Src s = make_src<L::D, MODE == 1>(x, y, z, g, v);
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const long long off = s.o[0][L::c(q, 0) + 1] + s.o[1][L::c(q, 1) + 1] +
+                                  s.o[2][L::c(q, 2) + 1];
+            f[q] = ld<NT>(&fin[q * v.sq + off]);
+        });
+        if constexpr (MODE == 0 && WALLS) {
+            const int p[3] = {x, y, z};
+            int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            bool edge = false;
+#pragma unroll
+            for (int d = 0; d < L::D; ++d) {
+                lo[d] = !g.per[d] && p[d] == 0;
+                hi[d] = !g.per[d] && p[d] == g.n[d] - 1;
+                edge = edge || lo[d] || hi[d];
+            }
+            if (edge) {
+                for_q<L>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    fix_wall<L, q>(f[q], fin, self, v, g, lo, hi);
+                });
+            }
+        }
+    }
+}
+
+// rho and u of Q populations: src/LBM/Macroscopic.H:36-45,115-127,186-201.
+template <class L, bool FORCED>
+__device__ __forceinline__ void moments(const double (&f)[L::Q], const double F[3], double& rho,
+                                        double& u0, double& u1, double& u2)
+{
+    rho = 0.;
+    double j0 = 0., j1 = 0., j2 = 0.;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        rho += f[q];
+    });
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (L::c(q, 0) != 0) j0 = (L::c(q, 0) > 0) ? j0 + f[q] : j0 - f[q];
+        if constexpr (L::c(q, 1) != 0) j1 = (L::c(q, 1) > 0) ? j1 + f[q] : j1 - f[q];
+        if constexpr (L::D == 3 && L::c(q, 2) != 0) j2 = (L::c(q, 2) > 0) ? j2 + f[q] : j2 - f[q];
+    });
+    if constexpr (FORCED) {
+        u0 = (j0 + 0.5 * F[0]) / rho;
+        u1 = (j1 + 0.5 * F[1]) / rho;
+        u2 = (L::D == 3) ? (j2 + 0.5 * F[2]) / rho : 0.0;
+    } else {
+        u0 = j0 / rho;
+        u1 = j1 / rho;
+        u2 = (L::D == 3) ? j2 / rho : 0.0;
+    }
+}
+
+
+// BGK (+Guo) collision of pulled populations and store:
+// src/LBM/Collision.cpp:24-53 (plain) and :62-92 (forced).
+template <class L, bool FORCED>
+__device__ __forceinline__ void collide_store(const double (&f)[L::Q], double* __restrict__ fout,
+                                              long long self, const View& vo, const double omega,
+                                              const double F0, const double F1, const double F2)
+{
+    const double F[3] = {F0, F1, F2};
+    double rho, u0, u1, u2;
+    moments<L, FORCED>(f, F, rho, u0, u1, u2);
+    const double usq = usq_of<L>(u0, u1, u2);
+    const double omw = 1 - omega;
+    double pref = 0.0, uF = 0.0;
+    if constexpr (FORCED) {
+        pref = 1.0 - 0.5 * omega;
+        uF = (L::D == 2) ? u0 * F0 + u1 * F1 : u0 * F0 + u1 * F1 + u2 * F2;
+    }
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const double fe = feq<L, q>(rho, u0, u1, u2, usq);
+        double out = omw * f[q] + omega * fe;
+        if constexpr (FORCED) out = out + guo<L, q>(u0, u1, u2, F0, F1, F2, pref, uF);
+        __builtin_nontemporal_store(out, &fout[q * vo.sq + self]);
+    });
+}
+
+}  // namespace mfa

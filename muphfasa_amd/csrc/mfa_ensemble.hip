@@ -1,0 +1,534 @@
+// mfa_ensemble.hip -- batched engine: many independent 2-D lattices of
+// uneven sizes advance together, one kernel launch per time step for the
+// whole ensemble.
+//
+// Each lattice l (nx_l x ny_l cells, its own omega) stores its populations
+// contiguously, component-major ([q][y][x]) at pop_off_l.  The work is a
+// flat list of tiles (lattice, x0, y0) of TX x TY cells, one cell per thread;
+// a fixed grid of blocks walks the list with a block stride (persistent
+// blocks, no atomics, deterministic).  Retired (converged) lattices are
+// removed from the tile list on the host.  Every lattice shares one resolved
+// boundary table (e.g. the lid cavity); an optional per-cell solid mask adds
+// half-way bounce-back obstacles inside the domain.  Per-cell arithmetic is
+// the single-lattice engine's (FP64: the reference's exact expressions; FP32:
+// shifted deviations collided in FP32), so an FP64 ensemble member is
+// bit-identical to the same lattice run alone through mfa_step.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mfa_ensemble.h"
+#include "kernels_lowprec.hpp"
+
+namespace mfa {
+
+struct LatInfo {
+    int nx, ny;
+    double omega;
+    long long pop_off;   // element offset of the lattice's population block
+    long long cell_off;  // cell offset (masks, moments)
+};
+
+struct Tile {
+    int lat, x0, y0, pad;
+};
+
+struct BcTable {
+    signed char rtype[27];
+    double rrho[27];
+    double ru[27][3];
+};
+
+template <class L>
+__device__ __forceinline__ GeoRef geo_for(const LatInfo& li, const BcTable* bc)
+{
+    GeoRef g;
+    g.n[0] = li.nx;
+    g.n[1] = li.ny;
+    g.n[2] = 1;
+    g.per[0] = g.per[1] = 0;
+    g.per[2] = 1;
+    g.walls = 1;
+    g.rtype = bc->rtype;
+    g.rrho = bc->rrho;
+    g.ru = bc->ru;
+    return g;
+}
+
+// Half-way bounce-back on solid obstacle cells: a population pulled from a
+// solid neighbour is the cell's own population pointing into it.
+template <class L, class V, class Ld>
+__device__ __forceinline__ void obstacle_bb(V (&f)[L::Q], const unsigned char* __restrict__ mask, int x,
+                                            int y, int nx, int ny, Ld&& load_self)
+{
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int sx = x - L::c(q, 0), sy = y - L::c(q, 1);
+        if (sx >= 0 && sx < nx && sy >= 0 && sy < ny && mask[sx + sy * nx])
+            f[q] = load_self(L::opp(q));
+    });
+}
+
+template <class L, class T, bool MASKED>
+__global__ __launch_bounds__(256) void k_batch_step(const T* __restrict__ fin, T* __restrict__ fout,
+                                                    const unsigned char* __restrict__ masks,
+                                                    const Tile* __restrict__ tiles, int ntiles,
+                                                    const LatInfo* __restrict__ lats,
+                                                    const BcTable* __restrict__ bc)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        const LatInfo li = lats[tl.lat];
+        const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
+        if (x >= li.nx || y >= li.ny) continue;
+        const View v{li.nx, (long long)li.nx * li.ny, (long long)li.nx * li.ny};
+        const T* a = fin + li.pop_off;
+        T* b = fout + li.pop_off;
+        const long long self = x + (long long)y * li.nx;
+        const unsigned char* mask = MASKED ? masks + li.cell_off : nullptr;
+        const GeoRef g = geo_for<L>(li, bc);
+        if constexpr (std::is_same<T, double>::value) {
+            if (MASKED && mask[self]) {
+                for_q<L>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    b[q * v.sq + self] = L::w(q);
+                });
+                continue;
+            }
+            double f[L::Q];
+            pull<L, 0, true, false>(f, a, x, y, 0, g, v);
+            if constexpr (MASKED)
+                obstacle_bb<L>(f, mask, x, y, li.nx, li.ny, [&](int m) { return a[m * v.sq + self]; });
+            collide_store<L, false>(f, b, self, v, li.omega, 0.0, 0.0, 0.0);
+        } else {
+            if (MASKED && mask[self]) {
+                for_q<L>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    b[q * v.sq + self] = from_f<T>(0.0f);
+                });
+                continue;
+            }
+            float d[L::Q];
+            pull_dev<L, true>(d, a, x, y, 0, g, v);
+            if constexpr (MASKED)
+                obstacle_bb<L>(d, mask, x, y, li.nx, li.ny, [&](int m) { return to_f(a[m * v.sq + self]); });
+            collide_store_dev<L, false>(d, b, self, v, (float)li.omega, 0.f, 0.f, 0.f);
+        }
+    }
+}
+
+// rho, u of every cell (FP64), pulled populations like updateMacroscopic.
+template <class L, class T, bool MASKED>
+__global__ __launch_bounds__(256) void k_batch_moments(const T* __restrict__ fin,
+                                                       const unsigned char* __restrict__ masks,
+                                                       const Tile* __restrict__ tiles, int ntiles,
+                                                       const LatInfo* __restrict__ lats,
+                                                       const BcTable* __restrict__ bc,
+                                                       double* __restrict__ rho, double* __restrict__ vel,
+                                                       long long total_cells)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        const LatInfo li = lats[tl.lat];
+        const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
+        if (x >= li.nx || y >= li.ny) continue;
+        const View v{li.nx, (long long)li.nx * li.ny, (long long)li.nx * li.ny};
+        const T* a = fin + li.pop_off;
+        const long long self = x + (long long)y * li.nx;
+        const long long cell = li.cell_off + self;
+        const unsigned char* mask = MASKED ? masks + li.cell_off : nullptr;
+        if (MASKED && mask[self]) {
+            rho[cell] = 1.0;
+            vel[cell] = 0.0;
+            vel[total_cells + cell] = 0.0;
+            continue;
+        }
+        const GeoRef g = geo_for<L>(li, bc);
+        double f[L::Q];
+        if constexpr (std::is_same<T, double>::value) {
+            pull<L, 0, true, false>(f, a, x, y, 0, g, v);
+            if constexpr (MASKED)
+                obstacle_bb<L>(f, mask, x, y, li.nx, li.ny, [&](int m) { return a[m * v.sq + self]; });
+        } else {
+            float d[L::Q];
+            pull_dev<L, true>(d, a, x, y, 0, g, v);
+            if constexpr (MASKED)
+                obstacle_bb<L>(d, mask, x, y, li.nx, li.ny, [&](int m) { return to_f(a[m * v.sq + self]); });
+            for_q<L>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                f[q] = L::w(q) + (double)d[q];
+            });
+        }
+        const double F[3] = {0.0, 0.0, 0.0};
+        double r, u0, u1, u2;
+        moments<L, false>(f, F, r, u0, u1, u2);
+        rho[cell] = r;
+        vel[cell] = u0;
+        vel[total_cells + cell] = u1;
+    }
+}
+
+// Equilibrium initialisation from per-cell rho/u (warm start) or rest.
+template <class L, class T, bool MASKED>
+__global__ __launch_bounds__(256) void k_batch_init(T* __restrict__ fout, const unsigned char* __restrict__ masks,
+                                                    const Tile* __restrict__ tiles, int ntiles,
+                                                    const LatInfo* __restrict__ lats,
+                                                    const double* __restrict__ rho,
+                                                    const double* __restrict__ vel, long long total_cells)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        const LatInfo li = lats[tl.lat];
+        const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
+        if (x >= li.nx || y >= li.ny) continue;
+        const long long sq = (long long)li.nx * li.ny;
+        const long long self = x + (long long)y * li.nx;
+        const long long cell = li.cell_off + self;
+        const bool solid = MASKED && masks[cell];
+        const double r = (rho && !solid) ? rho[cell] : 1.0;
+        const double u0 = (vel && !solid) ? vel[cell] : 0.0;
+        const double u1 = (vel && !solid) ? vel[total_cells + cell] : 0.0;
+        const double usq = usq_of<L>(u0, u1, 0.0);
+        T* b = fout + li.pop_off;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const double fe = feq<L, q>(r, u0, u1, 0.0, usq);
+            if constexpr (std::is_same<T, double>::value) b[q * sq + self] = fe;
+            else b[q * sq + self] = from_f<T>((float)(fe - L::w(q)));
+        });
+    }
+}
+
+// Per-tile partial sums of |u - u_prev|^2 and |u|^2 over fluid cells.
+template <bool MASKED>
+__global__ __launch_bounds__(256) void k_batch_residual(const double* __restrict__ vel,
+                                                        const double* __restrict__ vel_prev,
+                                                        const unsigned char* __restrict__ masks,
+                                                        const Tile* __restrict__ tiles, int ntiles,
+                                                        const LatInfo* __restrict__ lats,
+                                                        long long total_cells,
+                                                        double* __restrict__ part)
+{
+    __shared__ double sh[4][2];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        const LatInfo li = lats[tl.lat];
+        const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
+        double a = 0.0, b = 0.0;
+        if (x < li.nx && y < li.ny) {
+            const long long cell = li.cell_off + x + (long long)y * li.nx;
+            if (!(MASKED && masks[cell])) {
+                const double u0 = vel[cell], u1 = vel[total_cells + cell];
+                const double d0 = u0 - vel_prev[cell], d1 = u1 - vel_prev[total_cells + cell];
+                a = d0 * d0 + d1 * d1;
+                b = u0 * u0 + u1 * u1;
+            }
+        }
+        const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_down(a, off, 64);
+            b += __shfl_down(b, off, 64);
+        }
+        if ((tid & 63) == 0) {
+            sh[tid >> 6][0] = a;
+            sh[tid >> 6][1] = b;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const int nw = (blockDim.x * blockDim.y + 63) / 64;
+            double sa = 0.0, sb = 0.0;
+            for (int w = 0; w < nw; ++w) {
+                sa += sh[w][0];
+                sb += sh[w][1];
+            }
+            part[2 * t] = sa;
+            part[2 * t + 1] = sb;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace mfa
+
+using namespace mfa;
+
+struct mfa_ensemble {
+    int lattice = MFA_D2Q9, dtype = MFA_F64, nlat = 0, tx = 64, ty = 4, masked = 0, grid = 0;
+    std::vector<LatInfo> lats;
+    std::vector<int> active;
+    std::vector<Tile> tiles;
+    long long total_pops = 0, total_cells = 0;
+    LatInfo* d_lats = nullptr;
+    Tile* d_tiles = nullptr;
+    BcTable* d_bc = nullptr;
+    double* d_part = nullptr;
+    int ntiles = 0, tiles_cap = 0;
+};
+
+static thread_local std::string g_ens_err;
+
+static int ens_fail(int code, const std::string& m)
+{
+    g_ens_err = m;
+    return code;
+}
+
+#define ENS_HIP(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return ens_fail(MFA_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+static int rebuild_tiles(mfa_ensemble* e)
+{
+    e->tiles.clear();
+    for (int l = 0; l < e->nlat; ++l) {
+        if (!e->active[l]) continue;
+        for (int y0 = 0; y0 < e->lats[l].ny; y0 += e->ty)
+            for (int x0 = 0; x0 < e->lats[l].nx; x0 += e->tx) e->tiles.push_back(Tile{l, x0, y0, 0});
+    }
+    e->ntiles = (int)e->tiles.size();
+    if (e->ntiles > e->tiles_cap) {
+        if (e->d_tiles) (void)hipFree(e->d_tiles);
+        if (e->d_part) (void)hipFree(e->d_part);
+        e->tiles_cap = e->ntiles;
+        ENS_HIP(hipMalloc((void**)&e->d_tiles, sizeof(Tile) * e->tiles_cap));
+        ENS_HIP(hipMalloc((void**)&e->d_part, sizeof(double) * 2 * e->tiles_cap));
+    }
+    if (e->ntiles)
+        ENS_HIP(hipMemcpy(e->d_tiles, e->tiles.data(), sizeof(Tile) * e->ntiles, hipMemcpyHostToDevice));
+    return MFA_OK;
+}
+
+static int grid_for(const mfa_ensemble* e)
+{
+    int g = e->grid > 0 ? e->grid : e->ntiles;
+    if (g > e->ntiles) g = e->ntiles;
+    return g < 1 ? 1 : g;
+}
+
+template <class Fn>
+static int with_type(int dtype, Fn&& fn)
+{
+    if (dtype == MFA_F64) return fn(double{});
+    if (dtype == MFA_F32) return fn(float{});
+    return ens_fail(MFA_ERR_UNSUPPORTED, "ensemble: storage must be F64 or F32");
+}
+
+extern "C" {
+
+const char* mfa_ensemble_error(void) { return g_ens_err.c_str(); }
+
+mfa_ensemble* mfa_ensemble_create(int lattice, int dtype, int nlat, const int* nx, const int* ny,
+                                  const double* omega, const mfa_geom_bc* bc, int tile_x, int tile_y,
+                                  int masked)
+{
+    if (lattice != MFA_D2Q9) { ens_fail(MFA_ERR_UNSUPPORTED, "ensemble: D2Q9 only"); return nullptr; }
+    if (dtype != MFA_F64 && dtype != MFA_F32) { ens_fail(MFA_ERR_UNSUPPORTED, "ensemble: F64 or F32"); return nullptr; }
+    if (nlat < 1 || !nx || !ny || !omega || !bc) { ens_fail(MFA_ERR_INVALID, "ensemble: bad arguments"); return nullptr; }
+    if (tile_x < 1 || tile_y < 1 || tile_x * tile_y > 256 || (tile_x * tile_y) % 64) {
+        ens_fail(MFA_ERR_INVALID, "ensemble: tile must hold a multiple of 64 and at most 256 cells");
+        return nullptr;
+    }
+    for (int d = 0; d < 2; ++d)
+        if (bc->periodic[d]) { ens_fail(MFA_ERR_UNSUPPORTED, "ensemble: walled lattices only"); return nullptr; }
+    auto* e = new mfa_ensemble();
+    e->lattice = lattice;
+    e->dtype = dtype;
+    e->nlat = nlat;
+    e->tx = tile_x;
+    e->ty = tile_y;
+    e->masked = masked ? 1 : 0;
+    long long pop = 0, cell = 0;
+    for (int l = 0; l < nlat; ++l) {
+        if (nx[l] < 1 || ny[l] < 1) { delete e; ens_fail(MFA_ERR_INVALID, "ensemble: empty lattice"); return nullptr; }
+        e->lats.push_back(LatInfo{nx[l], ny[l], omega[l], pop, cell});
+        pop += 9LL * nx[l] * ny[l];
+        cell += (long long)nx[l] * ny[l];
+    }
+    e->total_pops = pop;
+    e->total_cells = cell;
+    e->active.assign(nlat, 1);
+    BcTable t;
+    for (int i = 0; i < 27; ++i) {
+        t.rtype[i] = (signed char)bc->resolved_type[i];
+        t.rrho[i] = bc->resolved_rho[i];
+        for (int d = 0; d < 3; ++d) t.ru[i][d] = bc->resolved_u[i][d];
+    }
+    bool ok = hipMalloc((void**)&e->d_lats, sizeof(LatInfo) * nlat) == hipSuccess &&
+              hipMalloc((void**)&e->d_bc, sizeof(BcTable)) == hipSuccess &&
+              hipMemcpy(e->d_lats, e->lats.data(), sizeof(LatInfo) * nlat, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(e->d_bc, &t, sizeof(BcTable), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok || rebuild_tiles(e) != MFA_OK) {
+        ens_fail(MFA_ERR_HIP, "ensemble: device allocation failed");
+        mfa_ensemble_destroy(e);
+        return nullptr;
+    }
+    return e;
+}
+
+void mfa_ensemble_destroy(mfa_ensemble* e)
+{
+    if (!e) return;
+    if (e->d_lats) (void)hipFree(e->d_lats);
+    if (e->d_tiles) (void)hipFree(e->d_tiles);
+    if (e->d_bc) (void)hipFree(e->d_bc);
+    if (e->d_part) (void)hipFree(e->d_part);
+    delete e;
+}
+
+int mfa_ensemble_info(const mfa_ensemble* e, long long* total_populations, long long* total_cells,
+                      long long* pop_offsets, long long* cell_offsets, int* ntiles)
+{
+    if (!e) return ens_fail(MFA_ERR_INVALID, "null ensemble");
+    if (total_populations) *total_populations = e->total_pops;
+    if (total_cells) *total_cells = e->total_cells;
+    for (int l = 0; l < e->nlat; ++l) {
+        if (pop_offsets) pop_offsets[l] = e->lats[l].pop_off;
+        if (cell_offsets) cell_offsets[l] = e->lats[l].cell_off;
+    }
+    if (ntiles) *ntiles = e->ntiles;
+    return MFA_OK;
+}
+
+int mfa_ensemble_set_active(mfa_ensemble* e, const int* active)
+{
+    if (!e || !active) return ens_fail(MFA_ERR_INVALID, "null argument");
+    for (int l = 0; l < e->nlat; ++l) e->active[l] = active[l] ? 1 : 0;
+    return rebuild_tiles(e);
+}
+
+int mfa_ensemble_set_grid(mfa_ensemble* e, int blocks)
+{
+    if (!e || blocks < 0) return ens_fail(MFA_ERR_INVALID, "bad argument");
+    e->grid = blocks;
+    return MFA_OK;
+}
+
+int mfa_ensemble_init(mfa_ensemble* e, void* f, const unsigned char* masks, const double* rho,
+                      const double* vel, void* stream)
+{
+    if (!e || !f) return ens_fail(MFA_ERR_INVALID, "null argument");
+    if (e->masked && !masks) return ens_fail(MFA_ERR_INVALID, "ensemble was created with masks");
+    if (!e->ntiles) return MFA_OK;
+    return with_type(e->dtype, [&](auto tv) {
+        using T = decltype(tv);
+        using L = Lattice<MFA_D2Q9>;
+        const dim3 block(e->tx, e->ty);
+        if (e->masked)
+            hipLaunchKernelGGL((k_batch_init<L, T, true>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (T*)f, masks, e->d_tiles, e->ntiles, e->d_lats, rho, vel, e->total_cells);
+        else
+            hipLaunchKernelGGL((k_batch_init<L, T, false>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (T*)f, masks, e->d_tiles, e->ntiles, e->d_lats, rho, vel, e->total_cells);
+        ENS_HIP(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_ensemble_step(mfa_ensemble* e, const void* fin, void* fout, const unsigned char* masks,
+                      void* stream)
+{
+    if (!e || !fin || !fout || fin == fout) return ens_fail(MFA_ERR_INVALID, "bad populations");
+    if (e->masked && !masks) return ens_fail(MFA_ERR_INVALID, "ensemble was created with masks");
+    if (!e->ntiles) return MFA_OK;
+    return with_type(e->dtype, [&](auto tv) {
+        using T = decltype(tv);
+        using L = Lattice<MFA_D2Q9>;
+        const dim3 block(e->tx, e->ty);
+        if (e->masked)
+            hipLaunchKernelGGL((k_batch_step<L, T, true>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (const T*)fin, (T*)fout, masks, e->d_tiles, e->ntiles, e->d_lats, e->d_bc);
+        else
+            hipLaunchKernelGGL((k_batch_step<L, T, false>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (const T*)fin, (T*)fout, masks, e->d_tiles, e->ntiles, e->d_lats, e->d_bc);
+        ENS_HIP(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_ensemble_run(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks, int nsteps,
+                     int use_graph, void* stream)
+{
+    // nsteps launches alternating fa->fb, fb->fa (the state ends in fa when
+    // nsteps is even).  With use_graph the launches are captured once into a
+    // hipGraph on a private stream and replayed.
+    if (!e || nsteps < 0) return ens_fail(MFA_ERR_INVALID, "bad arguments");
+    if (!use_graph || nsteps == 0) {
+        for (int s = 0; s < nsteps; ++s) {
+            const int rc = mfa_ensemble_step(e, (s & 1) ? fb : fa, (s & 1) ? fa : fb, masks, stream);
+            if (rc) return rc;
+        }
+        return MFA_OK;
+    }
+    hipStream_t cap;
+    ENS_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    hipEvent_t ev;
+    ENS_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    ENS_HIP(hipEventRecord(ev, (hipStream_t)stream));
+    ENS_HIP(hipStreamWaitEvent(cap, ev, 0));
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    ENS_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    int rc = MFA_OK;
+    for (int s = 0; s < nsteps && rc == MFA_OK; ++s)
+        rc = mfa_ensemble_step(e, (s & 1) ? fb : fa, (s & 1) ? fa : fb, masks, cap);
+    const hipError_t ce = hipStreamEndCapture(cap, &graph);
+    if (rc != MFA_OK) return rc;
+    if (ce != hipSuccess) return ens_fail(MFA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    ENS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    ENS_HIP(hipGraphLaunch(exec, cap));
+    ENS_HIP(hipEventRecord(ev, cap));
+    ENS_HIP(hipStreamWaitEvent((hipStream_t)stream, ev, 0));
+    ENS_HIP(hipStreamSynchronize(cap));
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(cap);
+    return MFA_OK;
+}
+
+int mfa_ensemble_moments(mfa_ensemble* e, const void* f, const unsigned char* masks, double* rho,
+                         double* vel, void* stream)
+{
+    if (!e || !f || !rho || !vel) return ens_fail(MFA_ERR_INVALID, "null argument");
+    if (e->masked && !masks) return ens_fail(MFA_ERR_INVALID, "ensemble was created with masks");
+    if (!e->ntiles) return MFA_OK;
+    return with_type(e->dtype, [&](auto tv) {
+        using T = decltype(tv);
+        using L = Lattice<MFA_D2Q9>;
+        const dim3 block(e->tx, e->ty);
+        if (e->masked)
+            hipLaunchKernelGGL((k_batch_moments<L, T, true>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (const T*)f, masks, e->d_tiles, e->ntiles, e->d_lats, e->d_bc, rho, vel, e->total_cells);
+        else
+            hipLaunchKernelGGL((k_batch_moments<L, T, false>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (const T*)f, masks, e->d_tiles, e->ntiles, e->d_lats, e->d_bc, rho, vel, e->total_cells);
+        ENS_HIP(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_ensemble_residual(mfa_ensemble* e, const double* vel, const double* vel_prev,
+                          const unsigned char* masks, double* per_lattice, void* stream)
+{
+    // per_lattice (host, 2*nlat): sum |u-u_prev|^2, sum |u|^2 for active lattices
+    if (!e || !vel || !vel_prev || !per_lattice) return ens_fail(MFA_ERR_INVALID, "null argument");
+    if (e->masked && !masks) return ens_fail(MFA_ERR_INVALID, "ensemble was created with masks");
+    for (int l = 0; l < 2 * e->nlat; ++l) per_lattice[l] = 0.0;
+    if (!e->ntiles) return MFA_OK;
+    const dim3 block(e->tx, e->ty);
+    if (e->masked)
+        hipLaunchKernelGGL(k_batch_residual<true>, dim3(grid_for(e)), block, 0, (hipStream_t)stream, vel, vel_prev, masks, e->d_tiles, e->ntiles, e->d_lats, e->total_cells, e->d_part);
+    else
+        hipLaunchKernelGGL(k_batch_residual<false>, dim3(grid_for(e)), block, 0, (hipStream_t)stream, vel, vel_prev, masks, e->d_tiles, e->ntiles, e->d_lats, e->total_cells, e->d_part);
+    ENS_HIP(hipGetLastError());
+    std::vector<double> part(2 * (size_t)e->ntiles);
+    ENS_HIP(hipMemcpyAsync(part.data(), e->d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    ENS_HIP(hipStreamSynchronize((hipStream_t)stream));
+    for (int t = 0; t < e->ntiles; ++t) {
+        per_lattice[2 * e->tiles[t].lat] += part[2 * t];
+        per_lattice[2 * e->tiles[t].lat + 1] += part[2 * t + 1];
+    }
+    return MFA_OK;
+}
+
+}  // extern "C"

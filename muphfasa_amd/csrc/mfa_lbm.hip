@@ -19,248 +19,21 @@
 #include <string>
 #include <type_traits>
 
-#include "lattice.hpp"
+#include "kernels_common.hpp"
 
 namespace mfa {
 
 // ---------------------------------------------------------------- errors ---
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg)
+int fail(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
 }
 
-#define MFA_HIP_CHECK(expr)                                                         \
-    do {                                                                            \
-        hipError_t e_ = (expr);                                                     \
-        if (e_ != hipSuccess)                                                       \
-            return fail(MFA_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-// ------------------------------------------------------- kernel arguments --
-struct Geo {
-    int n[3];
-    int per[3];
-    int walls;            // some dimension is non-periodic
-    signed char rtype[27];
-    double rrho[27];
-    double ru[27][3];
-};
-
-struct View {             // one field, addressed relative to cell (0,0,0)
-    long long sy, sz, sq;
-};
-
-static View view_of(const mfa_field& f) { return View{f.sy, f.sz, f.sq}; }
-
-// ------------------------------------------------------------- pull stage --
-// Coordinates of the pull source for each lattice velocity component value
-// (index c+1): source = x - c.  Fused mode wraps periodic dimensions and
-// clamps non-periodic ones (the clamped load is replaced by the wall value).
-struct Src {
-    long long o[3][3];    // per dimension, per (c+1): element offset
-};
-
-template <int D, bool HALO>
-__device__ __forceinline__ Src make_src(int x, int y, int z, const Geo& g, const View& v)
-{
-    Src s;
-    const int p[3] = {x, y, z};
-    const long long st[3] = {1, v.sy, v.sz};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-#pragma unroll
-        for (int ci = 0; ci < 3; ++ci) {
-            const int c = ci - 1;
-            int q = p[d] - c;
-            if (d >= D) q = p[d];
-            else if (!HALO) {
-                if (q < 0) q = g.per[d] ? g.n[d] - 1 : 0;
-                else if (q >= g.n[d]) q = g.per[d] ? 0 : g.n[d] - 1;
-            }
-            s.o[d][ci] = (long long)q * st[d];
-        }
-    }
-    return s;
-}
-
-// Ghost value f(g, q) for g = x - c_q outside a non-periodic face, evaluated
-// from fin's valid cells as the reference's BCManager would have written it
-// (src/LBM/BoundaryConditions.H:124-189 and the src/LBM/BCTypes/ headers).
-template <class L, int q>
-__device__ __forceinline__ double wall_value(const double* __restrict__ fin, long long self,
-                                             const View& v, const Geo& g, int idx)
-{
-    const int t = g.rtype[idx];
-    if (t == MFA_BC_MOVING_WALL || t == MFA_BC_NO_SLIP || t == MFA_BC_INFLOW) {
-        constexpr int m = L::opp(q);
-        return fin[m * v.sq + self] +
-               bb_contrib<L, m>(g.rrho[idx], g.ru[idx][0], g.ru[idx][1], g.ru[idx][2]);
-    }
-    if (t == MFA_BC_OUTFLOW) return fin[q * v.sq + self];
-    if (t == MFA_BC_PRESSURE_OUTLET) {
-        // src/LBM/BCTypes/PressureOutletKernel.H:29-64
-        double rho = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
-#pragma unroll
-        for (int m = 0; m < L::Q; ++m) {
-            const double fm = fin[m * v.sq + self];
-            rho += fm;
-            j0 += L::c(m, 0) * fm;
-            j1 += L::c(m, 1) * fm;
-            if constexpr (L::D == 3) j2 += L::c(m, 2) * fm;
-        }
-        const double inv_rho = 1.0 / rho;
-        const double cu = cdot<L, q>(j0, j1, j2) * inv_rho;
-        double uu;
-        if constexpr (L::D == 2) uu = (j0 * j0 + j1 * j1) * inv_rho * inv_rho;
-        else uu = (j0 * j0 + j1 * j1 + j2 * j2) * inv_rho * inv_rho;
-        const double feq_even =
-            L::w(q) * g.rrho[idx] * (1.0 + 0.5 * cu * cu * 3.0 * 3.0 - 0.5 * uu * 3.0);
-        constexpr int m = L::opp(q);
-        return -fin[m * v.sq + self] + 2.0 * feq_even;
-    }
-    return __builtin_nan("");
-}
-
-template <class L, int q>
-__device__ __forceinline__ void fix_wall(double& fq, const double* __restrict__ fin, long long self,
-                                         const View& v, const Geo& g, const int lo[3],
-                                         const int hi[3])
-{
-    int code[3] = {0, 0, 0};
-    bool wall = false;
-#pragma unroll
-    for (int d = 0; d < L::D; ++d) {
-        const int c = L::c(q, d);
-        if (c == 1 && lo[d]) { code[d] = -1; wall = true; }
-        if (c == -1 && hi[d]) { code[d] = 1; wall = true; }
-    }
-    if (wall) {
-        const int idx = (code[0] + 1) + 3 * (code[1] + 1) + 9 * (code[2] + 1) * (L::D == 3);
-        fq = wall_value<L, q>(fin, self, v, g, idx);
-    }
-}
-
-template <class L, int q>
-struct Unroll {
-    template <class Fn>
-    static __device__ __forceinline__ void run(Fn&& fn)
-    {
-        fn(std::integral_constant<int, q>{});
-        if constexpr (q + 1 < L::Q) Unroll<L, q + 1>::run(fn);
-    }
-};
-
-template <class L, class Fn>
-__device__ __forceinline__ void for_q(Fn&& fn) { Unroll<L, 0>::run(fn); }
-
-// Pull the Q populations of cell (x,y,z).  mode: 0 fused (inline wrap and
-// walls), 1 halo (reads ghost cells), 2 unstreamed (the cell's own values).
-// WALLS: some dimension is non-periodic (compile-time, so the periodic
-// kernel carries no boundary code).  NT: non-temporal loads.
-template <bool NT>
-__device__ __forceinline__ double ld(const double* p)
-{
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-
-template <class L, int MODE, bool WALLS = true, bool NT = false>
-__device__ __forceinline__ void pull(double (&f)[L::Q], const double* __restrict__ fin, int x, int y,
-                                     int z, const Geo& g, const View& v)
-{
-    const long long self = x + y * v.sy + z * v.sz;
-    if constexpr (MODE == 2) {
-        for_q<L>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            f[q] = ld<NT>(&fin[q * v.sq + self]);
-        });
-        return;
-    } else {
-        const Src s = make_src<L::D, MODE == 1>(x, y, z, g, v);
-        for_q<L>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            const long long off = s.o[0][L::c(q, 0) + 1] + s.o[1][L::c(q, 1) + 1] +
-                                  s.o[2][L::c(q, 2) + 1];
-            f[q] = ld<NT>(&fin[q * v.sq + off]);
-        });
-        if constexpr (MODE == 0 && WALLS) {
-            const int p[3] = {x, y, z};
-            int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-            bool edge = false;
-#pragma unroll
-            for (int d = 0; d < L::D; ++d) {
-                lo[d] = !g.per[d] && p[d] == 0;
-                hi[d] = !g.per[d] && p[d] == g.n[d] - 1;
-                edge = edge || lo[d] || hi[d];
-            }
-            if (edge) {
-                for_q<L>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    fix_wall<L, q>(f[q], fin, self, v, g, lo, hi);
-                });
-            }
-        }
-    }
-}
-
-// rho and u of Q populations: src/LBM/Macroscopic.H:36-45,115-127,186-201.
-template <class L, bool FORCED>
-__device__ __forceinline__ void moments(const double (&f)[L::Q], const double F[3], double& rho,
-                                        double& u0, double& u1, double& u2)
-{
-    rho = 0.;
-    double j0 = 0., j1 = 0., j2 = 0.;
-    for_q<L>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        rho += f[q];
-    });
-    for_q<L>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if constexpr (L::c(q, 0) != 0) j0 = (L::c(q, 0) > 0) ? j0 + f[q] : j0 - f[q];
-        if constexpr (L::c(q, 1) != 0) j1 = (L::c(q, 1) > 0) ? j1 + f[q] : j1 - f[q];
-        if constexpr (L::D == 3 && L::c(q, 2) != 0) j2 = (L::c(q, 2) > 0) ? j2 + f[q] : j2 - f[q];
-    });
-    if constexpr (FORCED) {
-        u0 = (j0 + 0.5 * F[0]) / rho;
-        u1 = (j1 + 0.5 * F[1]) / rho;
-        u2 = (L::D == 3) ? (j2 + 0.5 * F[2]) / rho : 0.0;
-    } else {
-        u0 = j0 / rho;
-        u1 = j1 / rho;
-        u2 = (L::D == 3) ? j2 / rho : 0.0;
-    }
-}
 
 // ----------------------------------------------------------- step kernel --
-// BGK (+Guo) collision of pulled populations and store:
-// src/LBM/Collision.cpp:24-53 (plain) and :62-92 (forced).
-template <class L, bool FORCED>
-__device__ __forceinline__ void collide_store(const double (&f)[L::Q], double* __restrict__ fout,
-                                              long long self, const View& vo, const double omega,
-                                              const double F0, const double F1, const double F2)
-{
-    const double F[3] = {F0, F1, F2};
-    double rho, u0, u1, u2;
-    moments<L, FORCED>(f, F, rho, u0, u1, u2);
-    const double usq = usq_of<L>(u0, u1, u2);
-    const double omw = 1 - omega;
-    double pref = 0.0, uF = 0.0;
-    if constexpr (FORCED) {
-        pref = 1.0 - 0.5 * omega;
-        uF = (L::D == 2) ? u0 * F0 + u1 * F1 : u0 * F0 + u1 * F1 + u2 * F2;
-    }
-    for_q<L>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        const double fe = feq<L, q>(rho, u0, u1, u2, usq);
-        double out = omw * f[q] + omega * fe;
-        if constexpr (FORCED) out = out + guo<L, q>(u0, u1, u2, F0, F1, F2, pref, uF);
-        __builtin_nontemporal_store(out, &fout[q * vo.sq + self]);
-    });
-}
-
 // One thread per cell column of NC cells along z; block = (bx, by);
 // grid = (ceil(nx/bx), ceil(ny/by), ceil(nz/NC)).  All NC*Q loads of a
 // thread are issued before the first collision so they are in flight
@@ -545,6 +318,17 @@ __global__ __launch_bounds__(256) void k_copy(const dbl2* __restrict__ a, dbl2* 
     }
 }
 
+// reduced-precision storage (mfa_lowprec.hip)
+int step_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, double omega,
+             const double* force, hipStream_t st);
+int moments_dev(const mfa_field* f, mfa_field* rho, mfa_field* vel, const Geo& g, int lat,
+                const double* force, int streamed, hipStream_t st);
+int init_dev(mfa_field* rho, mfa_field* vel, mfa_field* f, const Geo& g, int lat,
+             const double* force, hipStream_t st);
+int convert_dev(const mfa_field* src, mfa_field* dst, const Geo& g, int lat, hipStream_t st);
+int reduce_dev(const mfa_field* f, const Geo& g, int lat, int what, double* part, int nblocks,
+               hipStream_t st);
+
 // -------------------------------------------------------------- helpers ---
 static int lattice_dim(int lat) { return lat == MFA_D2Q9 ? 2 : 3; }
 static int lattice_q(int lat)
@@ -558,12 +342,15 @@ static int lattice_q(int lat)
     }
 }
 
-static int check_field(const mfa_field* f, int ncomp_min, int need_grow, const char* what)
+static int check_field(const mfa_field* f, int ncomp_min, int need_grow, const char* what,
+                       bool allow_reduced = false)
 {
     if (!f || !f->data) return fail(MFA_ERR_INVALID, std::string(what) + ": null field");
     if (f->n[0] < 1 || f->n[1] < 1 || f->n[2] < 1)
         return fail(MFA_ERR_INVALID, std::string(what) + ": empty box");
-    if (f->dtype != MFA_F64)
+    if (f->dtype < MFA_F64 || f->dtype > MFA_BF16DEV)
+        return fail(MFA_ERR_INVALID, std::string(what) + ": unknown dtype");
+    if (f->dtype != MFA_F64 && !allow_reduced)
         return fail(MFA_ERR_UNSUPPORTED, std::string(what) + ": dtype not supported by this entry point");
     if (f->ncomp < ncomp_min)
         return fail(MFA_ERR_INVALID, std::string(what) + ": too few components");
@@ -816,13 +603,16 @@ int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, double
     const int Q = lattice_q(lat);
     if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_step: unknown lattice");
     int rc;
-    if ((rc = check_field(fin, Q, 0, "mfa_step fin")) || (rc = check_field(fout, Q, 0, "mfa_step fout")))
+    if ((rc = check_field(fin, Q, 0, "mfa_step fin", true)) ||
+        (rc = check_field(fout, Q, 0, "mfa_step fout", true)))
         return rc;
     if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
         return fail(MFA_ERR_INVALID, "mfa_step: fin/fout boxes differ");
     if (fin->data == fout->data) return fail(MFA_ERR_INVALID, "mfa_step: fin aliases fout");
+    if (fin->dtype != fout->dtype) return fail(MFA_ERR_INVALID, "mfa_step: fin/fout storage types differ");
     if ((rc = check_2d(fin, lat))) return rc;
     const Geo geo = geo_of(*fin, g, lat);
+    if (fin->dtype != MFA_F64) return step_dev(fin, fout, geo, lat, omega, force, (hipStream_t)stream);
     return with_lattice(lat, [&](auto L) {
         return launch_step<decltype(L)>(fin, fout, geo, omega, force, 0, (hipStream_t)stream, lat);
     });
@@ -839,6 +629,21 @@ int mfa_stream_copy(const void* src, void* dst, size_t bytes, int nontemporal, v
         hipLaunchKernelGGL(k_copy<false>, grid, dim3(256), 0, (hipStream_t)stream, (const dbl2*)src, (dbl2*)dst, n2);
     MFA_HIP_CHECK(hipGetLastError());
     return MFA_OK;
+}
+
+int mfa_convert(const mfa_field* src, mfa_field* dst, int lattice, void* stream)
+{
+    const int Q = lattice_q(lattice);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_convert: unknown lattice");
+    int rc;
+    if ((rc = check_field(src, Q, 0, "mfa_convert src", true)) ||
+        (rc = check_field(dst, Q, 0, "mfa_convert dst", true)))
+        return rc;
+    if (std::memcmp(src->n, dst->n, sizeof(src->n))) return fail(MFA_ERR_INVALID, "mfa_convert: boxes differ");
+    if ((src->dtype == MFA_F64) == (dst->dtype == MFA_F64))
+        return fail(MFA_ERR_INVALID, "mfa_convert: exactly one side must be FP64");
+    if ((rc = check_2d(src, lattice))) return rc;
+    return convert_dev(src, dst, geo_of(*src, nullptr, lattice), lattice, (hipStream_t)stream);
 }
 
 int mfa_set_launch(int lattice, const mfa_launch* cfg)
@@ -930,13 +735,15 @@ int mfa_moments(const mfa_field* f, mfa_field* rho, mfa_field* vel, const mfa_ge
     if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_moments: unknown lattice");
     const int D = lattice_dim(lat);
     int rc;
-    if ((rc = check_field(f, Q, 0, "mfa_moments f")) || (rc = check_field(rho, 1, 0, "mfa_moments rho")) ||
+    if ((rc = check_field(f, Q, 0, "mfa_moments f", true)) || (rc = check_field(rho, 1, 0, "mfa_moments rho")) ||
         (rc = check_field(vel, D, 0, "mfa_moments vel")))
         return rc;
     if (std::memcmp(f->n, rho->n, sizeof(f->n)) || std::memcmp(f->n, vel->n, sizeof(f->n)))
         return fail(MFA_ERR_INVALID, "mfa_moments: boxes differ");
     if ((rc = check_2d(f, lat))) return rc;
     const Geo geo = geo_of(*f, g, lat);
+    if (f->dtype != MFA_F64)
+        return moments_dev(f, rho, vel, geo, lat, force, streamed, (hipStream_t)stream);
     const View vi = view_of(*f), vr = view_of(*rho), vv = view_of(*vel);
     const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0,
                  F2 = (force && D == 3) ? force[2] : 0.0;
@@ -968,7 +775,7 @@ int mfa_init_equilibrium(mfa_field* rho, mfa_field* vel, mfa_field* f, int latti
     if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_init_equilibrium: unknown lattice");
     const int D = lattice_dim(lattice);
     int rc;
-    if ((rc = check_field(f, Q, 0, "mfa_init_equilibrium f")) ||
+    if ((rc = check_field(f, Q, 0, "mfa_init_equilibrium f", true)) ||
         (rc = check_field(rho, 1, 0, "mfa_init_equilibrium rho")) ||
         (rc = check_field(vel, D, 0, "mfa_init_equilibrium vel")))
         return rc;
@@ -976,6 +783,7 @@ int mfa_init_equilibrium(mfa_field* rho, mfa_field* vel, mfa_field* f, int latti
         return fail(MFA_ERR_INVALID, "mfa_init_equilibrium: boxes differ");
     if ((rc = check_2d(f, lattice))) return rc;
     const Geo geo = geo_of(*f, nullptr, lattice);
+    if (f->dtype != MFA_F64) return init_dev(rho, vel, f, geo, lattice, force, (hipStream_t)stream);
     const View vr = view_of(*rho), vv = view_of(*vel), vo = view_of(*f);
     const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0,
                  F2 = (force && D == 3) ? force[2] : 0.0;
@@ -999,7 +807,7 @@ int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void*
     if (!out_dev) return fail(MFA_ERR_INVALID, "mfa_reduce: null output");
     int rc;
     const int need = (what == MFA_RED_COMP_SUM) ? 1 : Q;
-    if ((rc = check_field(f, need, 0, "mfa_reduce f"))) return rc;
+    if ((rc = check_field(f, need, 0, "mfa_reduce f", true))) return rc;
     if (what == MFA_RED_COMP_SUM && f->ncomp > kRedMaxOut)
         return fail(MFA_ERR_INVALID, "mfa_reduce: more than 27 components");
     double* ws = nullptr;
@@ -1008,6 +816,13 @@ int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void*
     const View v = view_of(*f);
     hipStream_t st = (hipStream_t)stream;
     int nv = 0;
+    if (f->dtype != MFA_F64) {
+        if ((rc = reduce_dev(f, geo, lattice, what, ws, kRedBlocks, st))) return rc;
+        nv = (what == MFA_RED_MASS) ? 1 : lattice_dim(lattice);
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, ws, kRedBlocks, nv, out_dev);
+        MFA_HIP_CHECK(hipGetLastError());
+        return MFA_OK;
+    }
     rc = with_lattice(lattice, [&](auto L) {
         using LT = decltype(L);
         if (what == MFA_RED_MASS) {

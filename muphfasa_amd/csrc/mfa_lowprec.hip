@@ -1,0 +1,326 @@
+// mfa_lowprec.hip -- reduced-precision population storage for the step
+// engine.  Populations are stored as the shifted deviation d_q = f_q - w_q in
+// FP32, FP16 or BF16 and collided in FP32; global sums stay FP64.
+//
+// Storing the deviation instead of f keeps the stored magnitude near the
+// size of the flow perturbation (|d_q| ~ w_q * 3 |c.u| ~ 1e-2 for u ~ 0.05)
+// rather than w_q ~ 0.3, so the same mantissa resolves the dynamics 10-100x
+// more finely; the equilibrium is evaluated directly in deviation form,
+//   feq_q - w_q = w_q * (drho + rho * (3 c.u + 4.5 (c.u)^2 - 1.5 u.u)),
+// so w_q never enters the arithmetic and is never lost to rounding.
+// Conversions to FP16/BF16 round to nearest even.  Bytes per lattice update:
+// 2*Q*sizeof(T), i.e. 152 B (FP32) and 76 B (FP16/BF16) for D3Q19 versus
+// 304 B in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "kernels_lowprec.hpp"
+
+namespace mfa {
+
+template <class L, bool FORCED, bool WALLS, class T>
+__global__ __launch_bounds__(256) void k_step_dev(const T* __restrict__ fin, T* __restrict__ fout,
+                                                  const Geo g, const View vi, const View vo,
+                                                  const float omega, const float F0, const float F1,
+                                                  const float F2)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+    float d[L::Q];
+    pull_dev<L, WALLS>(d, fin, x, y, z, g, vi);
+    collide_store_dev<L, FORCED>(d, fout, x + y * vo.sy + z * vo.sz, vo, omega, F0, F1, F2);
+}
+
+// Moments of reduced storage, evaluated in FP64 from the dequantised
+// populations w_q + d_q with the reference's arithmetic.
+template <class L, bool FORCED, bool STREAMED, class T>
+__global__ __launch_bounds__(256) void k_moments_dev(const T* __restrict__ fin, double* __restrict__ rho_out,
+                                                     double* __restrict__ vel_out, const Geo g,
+                                                     const View vi, const View vr, const View vv,
+                                                     const double F0, const double F1, const double F2)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+    float d[L::Q];
+    if constexpr (STREAMED) {
+        pull_dev<L, true>(d, fin, x, y, z, g, vi);
+    } else {
+        const long long self = x + y * vi.sy + z * vi.sz;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            d[q] = to_f(fin[q * vi.sq + self]);
+        });
+    }
+    double f[L::Q];
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = L::w(q) + (double)d[q];
+    });
+    const double F[3] = {F0, F1, F2};
+    double rho, u0, u1, u2;
+    moments<L, FORCED>(f, F, rho, u0, u1, u2);
+    rho_out[x + y * vr.sy + z * vr.sz] = rho;
+    const long long sv = x + y * vv.sy + z * vv.sz;
+    vel_out[sv] = u0;
+    vel_out[sv + vv.sq] = u1;
+    if constexpr (L::D == 3) vel_out[sv + 2 * vv.sq] = u2;
+}
+
+// Equilibrium initialisation into reduced storage: feq in FP64 with the
+// reference's arithmetic, stored as T(feq - w); with a force the reported
+// rho/u are recomputed from the stored values.
+template <class L, bool FORCED, class T>
+__global__ __launch_bounds__(256) void k_init_dev(double* __restrict__ rho_io, double* __restrict__ vel_io,
+                                                  T* __restrict__ fout, const Geo g, const View vr,
+                                                  const View vv, const View vo, const double F0,
+                                                  const double F1, const double F2)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+    const long long cr = x + y * vr.sy + z * vr.sz;
+    const long long cv = x + y * vv.sy + z * vv.sz;
+    const double rho = rho_io[cr];
+    double u0 = vel_io[cv], u1 = vel_io[cv + vv.sq];
+    double u2 = (L::D == 3) ? vel_io[cv + 2 * vv.sq] : 0.0;
+    if constexpr (FORCED) {
+        u0 -= 0.5 * F0 / rho;
+        u1 -= 0.5 * F1 / rho;
+        if constexpr (L::D == 3) u2 -= 0.5 * F2 / rho;
+    }
+    const double usq = usq_of<L>(u0, u1, u2);
+    double f[L::Q];
+    const long long self = x + y * vo.sy + z * vo.sz;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const double fe = feq<L, q>(rho, u0, u1, u2, usq);
+        const T sv = from_f<T>((float)(fe - L::w(q)));
+        fout[q * vo.sq + self] = sv;
+        f[q] = L::w(q) + (double)to_f(sv);
+    });
+    if constexpr (FORCED) {
+        const double F[3] = {F0, F1, F2};
+        double r2, v0, v1, v2;
+        moments<L, true>(f, F, r2, v0, v1, v2);
+        rho_io[cr] = r2;
+        vel_io[cv] = v0;
+        vel_io[cv + vv.sq] = v1;
+        if constexpr (L::D == 3) vel_io[cv + 2 * vv.sq] = v2;
+    }
+}
+
+// Conversion between FP64 populations and reduced storage (valid cells).
+template <class L, class T, bool TO_DEV>
+__global__ __launch_bounds__(256) void k_convert(const void* __restrict__ src, void* __restrict__ dst,
+                                                 const Geo g, const View vs, const View vd)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const long long is = x + y * vs.sy + z * vs.sz + q * vs.sq;
+        const long long id = x + y * vd.sy + z * vd.sz + q * vd.sq;
+        if constexpr (TO_DEV) ((T*)dst)[id] = from_f<T>((float)(((const double*)src)[is] - L::w(q)));
+        else ((double*)dst)[id] = L::w(q) + (double)to_f(((const T*)src)[is]);
+    });
+}
+
+// FP64 mass / momentum of reduced storage (per-block partials, see
+// k_reduce_final in mfa_lbm.hip for the deterministic final pass).
+template <class L, int WHAT, class T>
+__global__ __launch_bounds__(256) void k_reduce_dev(const T* __restrict__ f, const Geo g, const View v,
+                                                    double* __restrict__ part)
+{
+    double acc[3] = {0.0, 0.0, 0.0};
+    const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(c % g.n[0]);
+        const long long r = c / g.n[0];
+        const int y = (int)(r % g.n[1]);
+        const int z = (int)(r / g.n[1]);
+        const long long self = x + y * v.sy + z * v.sz;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const double d = (double)to_f(f[q * v.sq + self]);
+            if constexpr (WHAT == MFA_RED_MASS) acc[0] += L::w(q) + d;
+            else {
+                if constexpr (L::c(q, 0) != 0) acc[0] += L::c(q, 0) * d;
+                if constexpr (L::c(q, 1) != 0) acc[1] += L::c(q, 1) * d;
+                if constexpr (L::c(q, 2) != 0) acc[2] += L::c(q, 2) * d;
+            }
+        });
+    }
+    // block partial sums (fixed order)
+    __shared__ double sh[4][3];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double s = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) sh[wid][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double s = 0.0;
+        for (int w = 0; w < 4; ++w) s += sh[w][threadIdx.x];
+        part[blockIdx.x * 27 + threadIdx.x] = s;
+    }
+}
+
+// ------------------------------------------------------------ dispatch ----
+template <class Fn>
+static int with_lattice_dev(int lat, Fn&& fn)
+{
+    switch (lat) {
+    case MFA_D2Q9: return fn(Lattice<MFA_D2Q9>{});
+    case MFA_D3Q15: return fn(Lattice<MFA_D3Q15>{});
+    case MFA_D3Q19: return fn(Lattice<MFA_D3Q19>{});
+    case MFA_D3Q27: return fn(Lattice<MFA_D3Q27>{});
+    default: return fail(MFA_ERR_INVALID, "unknown lattice id");
+    }
+}
+
+template <class Fn>
+static int with_store_type(int dtype, Fn&& fn)
+{
+    switch (dtype) {
+    case MFA_F32: return fn(float{});
+    case MFA_F16DEV: return fn(f16_t{});
+    case MFA_BF16DEV: return fn(bf16_t{});
+    default: return fail(MFA_ERR_UNSUPPORTED, "reduced-precision path: unknown storage type");
+    }
+}
+
+static void shape(const int n[3], dim3& grid, dim3& block)
+{
+    const int bx = n[0] <= 64 ? 64 : (n[0] <= 128 ? 128 : 256);
+    block = dim3(bx, 256 / bx, 1);
+    grid = dim3((n[0] + bx - 1) / bx, (n[1] + block.y - 1) / block.y, n[2]);
+}
+
+int step_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, double omega,
+             const double* force, hipStream_t st)
+{
+    const View vi{fin->sy, fin->sz, fin->sq}, vo{fout->sy, fout->sz, fout->sq};
+    return with_lattice_dev(lat, [&](auto Lc) {
+        using L = decltype(Lc);
+        return with_store_type(fin->dtype, [&](auto tv) {
+            using T = decltype(tv);
+            dim3 grid, block;
+            shape(g.n, grid, block);
+            const float om = (float)omega;
+            const float F0 = force ? (float)force[0] : 0.f, F1 = force ? (float)force[1] : 0.f,
+                        F2 = (force && L::D == 3) ? (float)force[2] : 0.f;
+            const T* a = (const T*)fin->data;
+            T* b = (T*)fout->data;
+            if (force) {
+                if (g.walls) hipLaunchKernelGGL((k_step_dev<L, true, true, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
+                else hipLaunchKernelGGL((k_step_dev<L, true, false, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
+            } else {
+                if (g.walls) hipLaunchKernelGGL((k_step_dev<L, false, true, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
+                else hipLaunchKernelGGL((k_step_dev<L, false, false, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
+            }
+            MFA_HIP_CHECK(hipGetLastError());
+            return (int)MFA_OK;
+        });
+    });
+}
+
+int moments_dev(const mfa_field* f, mfa_field* rho, mfa_field* vel, const Geo& g, int lat,
+                const double* force, int streamed, hipStream_t st)
+{
+    const View vi{f->sy, f->sz, f->sq}, vr{rho->sy, rho->sz, rho->sq}, vv{vel->sy, vel->sz, vel->sq};
+    return with_lattice_dev(lat, [&](auto Lc) {
+        using L = decltype(Lc);
+        return with_store_type(f->dtype, [&](auto tv) {
+            using T = decltype(tv);
+            dim3 grid, block;
+            shape(g.n, grid, block);
+            const double F0 = force ? force[0] : 0., F1 = force ? force[1] : 0.,
+                         F2 = (force && L::D == 3) ? force[2] : 0.;
+            const T* a = (const T*)f->data;
+            double* r = (double*)rho->data;
+            double* u = (double*)vel->data;
+            if (force) {
+                if (streamed) hipLaunchKernelGGL((k_moments_dev<L, true, true, T>), grid, block, 0, st, a, r, u, g, vi, vr, vv, F0, F1, F2);
+                else hipLaunchKernelGGL((k_moments_dev<L, true, false, T>), grid, block, 0, st, a, r, u, g, vi, vr, vv, F0, F1, F2);
+            } else {
+                if (streamed) hipLaunchKernelGGL((k_moments_dev<L, false, true, T>), grid, block, 0, st, a, r, u, g, vi, vr, vv, F0, F1, F2);
+                else hipLaunchKernelGGL((k_moments_dev<L, false, false, T>), grid, block, 0, st, a, r, u, g, vi, vr, vv, F0, F1, F2);
+            }
+            MFA_HIP_CHECK(hipGetLastError());
+            return (int)MFA_OK;
+        });
+    });
+}
+
+int init_dev(mfa_field* rho, mfa_field* vel, mfa_field* f, const Geo& g, int lat, const double* force,
+             hipStream_t st)
+{
+    const View vr{rho->sy, rho->sz, rho->sq}, vv{vel->sy, vel->sz, vel->sq}, vo{f->sy, f->sz, f->sq};
+    return with_lattice_dev(lat, [&](auto Lc) {
+        using L = decltype(Lc);
+        return with_store_type(f->dtype, [&](auto tv) {
+            using T = decltype(tv);
+            dim3 grid, block;
+            shape(g.n, grid, block);
+            const double F0 = force ? force[0] : 0., F1 = force ? force[1] : 0.,
+                         F2 = (force && L::D == 3) ? force[2] : 0.;
+            if (force) hipLaunchKernelGGL((k_init_dev<L, true, T>), grid, block, 0, st, (double*)rho->data, (double*)vel->data, (T*)f->data, g, vr, vv, vo, F0, F1, F2);
+            else hipLaunchKernelGGL((k_init_dev<L, false, T>), grid, block, 0, st, (double*)rho->data, (double*)vel->data, (T*)f->data, g, vr, vv, vo, F0, F1, F2);
+            MFA_HIP_CHECK(hipGetLastError());
+            return (int)MFA_OK;
+        });
+    });
+}
+
+int convert_dev(const mfa_field* src, mfa_field* dst, const Geo& g, int lat, hipStream_t st)
+{
+    const View vs{src->sy, src->sz, src->sq}, vd{dst->sy, dst->sz, dst->sq};
+    const bool to_dev = src->dtype == MFA_F64;
+    const int dt = to_dev ? dst->dtype : src->dtype;
+    return with_lattice_dev(lat, [&](auto Lc) {
+        using L = decltype(Lc);
+        return with_store_type(dt, [&](auto tv) {
+            using T = decltype(tv);
+            dim3 grid, block;
+            shape(g.n, grid, block);
+            if (to_dev) hipLaunchKernelGGL((k_convert<L, T, true>), grid, block, 0, st, src->data, dst->data, g, vs, vd);
+            else hipLaunchKernelGGL((k_convert<L, T, false>), grid, block, 0, st, src->data, dst->data, g, vs, vd);
+            MFA_HIP_CHECK(hipGetLastError());
+            return (int)MFA_OK;
+        });
+    });
+}
+
+int reduce_dev(const mfa_field* f, const Geo& g, int lat, int what, double* part, int nblocks,
+               hipStream_t st)
+{
+    const View v{f->sy, f->sz, f->sq};
+    return with_lattice_dev(lat, [&](auto Lc) {
+        using L = decltype(Lc);
+        return with_store_type(f->dtype, [&](auto tv) {
+            using T = decltype(tv);
+            if (what == MFA_RED_MASS)
+                hipLaunchKernelGGL((k_reduce_dev<L, MFA_RED_MASS, T>), dim3(nblocks), dim3(256), 0, st, (const T*)f->data, g, v, part);
+            else if (what == MFA_RED_MOMENTUM)
+                hipLaunchKernelGGL((k_reduce_dev<L, MFA_RED_MOMENTUM, T>), dim3(nblocks), dim3(256), 0, st, (const T*)f->data, g, v, part);
+            else
+                return fail(MFA_ERR_UNSUPPORTED, "mfa_reduce: per-component sums need FP64 storage");
+            MFA_HIP_CHECK(hipGetLastError());
+            return (int)MFA_OK;
+        });
+    });
+}
+
+}  // namespace mfa

@@ -320,6 +320,41 @@ void orc_step(int lattice, const double* fin, double* fout, const int* n,
             }
 }
 
+/* ---- solid obstacles (no reference counterpart: the reference has no
+ *      interior solids).  Half-way bounce-back: a fluid cell pulling from a
+ *      solid neighbour takes its own post-collision population pointing into
+ *      the solid, f_q(x) = f*_opp(q)(x); solid cells hold w_q.  Pull sources
+ *      outside the domain come from the ghost layer as in orc_step.
+ *      mask: (n2,n1,n0) uint8, 1 = solid. ---------------------------------- */
+void orc_step_masked(int lattice, const double* fin, double* fout, const int* n, double omega,
+                     const unsigned char* mask)
+{
+    const orc_lattice* L = &k_lat[lattice];
+    fab_t fb = make_fab(L, n);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const long c = i + (long)n[0] * (j + (long)n[1] * k);
+                if (mask[c]) {
+                    for (int q = 0; q < L->Q; ++q) fout[fidx(&fb, i, j, k, q)] = L->w[q];
+                    continue;
+                }
+                double f[27], u[3], rho;
+                for (int q = 0; q < L->Q; ++q) {
+                    const int si = i - L->cx[q], sj = j - L->cy[q], sk = k - L->cz[q];
+                    const int inside = si >= 0 && si < n[0] && sj >= 0 && sj < n[1] && sk >= 0 &&
+                                       sk < n[2];
+                    if (inside && mask[si + (long)n[0] * (sj + (long)n[1] * sk)])
+                        f[q] = fin[fidx(&fb, i, j, k, L->opp[q])];
+                    else
+                        f[q] = fin[fidx(&fb, si, sj, sk, q)];
+                }
+                cell_moments(L, f, NULL, &rho, u);
+                for (int q = 0; q < L->Q; ++q)
+                    fout[fidx(&fb, i, j, k, q)] = bgk(f[q], feq_cell(L, rho, u, q), omega);
+            }
+}
+
 /* ---- periodic ghost fill (FillBoundary(periodicity) on a single box,
  *      src/Simulation/SimulationSetup.cpp:227): a ghost is filled iff every
  *      dimension in which it lies outside the domain is periodic. --------- */

@@ -51,6 +51,8 @@ int orc_has_conflict(int nfaces, const int* types);
 /* field-level operations on ghosted fabs */
 void orc_step(int lattice, const double* fin, double* fout, const int* n,
               double omega, const double* F, int nthreads);
+void orc_step_masked(int lattice, const double* fin, double* fout, const int* n, double omega,
+                     const unsigned char* mask);
 void orc_fill_periodic(int lattice, double* f, const int* n, const int* periodic);
 int  orc_apply_bc(int lattice, double* f, const int* n, const int* periodic,
                   const int* rtype, const double* rrho, const double* ru);

@@ -60,6 +60,8 @@ def lib():
         L.orc_step.argtypes = [ctypes.c_int, _dp, _dp, _ip, ctypes.c_double, _dp,
                                ctypes.c_int]
         L.orc_fill_periodic.argtypes = [ctypes.c_int, _dp, _ip, _ip]
+        L.orc_step_masked.argtypes = [ctypes.c_int, _dp, _dp, _ip, ctypes.c_double,
+                                      ctypes.c_void_p]
         L.orc_apply_bc.restype = ctypes.c_int
         L.orc_apply_bc.argtypes = [ctypes.c_int, _dp, _ip, _ip, _ip, _dp, _dp]
         L.orc_stream.argtypes = [ctypes.c_int, _dp, _dp, _ip]
@@ -191,6 +193,23 @@ def resolve_bc(dim, face_type, face_rho=None, face_u=None):
 def step(lat, fin, fout, n, omega, F=None, nthreads=1):
     Fa, Fp = _force(F)
     lib().orc_step(lat.id, _d(fin), _d(fout), _i(n3(n)), omega, Fp, nthreads)
+
+
+def step_masked(lat, fin, fout, n, omega, mask):
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    lib().orc_step_masked(lat.id, _d(fin), _d(fout), _i(n3(n)), omega, m.ctypes.data)
+
+
+def run_masked(lat, f, n, omega, periodic, resolved, mask, nsteps):
+    """advance() with solid obstacles: masked step, swap, ghost fill."""
+    fa = np.ascontiguousarray(f, dtype=np.float64).copy()
+    fb = np.full_like(fa, np.nan)
+    for _ in range(nsteps):
+        step_masked(lat, fa, fb, n, omega, mask)
+        fa, fb = fb, fa
+        fill_periodic(lat, fa, n, periodic)
+        apply_bc(lat, fa, n, periodic, resolved)
+    return fa
 
 
 def fill_periodic(lat, f, n, periodic):

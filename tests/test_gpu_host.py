@@ -57,7 +57,6 @@ def test_periodic_noise_init_and_steps_match_oracle():
     sim = host.Simulation(text)
     sim.initialize()
     f0 = sim.f()
-    sim.update_macroscopic(-1) if False else None
     L = O.Lattice("D3Q19")
     n = [32, 32, 32]
     # the initial populations are feq(rho, u - F/2rho) of the seeded noise
@@ -130,7 +129,6 @@ def test_closed_box_mass_conservation_with_refreshed_moments():
             "lbm.init_amp = 0.01\namr.plot_int = 0\namr.chk_int = 0\n")
     sim = host.Simulation(text)
     sim.initialize()
-    sim.update_macroscopic(-1) if False else None
     r0 = O.moments(O.Lattice("D2Q9"), _oracle_fab(O.Lattice("D2Q9"), [12, 6], sim.f()), [12, 6],
                    streamed=False)[0].sum()
     sim.advance(50)

@@ -1,0 +1,158 @@
+"""Reduced-precision storage (FP32 / FP16 / BF16 shifted deviations, FP32
+collision).
+
+* One step against a float32 reference of the deviation-form BGK+Guo update
+  written here in numpy (same inputs: the dequantised stored deviations),
+  to within a few units of the storage type's last place.
+* Round trip FP64 -> reduced -> FP64 within half an ulp of the storage type.
+* Multi-step error against the FP64 engine on a 3-D Taylor-Green vortex with
+  a stated bound per storage type, and FP64 mass bookkeeping.
+"""
+import numpy as np
+import pytest
+
+from muphfasa_amd import abi
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TORCH_DT = {"f32": "float32", "f16dev": "float16", "bf16dev": "bfloat16"}
+# unit roundoff of each storage type (relative to the stored deviation)
+EPS = {"f32": 2.0 ** -24, "f16dev": 2.0 ** -11, "bf16dev": 2.0 ** -8}
+
+
+def _dev_tensor(torch, Q, n, dt, dev):
+    return torch.empty((Q, n[2], n[1], n[0]), dtype=getattr(torch, TORCH_DT[dt]), device=dev)
+
+
+def _ref_step_f32(L, d, omega, F, periodic=True):
+    """float32 deviation-form pull + BGK (+Guo) on a periodic box."""
+    f32 = np.float32
+    Q, D = L.Q, L.D
+    pulled = np.empty_like(d)
+    for q in range(Q):
+        c = L.c[q]
+        pulled[q] = np.roll(d[q], shift=(c[2], c[1], c[0]), axis=(0, 1, 2))
+    drho = np.zeros(d.shape[1:], f32)
+    j = [np.zeros(d.shape[1:], f32) for _ in range(3)]
+    for q in range(Q):
+        drho = drho + pulled[q]
+        for a in range(D):
+            if L.c[q, a] > 0:
+                j[a] = j[a] + pulled[q]
+            elif L.c[q, a] < 0:
+                j[a] = j[a] - pulled[q]
+    rho = f32(1) + drho
+    inv = f32(1) / rho
+    Fv = [f32(F[a]) if F is not None else f32(0) for a in range(3)]
+    u = [((j[a] + f32(0.5) * Fv[a]) if F is not None else j[a]) * inv for a in range(3)]
+    if D == 2:
+        u[2] = np.zeros_like(u[0])
+    usq = u[0] * u[0] + u[1] * u[1] + (u[2] * u[2] if D == 3 else 0)
+    om = f32(omega)
+    out = np.empty_like(d)
+    uF = u[0] * Fv[0] + u[1] * Fv[1] + u[2] * Fv[2]
+    for q in range(Q):
+        cu = sum(f32(L.c[q, a]) * u[a] for a in range(D))
+        fe = f32(L.w[q]) * (drho + rho * (f32(3) * cu + f32(4.5) * cu * cu - f32(1.5) * usq))
+        o = (f32(1) - om) * pulled[q] + om * fe
+        if F is not None:
+            cF = sum(f32(L.c[q, a]) * Fv[a] for a in range(D))
+            o = o + (f32(1) - f32(0.5) * om) * f32(L.w[q]) * f32(3) * (cF + f32(3) * cu * cF - uF)
+        out[q] = o
+    return out
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+@pytest.mark.parametrize("dt", ["f32", "f16dev", "bf16dev"])
+@pytest.mark.parametrize("forced", [False, True])
+def test_one_step_matches_float32_reference(dev, lat, dt, forced):
+    import torch
+    L = O.Lattice(lat)
+    n = [24, 10, 7] if L.D == 3 else [40, 17, 1]
+    rng = np.random.default_rng(3)
+    f64 = L.w[:, None, None, None] * (1 + 0.05 * rng.standard_normal((L.Q, n[2], n[1], n[0])))
+    src = torch.from_numpy(f64).to(dev)
+    a = _dev_tensor(torch, L.Q, n, dt, dev)
+    b = torch.empty_like(a)
+    D = L.D
+    fa = abi.field(a, n, 0, D, abi.DTYPE[dt])
+    fb = abi.field(b, n, 0, D, abi.DTYPE[dt])
+    abi.convert(abi.field(src, n, 0, D), fa, abi.LATTICE[lat])
+    g = abi.resolve_bc(abi.LATTICE[lat], [abi.BC["Periodic"]] * 6)
+    F = [2e-4, -1e-4, 5e-5][:D] if forced else None
+    abi.step(fa, fb, g, 1.0 / 0.6, F)
+    torch.cuda.synchronize()
+    d_in = a.float().cpu().numpy()
+    expect = _ref_step_f32(L, d_in, 1.0 / 0.6, F)
+    got = b.float().cpu().numpy()
+    # result rounded once to the storage type: |got - expect| <= ~1 ulp(T) of
+    # the deviation plus float32 evaluation-order noise
+    scale = np.abs(expect) + 1e-3 * L.w[:, None, None, None]
+    err = np.abs(got - expect) / scale
+    assert err.max() <= 2.5 * EPS[dt] + 4 * 2.0 ** -24, (dt, err.max())
+
+
+@pytest.mark.parametrize("dt", ["f32", "f16dev", "bf16dev"])
+def test_convert_round_trip(dev, dt):
+    import torch
+    L = O.Lattice("D3Q19")
+    n = [16, 8, 4]
+    rng = np.random.default_rng(9)
+    f64 = L.w[:, None, None, None] * (1 + 0.02 * rng.standard_normal((L.Q, n[2], n[1], n[0])))
+    src = torch.from_numpy(f64).to(dev)
+    a = _dev_tensor(torch, L.Q, n, dt, dev)
+    back = torch.empty_like(src)
+    abi.convert(abi.field(src, n, 0, 3), abi.field(a, n, 0, 3, abi.DTYPE[dt]), 2)
+    abi.convert(abi.field(a, n, 0, 3, abi.DTYPE[dt]), abi.field(back, n, 0, 3), 2)
+    torch.cuda.synchronize()
+    dev_true = f64 - L.w[:, None, None, None]
+    err = np.abs(back.cpu().numpy() - f64)
+    assert np.all(err <= EPS[dt] * np.abs(dev_true) * 1.0001 + 1e-300)
+    with pytest.raises(abi.MfaError):
+        abi.convert(abi.field(src, n, 0, 3), abi.field(back, n, 0, 3), 2)
+
+
+# bound on max |u - u_fp64| / u0 after the TGV run below, per storage type
+TGV_BOUND = {"f32": 1e-4, "f16dev": 2e-2, "bf16dev": 1.5e-1}
+
+
+@pytest.mark.parametrize("dt", ["f32", "f16dev", "bf16dev"])
+def test_tgv_error_against_fp64(dev, dt):
+    import torch
+    L = O.Lattice("D3Q27")
+    N = 48
+    n = [N, N, N]
+    u0 = 0.05
+    tau = 0.6
+    x = 2 * np.pi * (np.arange(N) + 0.5) / N
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    vel = np.stack([u0 * np.sin(X) * np.cos(Y) * np.cos(Z), -u0 * np.cos(X) * np.sin(Y) * np.cos(Z),
+                    np.zeros_like(X)])
+    rho = 1 + 3 * u0 ** 2 / 16 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2)
+    g = abi.resolve_bc(3, [abi.BC["Periodic"]] * 6)
+    out = {}
+    for st in ("f64", dt):
+        r = torch.from_numpy(rho[None].copy()).to(dev)
+        v = torch.from_numpy(vel.copy()).to(dev)
+        tdt = torch.float64 if st == "f64" else getattr(torch, TORCH_DT[st])
+        a = torch.empty((27, N, N, N), dtype=tdt, device=dev)
+        b = torch.empty_like(a)
+        fa, fb = abi.field(a, n, 0, 3, abi.DTYPE[st]), abi.field(b, n, 0, 3, abi.DTYPE[st])
+        abi.init_equilibrium(abi.field(r, n, 0, 3), abi.field(v, n, 0, 3), fa, 3)
+        mt = torch.zeros(4, dtype=torch.float64, device=dev)
+        abi.reduce(fa, 3, abi.RED["mass"], mt.data_ptr())
+        for _ in range(300):
+            abi.step(fa, fb, g, 1.0 / tau)
+            fa, fb = fb, fa
+        abi.reduce(fa, 3, abi.RED["mass"], mt[1:].data_ptr())
+        abi.moments(fa, abi.field(r, n, 0, 3), abi.field(v, n, 0, 3), g)
+        torch.cuda.synchronize()
+        out[st] = (v.cpu().numpy(), mt[0].item(), mt[1].item())
+    v64, m0, m1 = out["f64"]
+    vlp, l0, l1 = out[dt]
+    assert abs(m1 - m0) / m0 < 1e-13
+    err = np.abs(vlp - v64).max() / u0
+    print(f"{dt}: max |u - u_fp64| / u0 = {err:.3e}; mass drift {abs(l1 - l0) / l0:.3e}")
+    assert err < TGV_BOUND[dt]
+    assert np.isfinite(vlp).all()
