@@ -1,0 +1,319 @@
+"""Learned warm start for steady lid-driven cavities with obstacles.
+
+A ~1.0 M-parameter convolutional encoder-decoder maps (solid mask, Reynolds
+number) to the steady (rho, u) of a 256x256 D2Q9 lid cavity.  Training and
+inference run entirely on the engine's hand-written BF16 MFMA kernels
+(include/mfa_surrogate.h) with FP32 accumulation and an FP32 Adam master
+copy; torch only owns the device memory.  The solver then starts from
+f = feq(rho_pred, u_pred) and runs to the same steady-state criterion as a
+cold start, so the plain LBM stays both the fallback and the judge.
+
+Network (NHWC, 3x3 convolutions, ReLU except the head, additive skips):
+  x0[8] -> e1[32] -> e1b[32] -s2-> d1[64] -> e2[64] -s2-> d2[128] -> e3[128]
+  -s2-> d3[128] -> b1[128] -> b2[128] -> up+e3 -> c3[128] -> c3p[64]
+  -> up+e2 -> c2[64] -> c2p[32] -> up+e1b -> c1[32] -> out[8]
+Outputs 0..2 are (rho-1)/u_lid^2, u_x/u_lid, u_y/u_lid.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import abi
+
+LAYERS = [  # name, cin, cout, stride, relu, input
+    ("e1", 8, 32, 1, True, "x0"),
+    ("e1b", 32, 32, 1, True, "e1"),
+    ("d1", 32, 64, 2, True, "e1b"),
+    ("e2", 64, 64, 1, True, "d1"),
+    ("d2", 64, 128, 2, True, "e2"),
+    ("e3", 128, 128, 1, True, "d2"),
+    ("d3", 128, 128, 2, True, "e3"),
+    ("b1", 128, 128, 1, True, "d3"),
+    ("b2", 128, 128, 1, True, "b1"),
+    ("c3", 128, 128, 1, True, "u3"),
+    ("c3p", 128, 64, 1, True, "c3"),
+    ("c2", 64, 64, 1, True, "u2"),
+    ("c2p", 64, 32, 1, True, "c2"),
+    ("c1", 32, 32, 1, True, "u1"),
+    ("out", 32, 8, 1, False, "c1"),
+]
+# additive skips: name -> (upsampled tensor, skip tensor)
+SKIPS = {"u3": ("b2", "e3"), "u2": ("c3p", "e2"), "u1": ("c2p", "e1b")}
+# tensors that are ReLU outputs (their gradient is masked by value > 0)
+RELU_OUT = {name for name, _, _, _, relu, _ in LAYERS if relu}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = abi.lib()
+        vp = ctypes.c_void_p
+        i = ctypes.c_int
+        f = ctypes.c_float
+        ll = ctypes.c_longlong
+        L.mfa_nn_error.restype = ctypes.c_char_p
+        L.mfa_nn_conv3x3.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, i, i, vp]
+        L.mfa_nn_conv3x3_bwd_data.argtypes = [vp, vp, vp, vp, i, i, i, i, i, i, i, vp]
+        L.mfa_nn_conv3x3_bwd_weight.argtypes = [vp, vp, vp, vp, i, i, i, i, i, i, vp]
+        L.mfa_nn_fill.argtypes = [vp, ll, f, vp]
+        L.mfa_nn_prep_weights.argtypes = [vp, vp, vp, i, i, vp]
+        L.mfa_nn_upsample_add.argtypes = [vp, vp, vp, i, i, i, i, vp]
+        L.mfa_nn_upsample_bwd.argtypes = [vp, vp, vp, i, i, i, i, i, vp]
+        L.mfa_nn_masked_add.argtypes = [vp, vp, vp, ll, i, vp]
+        L.mfa_nn_mse.argtypes = [vp, vp, vp, vp, vp, ll, i, i, f, vp]
+        L.mfa_nn_adam.argtypes = [vp, vp, vp, vp, ll, f, f, f, f, i, f, vp]
+        L.mfa_nn_make_input.argtypes = [vp, vp, vp, i, i, i, vp]
+        for n in ("mfa_nn_conv3x3", "mfa_nn_conv3x3_bwd_data", "mfa_nn_conv3x3_bwd_weight",
+                  "mfa_nn_fill", "mfa_nn_prep_weights", "mfa_nn_upsample_add",
+                  "mfa_nn_upsample_bwd", "mfa_nn_masked_add", "mfa_nn_mse", "mfa_nn_adam",
+                  "mfa_nn_make_input"):
+            getattr(L, n).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ck(rc):
+    if rc != 0:
+        raise abi.MfaError(rc, lib().mfa_nn_error().decode())
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _s():
+    return abi.stream_ptr()
+
+
+# ---- thin kernel wrappers (NHWC tensors) ------------------------------------
+def conv(x, w, b, y=None, yf=None, stride=1, relu=True):
+    N, H, W, Cin = x.shape
+    Cout = w.shape[0]
+    _ck(lib().mfa_nn_conv3x3(_p(x), _p(w), _p(b), _p(y), _p(yf), N, H, W, Cin, Cout, stride,
+                             1 if relu else 0, _s()))
+
+
+def conv_bwd_data(dy, wt, dx, mask, stride, accumulate=False):
+    N, H, W, Cin = dx.shape
+    Cout = dy.shape[3]
+    _ck(lib().mfa_nn_conv3x3_bwd_data(_p(dy), _p(wt), _p(dx), _p(mask), N, H, W, Cin, Cout, stride,
+                                      1 if accumulate else 0, _s()))
+
+
+def conv_bwd_weight(x, dy, dw, db, stride):
+    N, H, W, Cin = x.shape
+    Cout = dy.shape[3]
+    _ck(lib().mfa_nn_conv3x3_bwd_weight(_p(x), _p(dy), _p(dw), _p(db), N, H, W, Cin, Cout, stride,
+                                        _s()))
+
+
+def fill(t, v=0.0):
+    _ck(lib().mfa_nn_fill(_p(t), t.numel(), v, _s()))
+
+
+def upsample_add(x, skip, y):
+    N, H, W, C = x.shape
+    _ck(lib().mfa_nn_upsample_add(_p(x), _p(skip), _p(y), N, H, W, C, _s()))
+
+
+def upsample_bwd(dy, dx, mask, accumulate=False):
+    N, H, W, C = dx.shape
+    _ck(lib().mfa_nn_upsample_bwd(_p(dy), _p(dx), _p(mask), N, H, W, C, 1 if accumulate else 0,
+                                  _s()))
+
+
+def masked_add(src, dst, mask, accumulate=False):
+    _ck(lib().mfa_nn_masked_add(_p(src), _p(dst), _p(mask), src.numel(), 1 if accumulate else 0,
+                                _s()))
+
+
+class WarmStartNet:
+    """Parameters (FP32 master + BF16 working copies) and the hand-scheduled
+    forward / backward passes over the MFMA kernels."""
+
+    def __init__(self, H=256, W=256, seed=0, dev="cuda:0"):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(dev)
+        self.H, self.W = H, W
+        rng = np.random.default_rng(seed)
+        self.layers = {}
+        for name, cin, cout, stride, relu, src in LAYERS:
+            std = math.sqrt(2.0 / (9 * cin)) if relu else math.sqrt(1.0 / (9 * cin))
+            w = rng.standard_normal((cout, 9 * cin)) * std
+            if name == "e1":
+                w.reshape(cout, 9, cin)[:, :, 5:] = 0.0     # padded input channels
+            self.layers[name] = {
+                "cin": cin, "cout": cout, "stride": stride, "relu": relu, "src": src,
+                "w32": torch.tensor(w, dtype=torch.float32, device=self.dev),
+                "b32": torch.zeros(cout, dtype=torch.float32, device=self.dev),
+                "w": torch.empty((cout, 3, 3, cin), dtype=torch.bfloat16, device=self.dev),
+                "wt": torch.empty((3, 3, cin, cout), dtype=torch.bfloat16, device=self.dev),
+            }
+        self.nparams = sum(l["w32"].numel() + l["b32"].numel() for l in self.layers.values())
+        self._adam = None
+        self.t = 0
+        self.prep()
+
+    # -- parameters --------------------------------------------------------
+    def prep(self):
+        for l in self.layers.values():
+            _ck(lib().mfa_nn_prep_weights(_p(l["w32"]), _p(l["w"]), _p(l["wt"]), l["cout"], l["cin"],
+                                          _s()))
+
+    def state_dict(self):
+        return {k: (l["w32"].cpu().numpy(), l["b32"].cpu().numpy()) for k, l in self.layers.items()}
+
+    def load_state(self, sd):
+        torch = self.torch
+        for k, (w, b) in sd.items():
+            self.layers[k]["w32"].copy_(torch.from_numpy(np.asarray(w, np.float32)))
+            self.layers[k]["b32"].copy_(torch.from_numpy(np.asarray(b, np.float32)))
+        self.prep()
+
+    def save(self, path):
+        sd = self.state_dict()
+        np.savez_compressed(path, **{f"{k}.w": v[0] for k, v in sd.items()},
+                            **{f"{k}.b": v[1] for k, v in sd.items()})
+
+    def load(self, path):
+        z = np.load(path)
+        self.load_state({k: (z[f"{k}.w"], z[f"{k}.b"]) for k in self.layers})
+
+    # -- forward -----------------------------------------------------------
+    def shapes(self, N):
+        H, W = self.H, self.W
+        sh = {"x0": (N, H, W, 8)}
+        for name, cin, cout, stride, relu, src in LAYERS:
+            if src in SKIPS:
+                a, b = SKIPS[src]
+                sh[src] = sh[b][:3] + (sh[a][3],)
+            n, h, w, _ = sh[src]
+            sh[name] = (n, (h + stride - 1) // stride, (w + stride - 1) // stride, cout)
+        return sh
+
+    def forward(self, x0, keep=True):
+        """x0: (N,H,W,8) bf16.  Returns the FP32 head output (N,H,W,8)."""
+        torch = self.torch
+        N = x0.shape[0]
+        sh = self.shapes(N)
+        act = {"x0": x0}
+        for name, cin, cout, stride, relu, src in LAYERS:
+            if src in SKIPS and src not in act:
+                a, b = SKIPS[src]
+                act[src] = torch.empty(sh[src], dtype=torch.bfloat16, device=self.dev)
+                upsample_add(act[a], act[b], act[src])
+            l = self.layers[name]
+            if name == "out":
+                yf = torch.empty(sh[name], dtype=torch.float32, device=self.dev)
+                conv(act[src], l["w"], l["b32"], None, yf, stride, relu=False)
+                act[name] = yf
+            else:
+                y = torch.empty(sh[name], dtype=torch.bfloat16, device=self.dev)
+                conv(act[src], l["w"], l["b32"], y, None, stride, relu)
+                act[name] = y
+        self.act = act if keep else None
+        return act["out"]
+
+    # -- backward ----------------------------------------------------------
+    def zero_grad(self):
+        torch = self.torch
+        for l in self.layers.values():
+            if "gw" not in l:
+                l["gw"] = torch.zeros_like(l["w32"])
+                l["gb"] = torch.zeros_like(l["b32"])
+            fill(l["gw"])
+            fill(l["gb"])
+
+    def backward(self, g_out):
+        """g_out: (N,H,W,8) bf16 gradient of the loss w.r.t. the head output."""
+        torch = self.torch
+        act = self.act
+        grad = {"out": g_out}
+        # gradients are w.r.t. pre-activation of each layer's output
+        for name, cin, cout, stride, relu, src in reversed(LAYERS):
+            l = self.layers[name]
+            g = grad.pop(name)
+            conv_bwd_weight(act[src], g, l["gw"], l["gb"], stride)
+            if name == "e1":
+                break
+            mask = act[src] if src in RELU_OUT else None
+            acc = src in grad
+            if not acc:
+                grad[src] = torch.empty_like(act[src])
+            conv_bwd_data(g, l["wt"], grad[src], mask, stride, accumulate=acc)
+            if src in SKIPS:
+                a, b = SKIPS[src]
+                gu = grad.pop(src)
+                ga = torch.empty_like(act[a])
+                upsample_bwd(gu, ga, act[a])
+                grad[a] = ga
+                accb = b in grad
+                if not accb:
+                    grad[b] = torch.empty_like(act[b])
+                masked_add(gu, grad[b], act[b], accumulate=accb)
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
+        torch = self.torch
+        if self._adam is None:
+            self._adam = {k: (torch.zeros_like(l["w32"]), torch.zeros_like(l["w32"]),
+                              torch.zeros_like(l["b32"]), torch.zeros_like(l["b32"]))
+                          for k, l in self.layers.items()}
+        self.t += 1
+        for k, l in self.layers.items():
+            mw, vw, mb, vb = self._adam[k]
+            _ck(lib().mfa_nn_adam(_p(l["w32"]), _p(l["gw"]), _p(mw), _p(vw), l["w32"].numel(), lr,
+                                  beta1, beta2, eps, self.t, wd, _s()))
+            _ck(lib().mfa_nn_adam(_p(l["b32"]), _p(l["gb"]), _p(mb), _p(vb), l["b32"].numel(), lr,
+                                  beta1, beta2, eps, self.t, 0.0, _s()))
+        self.prep()
+
+
+# ---- data -------------------------------------------------------------------
+def re_feature(Re):
+    """log-Re mapped to [-1, 1] over [100, 3200]."""
+    lo, hi = math.log(100.0), math.log(3200.0)
+    return (2.0 * (np.log(np.asarray(Re, dtype=np.float64)) - lo) / (hi - lo) - 1.0).astype(np.float32)
+
+
+def random_obstacles(rng, N=256, max_boxes=3, side=(8, 48)):
+    m = np.zeros((N, N), dtype=np.uint8)
+    for _ in range(int(rng.integers(0, max_boxes + 1))):
+        w = int(rng.integers(side[0], side[1] + 1))
+        h = int(rng.integers(side[0], side[1] + 1))
+        x0 = int(rng.integers(0, N - w + 1))
+        y0 = int(rng.integers(0, N - h + 1))
+        m[y0:y0 + h, x0:x0 + w] = 1
+    return m
+
+
+def make_cases(n, seed, N=256, re_range=(100.0, 3200.0), u_lid=0.1):
+    """Configuration 5 cases: obstacle masks, Re log-uniform, tau for lid u."""
+    rng = np.random.default_rng(seed)
+    masks = np.stack([random_obstacles(rng, N) for _ in range(n)])
+    Re = np.exp(rng.uniform(math.log(re_range[0]), math.log(re_range[1]), size=n))
+    tau = 0.5 + 3.0 * u_lid * N / Re
+    return masks, Re, tau
+
+
+def targets_from_fields(rho, vel, u_lid=0.1):
+    """(n,H,W) rho, (n,2,H,W) u -> (n,H,W,3) float32 scaled targets."""
+    t = np.stack([(rho - 1.0) / u_lid ** 2, vel[:, 0] / u_lid, vel[:, 1] / u_lid], axis=-1)
+    return t.astype(np.float32)
+
+
+def fields_from_prediction(pred, u_lid=0.1):
+    """(n,H,W,>=3) scaled prediction -> rho (n,H,W), u (n,2,H,W) float64."""
+    p = np.asarray(pred, dtype=np.float64)
+    rho = 1.0 + p[..., 0] * u_lid ** 2
+    u = np.stack([p[..., 1], p[..., 2]], axis=1) * u_lid
+    return rho, u

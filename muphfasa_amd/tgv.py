@@ -1,0 +1,119 @@
+"""3-D Taylor-Green vortex precision study (configuration 4).
+
+D3Q27 BGK on a periodic N^3 box (default 384^3), u0 = 0.05, Re = 800 on
+L = N / (2 pi) (nu = u0 L / Re, tau = 0.5 + 3 nu), run for the given number
+of steps with populations stored as FP64 (the reference arithmetic), FP32,
+FP16 and BF16 shifted deviations (collided in FP32).  Reports MLUPS per
+storage type and, against the FP64 run, the relative L2 error of u and the
+kinetic-energy history.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+from . import abi
+
+TORCH_DT = {"f64": "float64", "f32": "float32", "f16dev": "float16", "bf16dev": "bfloat16"}
+
+
+def tgv_fields(torch, N, u0, dev):
+    x = (torch.arange(N, dtype=torch.float64, device=dev) + 0.5) * (2 * math.pi / N)
+    Z, Y, X = torch.meshgrid(x, x, x, indexing="ij")
+    vel = torch.stack([u0 * torch.sin(X) * torch.cos(Y) * torch.cos(Z),
+                       -u0 * torch.cos(X) * torch.sin(Y) * torch.cos(Z),
+                       torch.zeros_like(X)]).contiguous()
+    rho = (1.0 + 3.0 * u0 ** 2 / 16.0 * (torch.cos(2 * X) + torch.cos(2 * Y)) *
+           (torch.cos(2 * Z) + 2.0))[None].contiguous()
+    return rho, vel
+
+
+def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32", "f16dev", "bf16dev"),
+        dev="cuda:0"):
+    import torch
+    L = N / (2 * math.pi)
+    nu = u0 * L / Re
+    tau = 0.5 + 3.0 * nu
+    omega = 1.0 / tau
+    n = [N, N, N]
+    g = abi.resolve_bc(3, [abi.BC["Periodic"]] * 6)
+    results = {}
+    ref_vel = {}
+    for st in storages:
+        tdt = getattr(torch, TORCH_DT[st])
+        a = torch.empty((27, N, N, N), dtype=tdt, device=dev)
+        b = torch.empty_like(a)
+        fa = abi.field(a, n, 0, 3, abi.DTYPE[st])
+        fb = abi.field(b, n, 0, 3, abi.DTYPE[st])
+        rho, vel = tgv_fields(torch, N, u0, dev)
+        fr, fv = abi.field(rho, n, 0, 3), abi.field(vel, n, 0, 3)
+        abi.init_equilibrium(fr, fv, fa, 3)
+        out = torch.zeros(8, dtype=torch.float64, device=dev)
+        abi.reduce(fa, 3, abi.RED["mass"], out.data_ptr())
+        torch.cuda.synchronize()
+        m0 = out[0].item()
+        ke, errs, t_kernel = [], [], 0.0
+        done = 0
+        while done < steps:
+            k = min(sample, steps - done)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(k):
+                abi.step(fa, fb, g, omega)
+                a, b, fa, fb = b, a, fb, fa
+            e1.record()
+            torch.cuda.synchronize()
+            t_kernel += e0.elapsed_time(e1) * 1e-3
+            done += k
+            abi.moments(fa, fr, fv, g)
+            energy = 0.5 * (rho[0] * (vel * vel).sum(0)).mean().item() / (u0 ** 2)
+            ke.append((done, energy))
+            if st == "f64":
+                ref_vel[done] = vel.clone()
+            elif done in ref_vel:
+                d = vel - ref_vel[done]
+                errs.append((done, math.sqrt((d * d).sum().item() / (ref_vel[done] ** 2).sum().item())))
+        abi.reduce(fa, 3, abi.RED["mass"], out.data_ptr())
+        torch.cuda.synchronize()
+        m1 = out[0].item()
+        bytes_lu = 2 * 27 * a.element_size()
+        mlups = N ** 3 * steps / t_kernel / 1e6
+        results[st] = {"mlups": round(mlups, 1), "bytes_per_lu": bytes_lu,
+                       "GBs": round(mlups * bytes_lu / 1e3, 1), "ke": ke,
+                       "l2_u_err_vs_f64": errs, "mass_rel_drift": abs(m1 - m0) / m0,
+                       "finite": bool(torch.isfinite(vel).all().item())}
+        if st != "f64":
+            ke64 = dict(results["f64"]["ke"])
+            results[st]["ke_rel_err_final"] = abs(ke[-1][1] - ke64[ke[-1][0]]) / ke64[ke[-1][0]]
+        print(json.dumps({"storage": st, **{k: v for k, v in results[st].items()
+                                             if k not in ("ke", "l2_u_err_vs_f64")},
+                          "l2_u_err_final": errs[-1][1] if errs else 0.0,
+                          "ke_final": ke[-1][1]}), flush=True)
+        del a, b
+        torch.cuda.empty_cache()
+    return {"N": N, "u0": u0, "Re": Re, "tau": tau, "steps": steps, "results": results}
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--n", type=int, default=384)
+    p.add_argument("--steps", type=int, default=5000)
+    p.add_argument("--sample", type=int, default=500)
+    p.add_argument("--out", default="")
+    a = p.parse_args()
+    r = run(a.n, steps=a.steps, sample=a.sample)
+    if a.out:
+        with open(a.out, "w") as fh:
+            json.dump(r, fh)
+    print(json.dumps({"tau": r["tau"], "summary": {k: {"mlups": v["mlups"],
+                      "l2_u_err_final": v["l2_u_err_vs_f64"][-1][1] if v["l2_u_err_vs_f64"] else 0.0,
+                      "ke_final": v["ke"][-1][1]} for k, v in r["results"].items()}}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,247 @@
+"""Learned warm start pipeline (configuration 5).
+
+  generate : the batched engine runs N obstacle cavities (256^2, Re log-uniform
+             in [100, 3200], lid u = 0.1) to a steady state and keeps (rho, u)
+  train    : the BF16-MFMA encoder-decoder is fitted to those fields
+  evaluate : held-out cases run to the 1e-8 criterion from rest (cold) and
+             from feq of the prediction (warm); steps, wall time and the
+             difference of the converged fields are reported
+
+Usage: python -m muphfasa_amd.warmstart --stage train --n-train 4096 ...
+       python -m muphfasa_amd.warmstart --stage eval --weights w.npz ...
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import abi
+from . import ensemble as E
+from . import surrogate as S
+
+
+def _log(**kw):
+    print(json.dumps(kw), flush=True)
+
+
+def generate(n, seed, dtype="f32", tol=1e-6, max_steps=60000, check=1000, N=256, tile=(64, 4)):
+    """Returns masks (n,N,N) uint8, Re (n,), fields on device (rho, vel) and
+    the run statistics."""
+    masks, Re, tau = S.make_cases(n, seed, N)
+    ens = E.Ensemble([(N, N)] * n, 1.0 / tau, dtype=dtype, tile=tile, masks=list(masks))
+    ens.init()
+    t0 = time.time()
+
+    def log(s, nact, rmax):
+        if s % (10 * check) == 0:
+            _log(stage="generate", step=s, active=nact, max_res=rmax, t=round(time.time() - t0, 1))
+    r = E.run_to_steady(ens, check, tol, max_steps, graph=True, log=log)
+    ens.set_active(np.ones(n, np.int32))
+    ens.moments()
+    stats = {"n": n, "tol": tol, "max_steps": max_steps, "dtype": dtype,
+             "converged": int((r["status"] == "converged").sum()),
+             "diverged": int((r["status"] == "diverged").sum()),
+             "median_steps": float(np.median(r["steps"])), "gpu_s": round(r["gpu_s"], 2),
+             "mlups": round(r["mlups_gpu"], 1)}
+    return masks, Re, ens, r, stats
+
+
+def device_targets(ens, n, N=256, u_lid=0.1):
+    """(n,N,N,3) float32 scaled targets built on the device from the ensemble fields."""
+    torch = ens.torch
+    tc = ens.total_cells
+    rho = ens.rho.view(n, N, N)
+    ux = ens.vel[:tc].view(n, N, N)
+    uy = ens.vel[tc:].view(n, N, N)
+    t = torch.stack([(rho - 1.0) / u_lid ** 2, ux / u_lid, uy / u_lid], dim=-1)
+    return t.to(torch.float32).contiguous()
+
+
+def make_inputs(net, masks_dev, re_feat_dev, start, count):
+    torch = net.torch
+    N = net.H
+    x0 = torch.empty((count, N, N, 8), dtype=torch.bfloat16, device=net.dev)
+    S._ck(S.lib().mfa_nn_make_input(S._p(masks_dev[start:start + count]),
+                                    S._p(re_feat_dev[start:start + count]), S._p(x0), count, N, N,
+                                    S._s()))
+    return x0
+
+
+def train(net, masks_dev, re_feat_dev, targets, epochs=20, batch=16, lr=2e-3, seed=0, log_every=50):
+    torch = net.torch
+    n = targets.shape[0]
+    N = net.H
+    rng = np.random.default_rng(seed)
+    g = torch.empty((batch, N, N, 8), dtype=torch.bfloat16, device=net.dev)
+    part = torch.zeros((batch * N * N + 255) // 256, dtype=torch.float32, device=net.dev)
+    steps_per_epoch = n // batch
+    total = epochs * steps_per_epoch
+    hist = []
+    t0 = time.time()
+    it = 0
+    x_buf = torch.empty((batch, N, N, 8), dtype=torch.bfloat16, device=net.dev)
+    t_buf = torch.empty((batch, N, N, 3), dtype=torch.float32, device=net.dev)
+    m_buf = torch.empty((batch, N, N), dtype=torch.uint8, device=net.dev)
+    r_buf = torch.empty(batch, dtype=torch.float32, device=net.dev)
+    for ep in range(epochs):
+        perm = torch.from_numpy(rng.permutation(n)).to(net.dev)
+        ep_loss = 0.0
+        for bi in range(steps_per_epoch):
+            idx = perm[bi * batch:(bi + 1) * batch]
+            torch.index_select(masks_dev, 0, idx, out=m_buf)
+            torch.index_select(re_feat_dev, 0, idx, out=r_buf)
+            torch.index_select(targets, 0, idx, out=t_buf)
+            S._ck(S.lib().mfa_nn_make_input(S._p(m_buf), S._p(r_buf), S._p(x_buf), batch, N, N,
+                                            S._s()))
+            lr_t = lr * 0.5 * (1 + math.cos(math.pi * it / total))
+            out = net.forward(x_buf)
+            count = float(batch * N * N * 3)
+            S._ck(S.lib().mfa_nn_mse(S._p(out), S._p(t_buf), S._p(m_buf), S._p(g), S._p(part),
+                                     batch * N * N, 8, 3, 1.0 / count, S._s()))
+            net.zero_grad()
+            net.backward(g)
+            net.adam_step(lr_t)
+            if it % log_every == 0 or bi == steps_per_epoch - 1:
+                l = part.sum().item() / count
+                ep_loss = l
+                hist.append((it, l))
+                if it % (4 * log_every) == 0:
+                    _log(stage="train", epoch=ep, it=it, loss=l, lr=lr_t, t=round(time.time() - t0, 1))
+            it += 1
+        _log(stage="train", epoch=ep, loss=ep_loss, t=round(time.time() - t0, 1))
+    return hist, time.time() - t0
+
+
+def predict(net, masks_dev, re_feat_dev, batch=16):
+    """Scaled predictions (n,N,N,8) float32 on the device."""
+    torch = net.torch
+    n = masks_dev.shape[0]
+    N = net.H
+    out = torch.empty((n, N, N, 8), dtype=torch.float32, device=net.dev)
+    for s in range(0, n, batch):
+        c = min(batch, n - s)
+        x0 = make_inputs(net, masks_dev, re_feat_dev, s, c)
+        out[s:s + c] = net.forward(x0, keep=False)
+    return out
+
+
+def fields_on_device(pred, masks_dev, u_lid=0.1):
+    """rho (n*N*N,), vel (2*n*N*N,) float64 in the ensemble's cell order."""
+    torch = pred.torch if hasattr(pred, "torch") else None
+    import torch
+    p = pred.double()
+    fluid = (masks_dev == 0).double()
+    rho = (1.0 + p[..., 0] * u_lid ** 2) * fluid + (1 - fluid)
+    ux = p[..., 1] * u_lid * fluid
+    uy = p[..., 2] * u_lid * fluid
+    return rho.reshape(-1).contiguous(), torch.cat([ux.reshape(-1), uy.reshape(-1)]).contiguous()
+
+
+def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, dtype="f64", N=256,
+             tile=(64, 4)):
+    import torch
+    masks, Re, tau = S.make_cases(n_eval, seed, N)
+    masks_dev = torch.from_numpy(masks).to(net.dev)
+    re_dev = torch.from_numpy(S.re_feature(Re)).to(net.dev)
+    res = {}
+    for mode in ("warm", "cold"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens = E.Ensemble([(N, N)] * n_eval, 1.0 / tau, dtype=dtype, tile=tile, masks=list(masks))
+        t_inf = 0.0
+        if mode == "warm":
+            ti = time.perf_counter()
+            pred = predict(net, masks_dev, re_dev)
+            rho, vel = fields_on_device(pred, masks_dev)
+            torch.cuda.synchronize()
+            t_inf = time.perf_counter() - ti
+            ens.init(rho, vel)
+            del pred
+        else:
+            ens.init()
+
+        def log(s, nact, rmax, mode=mode):
+            if s % (20 * check) == 0:
+                _log(stage="eval", mode=mode, step=s, active=nact, max_res=rmax,
+                     t=round(time.perf_counter() - t0, 1))
+        r = E.run_to_steady(ens, check, tol, max_steps, graph=True, log=log)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ens.set_active(np.ones(n_eval, np.int32))
+        ens.moments()
+        vel = ens.vel.clone()
+        res[mode] = {"steps": r["steps"], "status": r["status"], "wall_s": wall, "gpu_s": r["gpu_s"],
+                     "inference_s": t_inf, "vel": vel}
+        del ens
+    cw, cc = res["warm"], res["cold"]
+    both = (cw["status"] == "converged") & (cc["status"] == "converged")
+    tc = n_eval * N * N
+    dv = (cw["vel"] - cc["vel"]).abs()
+    dmax = torch.maximum(dv[:tc], dv[tc:]).view(n_eval, -1).max(dim=1).values.cpu().numpy() / 0.1
+    ratio = cc["steps"][both] / np.maximum(cw["steps"][both], 1)
+    summary = {
+        "n_eval": n_eval, "tol": tol, "max_steps": max_steps, "dtype": dtype,
+        "cold_converged": int((cc["status"] == "converged").sum()),
+        "warm_converged": int((cw["status"] == "converged").sum()),
+        "both_converged": int(both.sum()),
+        "cold_steps_total": int(cc["steps"].sum()), "warm_steps_total": int(cw["steps"].sum()),
+        "steps_ratio_median": float(np.median(ratio)) if both.any() else None,
+        "steps_ratio_mean": float(np.mean(ratio)) if both.any() else None,
+        "cold_wall_s": round(cc["wall_s"], 2), "warm_wall_s": round(cw["wall_s"], 2),
+        "warm_inference_s": round(cw["inference_s"], 3),
+        "wall_ratio": round(cc["wall_s"] / cw["wall_s"], 3),
+        "field_diff_max_over_both": float(dmax[both].max()) if both.any() else None,
+        "field_diff_median": float(np.median(dmax[both])) if both.any() else None,
+    }
+    return summary, res, (masks, Re)
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--stage", default="all", choices=["train", "eval", "all"])
+    p.add_argument("--n-train", type=int, default=4096)
+    p.add_argument("--gen-tol", type=float, default=1e-6)
+    p.add_argument("--gen-max-steps", type=int, default=60000)
+    p.add_argument("--epochs", type=int, default=20)
+    p.add_argument("--batch", type=int, default=16)
+    p.add_argument("--lr", type=float, default=2e-3)
+    p.add_argument("--weights", default="gpurun_out/warmstart_weights.npz")
+    p.add_argument("--n-eval", type=int, default=256)
+    p.add_argument("--eval-max-steps", type=int, default=200000)
+    p.add_argument("--tol", type=float, default=1e-8)
+    p.add_argument("--seed", type=int, default=2024)
+    a = p.parse_args()
+    import torch
+    net = S.WarmStartNet(seed=a.seed)
+    _log(stage="init", params=net.nparams)
+    if a.stage in ("train", "all"):
+        masks, Re, ens, r, stats = generate(a.n_train, a.seed, tol=a.gen_tol, max_steps=a.gen_max_steps)
+        _log(stage="generate_done", **stats)
+        targets = device_targets(ens, a.n_train)
+        del ens
+        torch.cuda.empty_cache()
+        masks_dev = torch.from_numpy(masks).to(net.dev)
+        re_dev = torch.from_numpy(S.re_feature(Re)).to(net.dev)
+        hist, t_train = train(net, masks_dev, re_dev, targets, a.epochs, a.batch, a.lr, a.seed)
+        os.makedirs(os.path.dirname(a.weights) or ".", exist_ok=True)
+        net.save(a.weights)
+        _log(stage="train_done", seconds=round(t_train, 1), final_loss=hist[-1][1],
+             first_loss=hist[0][1], weights=a.weights)
+        del targets
+        torch.cuda.empty_cache()
+    if a.stage in ("eval", "all"):
+        if a.stage == "eval":
+            net.load(a.weights)
+        summary, res, _ = evaluate(net, a.n_eval, seed=a.seed + 1, tol=a.tol,
+                                   max_steps=a.eval_max_steps)
+        _log(stage="eval_done", **summary)
+
+
+if __name__ == "__main__":
+    main()

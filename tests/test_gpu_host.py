@@ -68,6 +68,7 @@ def test_periodic_noise_init_and_steps_match_oracle():
     sim.advance(100)
     res = O.resolve_bc(3, [5] * 6)
     fab = _oracle_fab(L, n, f0)
+    O.fill_periodic(L, fab, n, [1, 1, 1])
     fe = O.run(L, fab, n, sim.omega, sim.force, [1, 1, 1], res, 100)
     assert np.array_equal(sim.f(), O.valid(L, fe))
 

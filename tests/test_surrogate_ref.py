@@ -1,0 +1,60 @@
+"""Pins the float64 numpy restatement of the warm-start network: its
+analytic gradients equal central finite differences of its own loss."""
+import numpy as np
+
+from oracle import surrogate_ref as R
+
+LAYERS = [("a", 8, 8, 1, True, "x0"), ("b", 8, 16, 2, True, "a"), ("c", 16, 8, 1, True, "b"),
+          ("d", 8, 8, 1, True, "u"), ("out", 8, 8, 1, False, "d")]
+SKIPS = {"u": ("c", "a")}
+
+
+def _setup(seed=0):
+    rng = np.random.default_rng(seed)
+    params = {n: (rng.standard_normal((co, 3, 3, ci)) * 0.3, rng.standard_normal(co) * 0.1)
+              for n, ci, co, s, r, src in LAYERS}
+    x0 = rng.standard_normal((2, 6, 6, 8))
+    target = rng.standard_normal((2, 6, 6, 3))
+    solid = (rng.random((2, 6, 6)) < 0.2).astype(np.uint8)
+    return params, x0, target, solid
+
+
+def _loss(params, x0, target, solid):
+    out, act, pre = R.forward(params, LAYERS, SKIPS, x0)
+    return R.mse(out, target, solid, 3)[0]
+
+
+def test_gradients_match_finite_differences():
+    params, x0, target, solid = _setup()
+    out, act, pre = R.forward(params, LAYERS, SKIPS, x0)
+    loss, g = R.mse(out, target, solid, 3)
+    grads = R.backward(params, LAYERS, SKIPS, act, pre, g)
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    for name in ("a", "b", "d", "out"):
+        w, b = params[name]
+        for _ in range(6):
+            idx = tuple(rng.integers(0, s) for s in w.shape)
+            wp = w.copy(); wp[idx] += h
+            wm = w.copy(); wm[idx] -= h
+            pp = dict(params); pp[name] = (wp, b)
+            pm = dict(params); pm[name] = (wm, b)
+            fd = (_loss(pp, x0, target, solid) - _loss(pm, x0, target, solid)) / (2 * h)
+            assert abs(fd - grads[name][0][idx]) <= 1e-6 * max(1.0, abs(fd)), (name, idx)
+        j = int(rng.integers(0, b.shape[0]))
+        bp = b.copy(); bp[j] += h
+        bm = b.copy(); bm[j] -= h
+        pp = dict(params); pp[name] = (w, bp)
+        pm = dict(params); pm[name] = (w, bm)
+        fd = (_loss(pp, x0, target, solid) - _loss(pm, x0, target, solid)) / (2 * h)
+        assert abs(fd - grads[name][1][j]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_conv_is_correlation_with_zero_padding():
+    # a single 3x3 kernel with one nonzero tap shifts the image
+    x = np.arange(25, dtype=float).reshape(1, 5, 5, 1)
+    w = np.zeros((1, 3, 3, 1)); w[0, 0, 2, 0] = 1.0      # tap (ky=0, kx=2) reads x[y-1, x+1]
+    y = R.conv3x3(x, w, np.zeros(1))
+    assert y[0, 2, 2, 0] == x[0, 1, 3, 0] and y[0, 0, 0, 0] == 0.0
+    y2 = R.conv3x3(x, w, np.zeros(1), stride=2)
+    assert y2.shape == (1, 3, 3, 1) and y2[0, 1, 1, 0] == x[0, 1, 3, 0]
