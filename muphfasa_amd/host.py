@@ -93,10 +93,20 @@ class Simulation:
         self.n = list(n)
         self.omega = om.value
         self.force = list(F)
-        self.time = t.value
-        self.start_step = s0.value
+        self._time = t.value
+        self._start_step = s0.value
         self.Q = {0: 9, 1: 15, 2: 19, 3: 27}[self.lattice]
         self.D = 2 if self.lattice == 0 else 3
+
+    @property
+    def time(self):
+        self.refresh()
+        return self._time
+
+    @property
+    def start_step(self):
+        self.refresh()
+        return self._start_step
 
     def initialize(self):
         _check(lib().mfa_sim_initialize(self._h))
