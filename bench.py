@@ -33,8 +33,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="target duration of the bounded CPU baseline sample")
-    p.add_argument("--copy-probe", action="store_true",
-                   help="also run a plain device copy of known size (counter calibration)")
+    p.add_argument("--no-copy-probe", action="store_true",
+                   help="skip the streaming-copy ceiling measurement (same bytes as one step)")
     p.add_argument("--seed", type=int, default=2024)
     return p.parse_args()
 
@@ -143,7 +143,7 @@ def main():
     traffic = traffic_from_profiles(n)
 
     copy = None
-    if args.copy_probe:
+    if not args.no_copy_probe:
         from muphfasa_amd import autotune
         del out
         copy = autotune.copy_ceiling(N3 * 19 * 8)
@@ -181,6 +181,7 @@ def main():
                      "algorithmic_bytes_per_launch": N3 * bytes_per_lu},
         "mass_rel_drift": abs(mass1 - mass0) / mass0,
         "copy_ceiling_GBs": copy,
+        "frac_of_copy_ceiling": round(achieved / copy["nt"], 4) if copy else None,
         "launch": abi.get_launch(lattice),
         "wall_s_timed": round(wall, 4),
         "cpu_baseline": cpu,

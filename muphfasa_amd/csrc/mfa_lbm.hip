@@ -390,11 +390,15 @@ static void launch_shape(const int n[3], dim3& grid, dim3& block, int grow)
     grid = dim3((nx + bx - 1) / bx, (n[1] + 2 * grow + by - 1) / by, 1);
 }
 
-// Launch configuration of the fused step kernel, per lattice (set by the
-// autotuner through mfa_set_launch; defaults measured on MI355X).
+// Launch configuration of the fused step kernel, per lattice index (D2Q9,
+// D3Q15, D3Q19, D3Q27), set by the autotuner through mfa_set_launch.  The
+// defaults are the autotuner's picks on MI355X for D2Q9 128^2, D3Q19 256^3 and
+// D3Q27 192^3 FP64 forced periodic; D3Q15 takes D3Q19's.
 static std::mutex g_cfg_mu;
+static const mfa_launch k_default_cfg[4] = {
+    {256, 1, 1, 0}, {256, 1, 1, 1}, {256, 1, 1, 1}, {64, 4, 1, 0}};
 static mfa_launch g_cfg[4] = {
-    {256, 1, 1, 0}, {256, 1, 1, 0}, {256, 1, 1, 0}, {256, 1, 1, 0}};
+    {256, 1, 1, 0}, {256, 1, 1, 1}, {256, 1, 1, 1}, {64, 4, 1, 0}};
 
 static bool valid_cfg(const mfa_launch& c)
 {
@@ -649,8 +653,7 @@ int mfa_convert(const mfa_field* src, mfa_field* dst, int lattice, void* stream)
 int mfa_set_launch(int lattice, const mfa_launch* cfg)
 {
     if (lattice_q(lattice) < 0) return fail(MFA_ERR_INVALID, "mfa_set_launch: unknown lattice");
-    const mfa_launch def = {256, 1, 1, 0};
-    const mfa_launch c = cfg ? *cfg : def;
+    const mfa_launch c = cfg ? *cfg : k_default_cfg[lattice];
     if (!valid_cfg(c)) return fail(MFA_ERR_INVALID, "mfa_set_launch: unsupported launch configuration");
     std::lock_guard<std::mutex> lk(g_cfg_mu);
     g_cfg[lattice] = c;

@@ -73,7 +73,16 @@ def make_inputs(net, masks_dev, re_feat_dev, start, count):
     return x0
 
 
-def train(net, masks_dev, re_feat_dev, targets, epochs=20, batch=16, lr=2e-3, seed=0, log_every=50):
+def flops_per_sample(net):
+    """Forward FLOPs of one 256x256 sample (2 per multiply-add)."""
+    sh = net.shapes(1)
+    return sum(2 * 9 * cin * cout * sh[name][1] * sh[name][2]
+               for name, cin, cout, stride, relu, src in S.LAYERS)
+
+
+def train(net, masks_dev, re_feat_dev, targets, epochs=20, batch=16, lr=2e-3, seed=0, log_every=50,
+          seconds=None):
+    """Adam with a cosine schedule; stops after `epochs` or `seconds`."""
     torch = net.torch
     n = targets.shape[0]
     N = net.H
@@ -89,7 +98,11 @@ def train(net, masks_dev, re_feat_dev, targets, epochs=20, batch=16, lr=2e-3, se
     t_buf = torch.empty((batch, N, N, 3), dtype=torch.float32, device=net.dev)
     m_buf = torch.empty((batch, N, N), dtype=torch.uint8, device=net.dev)
     r_buf = torch.empty(batch, dtype=torch.float32, device=net.dev)
+    fl = 3 * flops_per_sample(net) * batch          # forward + data grad + weight grad
+    stop = False
     for ep in range(epochs):
+        if stop:
+            break
         perm = torch.from_numpy(rng.permutation(n)).to(net.dev)
         ep_loss = 0.0
         for bi in range(steps_per_epoch):
@@ -99,7 +112,8 @@ def train(net, masks_dev, re_feat_dev, targets, epochs=20, batch=16, lr=2e-3, se
             torch.index_select(targets, 0, idx, out=t_buf)
             S._ck(S.lib().mfa_nn_make_input(S._p(m_buf), S._p(r_buf), S._p(x_buf), batch, N, N,
                                             S._s()))
-            lr_t = lr * 0.5 * (1 + math.cos(math.pi * it / total))
+            frac = it / total if seconds is None else min(1.0, (time.time() - t0) / seconds)
+            lr_t = lr * 0.5 * (1 + math.cos(math.pi * frac))
             out = net.forward(x_buf)
             count = float(batch * N * N * 3)
             S._ck(S.lib().mfa_nn_mse(S._p(out), S._p(t_buf), S._p(m_buf), S._p(g), S._p(part),
@@ -114,7 +128,13 @@ def train(net, masks_dev, re_feat_dev, targets, epochs=20, batch=16, lr=2e-3, se
                 if it % (4 * log_every) == 0:
                     _log(stage="train", epoch=ep, it=it, loss=l, lr=lr_t, t=round(time.time() - t0, 1))
             it += 1
-        _log(stage="train", epoch=ep, loss=ep_loss, t=round(time.time() - t0, 1))
+            if seconds is not None and time.time() - t0 > seconds:
+                stop = True
+                break
+        torch.cuda.synchronize()
+        el = time.time() - t0
+        _log(stage="train", epoch=ep, loss=ep_loss, t=round(el, 1),
+             tflops=round(fl * it / el / 1e12, 2))
     return hist, time.time() - t0
 
 
@@ -133,7 +153,6 @@ def predict(net, masks_dev, re_feat_dev, batch=16):
 
 def fields_on_device(pred, masks_dev, u_lid=0.1):
     """rho (n*N*N,), vel (2*n*N*N,) float64 in the ensemble's cell order."""
-    torch = pred.torch if hasattr(pred, "torch") else None
     import torch
     p = pred.double()
     fluid = (masks_dev == 0).double()
@@ -211,6 +230,7 @@ def main():
     p.add_argument("--epochs", type=int, default=20)
     p.add_argument("--batch", type=int, default=16)
     p.add_argument("--lr", type=float, default=2e-3)
+    p.add_argument("--train-seconds", type=float, default=None)
     p.add_argument("--weights", default="gpurun_out/warmstart_weights.npz")
     p.add_argument("--n-eval", type=int, default=256)
     p.add_argument("--eval-max-steps", type=int, default=200000)
@@ -228,7 +248,8 @@ def main():
         torch.cuda.empty_cache()
         masks_dev = torch.from_numpy(masks).to(net.dev)
         re_dev = torch.from_numpy(S.re_feature(Re)).to(net.dev)
-        hist, t_train = train(net, masks_dev, re_dev, targets, a.epochs, a.batch, a.lr, a.seed)
+        hist, t_train = train(net, masks_dev, re_dev, targets, a.epochs, a.batch, a.lr, a.seed,
+                              seconds=a.train_seconds)
         os.makedirs(os.path.dirname(a.weights) or ".", exist_ok=True)
         net.save(a.weights)
         _log(stage="train_done", seconds=round(t_train, 1), final_loss=hist[-1][1],

@@ -137,3 +137,35 @@ def test_closed_box_mass_conservation_with_refreshed_moments():
     r1, v1 = sim.macro()
     assert abs(r1.sum() - r0) / r0 < 1e-12
     assert np.abs(v1).max() > 0
+
+
+@pytest.mark.parametrize("case,window,tol,trans,spread", [
+    ("poiseuille_periodic", (0.0, 1.0), 1.5e-3, 1e-6, 1e-8),
+    ("poiseuille_inflow", (0.5, 0.8), 5e-3, 4e-5, 2e-5),
+])
+def test_poiseuille_regression(case, window, tol, trans, spread):
+    """tests/regression/poiseuille/main.cpp:130-229 criteria: profile error and
+    mean velocity vs F H^2/(8 nu), transverse velocity and density spread."""
+    text = _read(case)
+    u = host.inputs_units(text)
+    sim = host.Simulation(text)
+    sim.initialize()
+    max_step = int([l.split("=")[1] for l in text.splitlines() if l.startswith("sim.max_step")][0])
+    sim.advance(max_step + 1)
+    sim.update_macroscopic(max_step)
+    rho, vel = sim.macro()
+    Nx, Ny = sim.n[0], sim.n[1]
+    H = float(Ny)
+    nu = (u["tau"] - 0.5) / 3.0
+    F = u["F0"]
+    U_max = F * H * H / (8.0 * nu)
+    i_lo, i_hi = int(round(window[0] * Nx)), int(round(window[1] * Nx))
+    y = np.arange(Ny) + 0.5
+    ua = F / (2 * nu) * y * (H - y)
+    ux = vel[0, 0, :, i_lo:i_hi]
+    err = np.abs(ux - ua[:, None]).max() / U_max
+    mean_err = abs(ux.mean() - 2.0 / 3.0 * U_max) / U_max
+    tr = np.abs(vel[1, 0, :, i_lo:i_hi]).max() / U_max
+    rs = rho[0, :, i_lo:i_hi].max() - rho[0, :, i_lo:i_hi].min()
+    print(case, err, mean_err, tr, rs)
+    assert err < tol and mean_err < tol and tr < trans and rs < spread

@@ -58,3 +58,19 @@ def test_conv_is_correlation_with_zero_padding():
     assert y[0, 2, 2, 0] == x[0, 1, 3, 0] and y[0, 0, 0, 0] == 0.0
     y2 = R.conv3x3(x, w, np.zeros(1), stride=2)
     assert y2.shape == (1, 3, 3, 1) and y2[0, 1, 1, 0] == x[0, 1, 3, 0]
+
+
+def test_warmstart_architecture_size_and_shapes():
+    """The network of muphfasa_amd/surrogate.py has ~1 M parameters and its
+    skip connections join tensors of equal shape (checked with the float64
+    restatement on a 32x32 input)."""
+    from muphfasa_amd import surrogate as S
+    nparams = sum(9 * cin * cout + cout for _, cin, cout, _, _, _ in S.LAYERS)
+    assert 0.9e6 < nparams < 1.1e6
+    rng = np.random.default_rng(0)
+    params = {n: (rng.standard_normal((co, 3, 3, ci)) * 0.05, np.zeros(co))
+              for n, ci, co, s, r, src in S.LAYERS}
+    out, act, pre = R.forward(params, S.LAYERS, S.SKIPS, rng.standard_normal((1, 32, 32, 8)))
+    assert out.shape == (1, 32, 32, 8)
+    for name, (a, b) in S.SKIPS.items():
+        assert act[name].shape == act[b].shape
