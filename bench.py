@@ -111,8 +111,10 @@ def main():
 
     out = torch.zeros(4, dtype=torch.float64, device=dev)
     abi.reduce(fa, lattice, abi.RED["mass"], out.data_ptr(), stream)
+    abi.reduce(fa, lattice, abi.RED["momentum"], out[1:].data_ptr(), stream)
     torch.cuda.synchronize()
     mass0 = out[0].item()
+    jx0 = out[1].item()
 
     cur, nxt = fa, fb
     for _ in range(args.warmup):
@@ -134,8 +136,11 @@ def main():
     ms_per_step = ms / args.steps
 
     abi.reduce(cur, lattice, abi.RED["mass"], out.data_ptr(), stream)
+    abi.reduce(cur, lattice, abi.RED["momentum"], out[1:].data_ptr(), stream)
     torch.cuda.synchronize()
     mass1 = out[0].item()
+    # the Guo force adds exactly N^3 F_x of x momentum per step
+    jx_growth = (out[1].item() - jx0) / ((args.warmup + args.steps) * N3 * F[0])
 
     mlups = N3 / (ms_per_step * 1e-3) / 1e6
     bytes_per_lu = 2 * 19 * 8
@@ -180,6 +185,7 @@ def main():
                      "kernel": "k_step<D3Q19,forced,fused>",
                      "algorithmic_bytes_per_launch": N3 * bytes_per_lu},
         "mass_rel_drift": abs(mass1 - mass0) / mass0,
+        "momentum_growth_over_N3F": jx_growth,
         "copy_ceiling_GBs": copy,
         "frac_of_copy_ceiling": round(achieved / copy["nt"], 4) if copy else None,
         "launch": abi.get_launch(lattice),

@@ -115,6 +115,15 @@ int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g,
 int mfa_set_launch(int lattice, const mfa_launch* cfg);
 int mfa_get_launch(int lattice, mfa_launch* cfg);
 
+/* The same for reduced storage (dtype MFA_F32, MFA_F16DEV or MFA_BF16DEV).
+ * Here cells_per_thread 2 means two x-adjacent cells per thread with paired
+ * loads and stores (block_x then counts threads); it is used when n_x is even
+ * and the fields' pointers and strides keep the pairs aligned, otherwise the
+ * one-cell kernel runs.  nt_loads applies to the paired kernel.  No
+ * counterpart in the reference (its populations are FP64 only). */
+int mfa_set_launch_dev(int lattice, int dtype, const mfa_launch* cfg);
+int mfa_get_launch_dev(int lattice, int dtype, mfa_launch* cfg);
+
 /* Literal pullCollideFusedMF (src/LBM/Collision.cpp:95-143): fin must have
  * ngrow >= 1 with ghosts filled (mfa_fill_periodic + mfa_apply_bc). */
 int mfa_pull_collide(const mfa_field* fin, mfa_field* fout, int lattice,

@@ -22,7 +22,8 @@ RED = {"mass": 0, "momentum": 1, "comp_sum": 2}
 
 EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_pull_collide",
            "mfa_fill_periodic", "mfa_apply_bc", "mfa_moments", "mfa_init_equilibrium",
-           "mfa_reduce", "mfa_residual", "mfa_set_launch", "mfa_get_launch", "mfa_stream_copy",
+           "mfa_reduce", "mfa_residual", "mfa_set_launch", "mfa_get_launch", "mfa_set_launch_dev",
+           "mfa_get_launch_dev", "mfa_stream_copy",
            "mfa_convert"]
 
 
@@ -49,6 +50,22 @@ class GeomBC(ctypes.Structure):
                 ("resolved_u", (ctypes.c_double * 3) * 27)]
 
 
+def load_library(path):
+    """dlopen one of this package's libraries.
+
+    torch ships its own HIP runtime under the same soname (libamdhip64.so.7)
+    as /opt/rocm's, and the dynamic loader binds every later user of that
+    soname to whichever copy was loaded first.  torch is imported before the
+    library so that the process has the one runtime torch was built against;
+    loaded the other way round, torch later sees no device.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    return ctypes.CDLL(path)
+
+
 _lib = None
 
 
@@ -60,7 +77,7 @@ def lib():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -m muphfasa_amd.build` "
                 "(there is no CPU fallback for the step engine)")
-        L = ctypes.CDLL(LIB_PATH)
+        L = load_library(LIB_PATH)
         F = ctypes.POINTER(Field)
         G = ctypes.POINTER(GeomBC)
         dp = ctypes.POINTER(ctypes.c_double)
@@ -78,6 +95,8 @@ def lib():
         L.mfa_residual.argtypes = [F, F, vp, vp]
         L.mfa_set_launch.argtypes = [ctypes.c_int, ctypes.POINTER(Launch)]
         L.mfa_get_launch.argtypes = [ctypes.c_int, ctypes.POINTER(Launch)]
+        L.mfa_set_launch_dev.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Launch)]
+        L.mfa_get_launch_dev.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Launch)]
         L.mfa_stream_copy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
         L.mfa_convert.argtypes = [F, F, ctypes.c_int, vp]
         for name in EXPORTS[2:]:
@@ -196,6 +215,22 @@ def set_launch(lattice: int, block_x=None, block_y=1, cells_per_thread=1, nt_loa
 def get_launch(lattice: int):
     c = Launch()
     check(lib().mfa_get_launch(lattice, ctypes.byref(c)))
+    return (c.block_x, c.block_y, c.cells_per_thread, c.nt_loads)
+
+
+def set_launch_dev(lattice: int, dtype: int, block_x=None, block_y=1, cells_per_thread=2, nt_loads=1):
+    """Launch configuration of the reduced-storage step (block_x None resets);
+    cells_per_thread 2 = two x-adjacent cells per thread with paired accesses."""
+    if block_x is None:
+        check(lib().mfa_set_launch_dev(lattice, dtype, None))
+        return
+    c = Launch(block_x, block_y, cells_per_thread, nt_loads)
+    check(lib().mfa_set_launch_dev(lattice, dtype, ctypes.byref(c)))
+
+
+def get_launch_dev(lattice: int, dtype: int):
+    c = Launch()
+    check(lib().mfa_get_launch_dev(lattice, dtype, ctypes.byref(c)))
     return (c.block_x, c.block_y, c.cells_per_thread, c.nt_loads)
 
 

@@ -63,6 +63,53 @@ def tune_step(lattice: str, n, forced: bool = True, periodic: bool = True, steps
     return out
 
 
+CANDIDATES_DEV = [(bx, by, nc, nt) for bx, by in ((256, 1), (128, 2), (64, 4), (64, 2), (128, 1), (64, 1))
+                  for nc, nt in ((1, 0), (2, 0), (2, 1))]
+
+
+def tune_step_dev(lattice: str, dtype: str, n, forced: bool = False, steps: int = 20,
+                  candidates=None, apply: bool = True, dev="cuda:0"):
+    """The same search for reduced storage (FP32 / FP16 / BF16 deviations):
+    block shape, one cell or two x-adjacent cells per thread, non-temporal
+    loads of the pairs."""
+    import torch
+    lid, did = abi.LATTICE[lattice], abi.DTYPE[dtype]
+    D, Q = abi.LATTICE_DQ[lid]
+    n = list(n) + [1] * (3 - len(n))
+    tdt = {"f32": torch.float32, "f16dev": torch.float16, "bf16dev": torch.bfloat16}[dtype]
+    a = torch.zeros((Q, n[2], n[1], n[0]), dtype=tdt, device=dev)
+    b = torch.zeros_like(a)
+    fa, fb = abi.field(a, n, 0, D, did), abi.field(b, n, 0, D, did)
+    g = abi.resolve_bc(lid, [abi.BC["Periodic"]] * (2 * D))
+    F = [1e-6, 0.0, 0.0][:D] if forced else None
+    out = []
+    stream = torch.cuda.current_stream()
+    for cfg in candidates or CANDIDATES_DEV:
+        abi.set_launch_dev(lid, did, *cfg)
+        for _ in range(3):
+            abi.step(fa, fb, g, 1.25, F, stream)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        cur, nxt = fa, fb
+        for _ in range(steps):
+            abi.step(cur, nxt, g, 1.25, F, stream)
+            cur, nxt = nxt, cur
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        cells = n[0] * n[1] * n[2]
+        out.append({"cfg": list(cfg), "ms": ms, "mlups": cells / ms / 1e3,
+                    "GBs": cells * 2 * Q * a.element_size() / ms / 1e6})
+        print(json.dumps({"lattice": lattice, "dtype": dtype, "n": n, **out[-1]}), flush=True)
+    out.sort(key=lambda r: r["ms"])
+    if apply:
+        abi.set_launch_dev(lid, did, *out[0]["cfg"])
+    else:
+        abi.set_launch_dev(lid, did, None)
+    return out
+
+
 def copy_ceiling(nbytes: int = 2 * 1024 ** 3, reps: int = 10, dev="cuda:0"):
     """GB/s (read + write) of mfa_stream_copy, plain and non-temporal."""
     import torch
@@ -85,8 +132,14 @@ def copy_ceiling(nbytes: int = 2 * 1024 ** 3, reps: int = 10, dev="cuda:0"):
 
 
 if __name__ == "__main__":
-    lat = sys.argv[1] if len(sys.argv) > 1 else "D3Q19"
-    size = [int(v) for v in sys.argv[2:]] or [256, 256, 256]
+    # python -m muphfasa_amd.autotune LATTICE [DTYPE] NX NY [NZ]
+    args = sys.argv[1:]
+    lat = args.pop(0) if args else "D3Q19"
+    dtype = args.pop(0) if args and not args[0].isdigit() else "f64"
+    size = [int(v) for v in args] or [256, 256, 256]
     print(json.dumps({"copy_ceiling_GBs": copy_ceiling()}), flush=True)
-    best = tune_step(lat, size)
+    if dtype == "f64":
+        best = tune_step(lat, size)
+    else:
+        best = tune_step_dev(lat, dtype, size)
     print(json.dumps({"best": best[0]}), flush=True)

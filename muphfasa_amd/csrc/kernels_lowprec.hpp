@@ -56,6 +56,41 @@ __device__ __forceinline__ float wall_value_dev(const T* __restrict__ fin, long 
     return __builtin_nanf("");
 }
 
+// Replace the pulled values whose source lies behind a non-periodic face by
+// the wall ghost value (cells on the domain edge only).
+template <class L, class T, class G>
+__device__ __forceinline__ void wall_fix_dev(float (&d)[L::Q], const T* __restrict__ fin, int x, int y,
+                                             int z, const G& g, const View& v)
+{
+    const int p[3] = {x, y, z};
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    bool edge = false;
+#pragma unroll
+    for (int dd = 0; dd < L::D; ++dd) {
+        lo[dd] = !g.per[dd] && p[dd] == 0;
+        hi[dd] = !g.per[dd] && p[dd] == g.n[dd] - 1;
+        edge = edge || lo[dd] || hi[dd];
+    }
+    if (edge) {
+        const long long self = x + y * v.sy + z * v.sz;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            int code[3] = {0, 0, 0};
+            bool wall = false;
+#pragma unroll
+            for (int dd = 0; dd < L::D; ++dd) {
+                const int c = L::c(q, dd);
+                if (c == 1 && lo[dd]) { code[dd] = -1; wall = true; }
+                if (c == -1 && hi[dd]) { code[dd] = 1; wall = true; }
+            }
+            if (wall) {
+                const int idx = (code[0] + 1) + 3 * (code[1] + 1) + (L::D == 3 ? 9 * (code[2] + 1) : 0);
+                d[q] = wall_value_dev<L, q>(fin, self, v, g, idx);
+            }
+        });
+    }
+}
+
 template <class L, bool WALLS, class T, class G>
 __device__ __forceinline__ void pull_dev(float (&d)[L::Q], const T* __restrict__ fin, int x, int y,
                                          int z, const G& g, const View& v)
@@ -66,42 +101,14 @@ __device__ __forceinline__ void pull_dev(float (&d)[L::Q], const T* __restrict__
         const long long off = s.o[0][L::c(q, 0) + 1] + s.o[1][L::c(q, 1) + 1] + s.o[2][L::c(q, 2) + 1];
         d[q] = to_f(ldnt(&fin[q * v.sq + off]));
     });
-    if constexpr (WALLS) {
-        const int p[3] = {x, y, z};
-        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-        bool edge = false;
-#pragma unroll
-        for (int dd = 0; dd < L::D; ++dd) {
-            lo[dd] = !g.per[dd] && p[dd] == 0;
-            hi[dd] = !g.per[dd] && p[dd] == g.n[dd] - 1;
-            edge = edge || lo[dd] || hi[dd];
-        }
-        if (edge) {
-            const long long self = x + y * v.sy + z * v.sz;
-            for_q<L>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                int code[3] = {0, 0, 0};
-                bool wall = false;
-#pragma unroll
-                for (int dd = 0; dd < L::D; ++dd) {
-                    const int c = L::c(q, dd);
-                    if (c == 1 && lo[dd]) { code[dd] = -1; wall = true; }
-                    if (c == -1 && hi[dd]) { code[dd] = 1; wall = true; }
-                }
-                if (wall) {
-                    const int idx = (code[0] + 1) + 3 * (code[1] + 1) + (L::D == 3 ? 9 * (code[2] + 1) : 0);
-                    d[q] = wall_value_dev<L, q>(fin, self, v, g, idx);
-                }
-            });
-        }
-    }
+    if constexpr (WALLS) wall_fix_dev<L>(d, fin, x, y, z, g, v);
 }
 
-// FP32 collision in deviation form and rounded store (RNE) of one cell.
-template <class L, bool FORCED, class T>
-__device__ __forceinline__ void collide_store_dev(const float (&d)[L::Q], T* __restrict__ fout,
-                                                  long long self, const View& vo, const float omega,
-                                                  const float F0, const float F1, const float F2)
+// FP32 collision in deviation form of one cell; out[q] is the post-collision
+// deviation before rounding to the storage type.
+template <class L, bool FORCED>
+__device__ __forceinline__ void collide_dev(const float (&d)[L::Q], float (&o)[L::Q], const float omega,
+                                            const float F0, const float F1, const float F2)
 {
     float drho = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
     for_q<L>([&](auto qc) {
@@ -126,7 +133,21 @@ __device__ __forceinline__ void collide_store_dev(const float (&d)[L::Q], T* __r
         const float fe = (float)L::w(q) * (drho + rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * usq));
         float out = omw * d[q] + omega * fe;
         if constexpr (FORCED) out += guo<L, q, float>(u0, u1, u2, F0, F1, F2, pref, uF);
-        const T sv = from_f<T>(out);
+        o[q] = out;
+    });
+}
+
+// Collision and rounded store (RNE) of one cell.
+template <class L, bool FORCED, class T>
+__device__ __forceinline__ void collide_store_dev(const float (&d)[L::Q], T* __restrict__ fout,
+                                                  long long self, const View& vo, const float omega,
+                                                  const float F0, const float F1, const float F2)
+{
+    float o[L::Q];
+    collide_dev<L, FORCED>(d, o, omega, F0, F1, F2);
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const T sv = from_f<T>(o[q]);
         if constexpr (sizeof(T) == 4) __builtin_nontemporal_store(sv, &fout[q * vo.sq + self]);
         else fout[q * vo.sq + self] = sv;
     });

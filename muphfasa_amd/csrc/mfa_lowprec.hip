@@ -13,7 +13,9 @@
 // 304 B in FP64.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstring>
+#include <mutex>
 
 #include "kernels_lowprec.hpp"
 
@@ -32,6 +34,102 @@ __global__ __launch_bounds__(256) void k_step_dev(const T* __restrict__ fin, T* 
     float d[L::Q];
     pull_dev<L, WALLS>(d, fin, x, y, z, g, vi);
     collide_store_dev<L, FORCED>(d, fout, x + y * vo.sy + z * vo.sz, vo, omega, F0, F1, F2);
+}
+
+// Storage word of T, and two of them loaded or stored as one access.
+template <class T> struct Word;
+template <> struct Word<float> {
+    typedef unsigned int U;
+    typedef unsigned int U2 __attribute__((ext_vector_type(2)));
+};
+template <> struct Word<f16_t> {
+    typedef unsigned short U;
+    typedef unsigned short U2 __attribute__((ext_vector_type(2)));
+};
+template <> struct Word<bf16_t> {
+    typedef unsigned short U;
+    typedef unsigned short U2 __attribute__((ext_vector_type(2)));
+};
+
+template <class T>
+__device__ __forceinline__ float word_to_f(typename Word<T>::U u) { return to_f(__builtin_bit_cast(T, u)); }
+template <class T>
+__device__ __forceinline__ typename Word<T>::U f_to_word(float v)
+{
+    return __builtin_bit_cast(typename Word<T>::U, from_f<T>(v));
+}
+
+// The same update with two x-adjacent cells per thread: each population row
+// is read and written as aligned pairs (4 B for FP16/BF16, 8 B for FP32)
+// instead of single 2- or 4-byte words.  A pull with c_x = +1 needs the pair
+// shifted right by one element: its first element is the last element of the
+// previous lane's pair (cross-lane shift), or, on lane 0 of a wavefront, one
+// extra load; c_x = -1 mirrors this.  Per cell the arithmetic is that of
+// k_step_dev, so the two kernels give identical bits.  Requirements (checked
+// by the host): even n_x, x strides of pairs aligned, blockDim.x a multiple
+// of 64 (a wavefront never spans two rows).
+template <class L, bool FORCED, bool WALLS, bool NT, class T>
+__global__ __launch_bounds__(256) void k_step_dev_x2(const T* __restrict__ fin, T* __restrict__ fout,
+                                                     const Geo g, const View vi, const View vo,
+                                                     const float omega, const float F0, const float F1,
+                                                     const float F2)
+{
+    using U = typename Word<T>::U;
+    using U2 = typename Word<T>::U2;
+    const U* __restrict__ in = (const U*)fin;
+    const int x0 = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    const int nx = g.n[0];
+    if (y >= g.n[1]) return;                 // whole wavefronts only
+    const bool active = x0 < nx;
+    const int lane = threadIdx.x & 63;
+    // x of the extra loads: the element left of x0 and right of x0+1
+    const int xl = x0 == 0 ? (g.per[0] ? nx - 1 : 0) : x0 - 1;
+    const int xr = x0 + 2 >= nx ? (g.per[0] ? 0 : nx - 1) : x0 + 2;
+    const bool edge_l = active && lane == 0;
+    const bool edge_r = active && (lane == 63 || x0 + 2 >= nx);
+    const Src s = make_src<L::D, false>(x0, y, z, g, vi);
+    float da[L::Q], db[L::Q];
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = L::c(q, 0);
+        const long long row = q * vi.sq + s.o[1][L::c(q, 1) + 1] + s.o[2][L::c(q, 2) + 1];
+        U2 p = {0, 0};
+        if (active) {
+            if constexpr (NT) p = __builtin_nontemporal_load((const U2*)&in[row + x0]);
+            else p = *(const U2*)&in[row + x0];
+        }
+        if constexpr (cx == 0) {
+            da[q] = word_to_f<T>(p.x);
+            db[q] = word_to_f<T>(p.y);
+        } else if constexpr (cx == 1) {      // sources x0-1, x0
+            U left = (U)__shfl_up((unsigned int)p.y, 1, 64);
+            if (edge_l) left = in[row + xl];
+            da[q] = word_to_f<T>(left);
+            db[q] = word_to_f<T>(p.x);
+        } else {                             // sources x0+1, x0+2
+            U right = (U)__shfl_down((unsigned int)p.x, 1, 64);
+            if (edge_r) right = in[row + xr];
+            da[q] = word_to_f<T>(p.y);
+            db[q] = word_to_f<T>(right);
+        }
+    });
+    if (!active) return;
+    if constexpr (WALLS) {
+        wall_fix_dev<L>(da, fin, x0, y, z, g, vi);
+        wall_fix_dev<L>(db, fin, x0 + 1, y, z, g, vi);
+    }
+    float oa[L::Q], ob[L::Q];
+    collide_dev<L, FORCED>(da, oa, omega, F0, F1, F2);
+    collide_dev<L, FORCED>(db, ob, omega, F0, F1, F2);
+    U* __restrict__ out = (U*)fout;
+    const long long self = x0 + y * vo.sy + z * vo.sz;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const U2 w = {f_to_word<T>(oa[q]), f_to_word<T>(ob[q])};
+        __builtin_nontemporal_store(w, (U2*)&out[q * vo.sq + self]);
+    });
 }
 
 // Moments of reduced storage, evaluated in FP64 from the dequantised
@@ -208,10 +306,50 @@ static void shape(const int n[3], dim3& grid, dim3& block)
     grid = dim3((n[0] + bx - 1) / bx, (n[1] + block.y - 1) / block.y, n[2]);
 }
 
+// Launch configuration of the reduced-storage step, per lattice and storage
+// type (index dtype - MFA_F32).  cells_per_thread 2 selects k_step_dev_x2
+// (block_x then counts threads, i.e. cell pairs); nt_loads applies to it.
+static std::mutex g_dev_mu;
+static const mfa_launch k_dev_default = {64, 4, 2, 1};
+static mfa_launch g_dev_cfg[4][3] = {
+    {k_dev_default, k_dev_default, k_dev_default}, {k_dev_default, k_dev_default, k_dev_default},
+    {k_dev_default, k_dev_default, k_dev_default}, {k_dev_default, k_dev_default, k_dev_default}};
+
+static mfa_launch dev_cfg(int lat, int dtype)
+{
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    return g_dev_cfg[lat][dtype - MFA_F32];
+}
+
+// Whether two fields can be walked in aligned x pairs.
+template <class T>
+static bool pairs_ok(const mfa_field* f)
+{
+    return f->n[0] % 2 == 0 && (uintptr_t)f->data % (2 * sizeof(T)) == 0 && f->sy % 2 == 0 &&
+           f->sz % 2 == 0 && f->sq % 2 == 0;
+}
+
+template <class L, bool FORCED, class T>
+static void launch_x2(const mfa_launch& c, const Geo& g, hipStream_t st, const T* a, T* b,
+                      const View& vi, const View& vo, float om, float F0, float F1, float F2)
+{
+    const dim3 block(c.block_x, c.block_y, 1);
+    const int pairs = g.n[0] / 2;
+    const dim3 grid((pairs + c.block_x - 1) / c.block_x, (g.n[1] + c.block_y - 1) / c.block_y, g.n[2]);
+    if (g.walls) {
+        if (c.nt_loads) hipLaunchKernelGGL((k_step_dev_x2<L, FORCED, true, true, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
+        else hipLaunchKernelGGL((k_step_dev_x2<L, FORCED, true, false, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
+    } else {
+        if (c.nt_loads) hipLaunchKernelGGL((k_step_dev_x2<L, FORCED, false, true, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
+        else hipLaunchKernelGGL((k_step_dev_x2<L, FORCED, false, false, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
+    }
+}
+
 int step_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, double omega,
              const double* force, hipStream_t st)
 {
     const View vi{fin->sy, fin->sz, fin->sq}, vo{fout->sy, fout->sz, fout->sq};
+    const mfa_launch c = dev_cfg(lat, fin->dtype);
     return with_lattice_dev(lat, [&](auto Lc) {
         using L = decltype(Lc);
         return with_store_type(fin->dtype, [&](auto tv) {
@@ -223,6 +361,16 @@ int step_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doubl
                         F2 = (force && L::D == 3) ? (float)force[2] : 0.f;
             const T* a = (const T*)fin->data;
             T* b = (T*)fout->data;
+            if (c.cells_per_thread == 2 && pairs_ok<T>(fin) && pairs_ok<T>(fout)) {
+                if (force) launch_x2<L, true, T>(c, g, st, a, b, vi, vo, om, F0, F1, F2);
+                else launch_x2<L, false, T>(c, g, st, a, b, vi, vo, om, F0, F1, F2);
+                MFA_HIP_CHECK(hipGetLastError());
+                return (int)MFA_OK;
+            }
+            if (c.cells_per_thread == 1) {
+                block = dim3(c.block_x, c.block_y, 1);
+                grid = dim3((g.n[0] + c.block_x - 1) / c.block_x, (g.n[1] + c.block_y - 1) / c.block_y, g.n[2]);
+            }
             if (force) {
                 if (g.walls) hipLaunchKernelGGL((k_step_dev<L, true, true, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
                 else hipLaunchKernelGGL((k_step_dev<L, true, false, T>), grid, block, 0, st, a, b, g, vi, vo, om, F0, F1, F2);
@@ -324,3 +472,27 @@ int reduce_dev(const mfa_field* f, const Geo& g, int lat, int what, double* part
 }
 
 }  // namespace mfa
+
+using namespace mfa;
+
+int mfa_set_launch_dev(int lattice, int dtype, const mfa_launch* cfg)
+{
+    if (lattice < MFA_D2Q9 || lattice > MFA_D3Q27 || dtype < MFA_F32 || dtype > MFA_BF16DEV)
+        return fail(MFA_ERR_INVALID, "mfa_set_launch_dev: unknown lattice or storage type");
+    const mfa_launch c = cfg ? *cfg : k_dev_default;
+    const bool bx_ok = c.block_x == 64 || c.block_x == 128 || c.block_x == 256;
+    if (!bx_ok || c.block_y < 1 || c.block_x * c.block_y > 256 ||
+        (c.cells_per_thread != 1 && c.cells_per_thread != 2) || (c.nt_loads != 0 && c.nt_loads != 1))
+        return fail(MFA_ERR_INVALID, "mfa_set_launch_dev: unsupported launch configuration");
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    g_dev_cfg[lattice][dtype - MFA_F32] = c;
+    return MFA_OK;
+}
+
+int mfa_get_launch_dev(int lattice, int dtype, mfa_launch* cfg)
+{
+    if (lattice < MFA_D2Q9 || lattice > MFA_D3Q27 || dtype < MFA_F32 || dtype > MFA_BF16DEV || !cfg)
+        return fail(MFA_ERR_INVALID, "mfa_get_launch_dev: bad argument");
+    *cfg = dev_cfg(lattice, dtype);
+    return MFA_OK;
+}

@@ -182,9 +182,11 @@ class Ensemble:
         return self.fa[p0:p0 + 9 * nx * ny].double().cpu().numpy().reshape(9, 1, ny, nx)
 
 
-def run_to_steady(ens: Ensemble, check=1000, tol=1e-8, max_steps=200000, graph=True, log=None):
-    """Advance until every lattice's residual < tol or max_steps.  Returns a
-    dict with per-lattice steps/residual/status and aggregate MLUPS."""
+def run_to_steady(ens: Ensemble, check=1000, tol=1e-8, max_steps=200000, graph=True, log=None,
+                  max_seconds=None):
+    """Advance until every lattice's residual < tol or max_steps (or, if
+    given, max_seconds of wall time).  Returns a dict with per-lattice
+    steps/residual/status and aggregate MLUPS."""
     torch = ens.torch
     steps = np.zeros(ens.nlat, dtype=np.int64)
     res = np.full(ens.nlat, np.inf)
@@ -197,6 +199,8 @@ def run_to_steady(ens: Ensemble, check=1000, tol=1e-8, max_steps=200000, graph=T
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     while ens.active.any() and done_steps < max_steps:
+        if max_seconds is not None and time.perf_counter() - t0 > max_seconds:
+            break
         k = min(check, max_steps - done_steps)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)

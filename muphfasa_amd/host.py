@@ -8,6 +8,8 @@ import os
 
 import numpy as np
 
+from . import abi
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libmuphfasa_host.so")
 EXE_PATH = os.path.join(HERE, "lib", "muphfasa")
@@ -23,7 +25,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build with `python -m muphfasa_amd.host_build`")
-        L = ctypes.CDLL(LIB_PATH)
+        L = abi.load_library(LIB_PATH)
         cp = ctypes.c_char_p
         L.mfa_sim_error.restype = cp
         L.mfa_sim_create.restype = ctypes.c_void_p

@@ -30,7 +30,8 @@ def _log(**kw):
     print(json.dumps(kw), flush=True)
 
 
-def generate(n, seed, dtype="f32", tol=1e-6, max_steps=60000, check=1000, N=256, tile=(64, 4)):
+def generate(n, seed, dtype="f32", tol=1e-6, max_steps=60000, check=1000, N=256, tile=(64, 4),
+             max_seconds=None):
     """Returns masks (n,N,N) uint8, Re (n,), fields on device (rho, vel) and
     the run statistics."""
     masks, Re, tau = S.make_cases(n, seed, N)
@@ -41,13 +42,16 @@ def generate(n, seed, dtype="f32", tol=1e-6, max_steps=60000, check=1000, N=256,
     def log(s, nact, rmax):
         if s % (10 * check) == 0:
             _log(stage="generate", step=s, active=nact, max_res=rmax, t=round(time.time() - t0, 1))
-    r = E.run_to_steady(ens, check, tol, max_steps, graph=True, log=log)
+    r = E.run_to_steady(ens, check, tol, max_steps, graph=True, log=log, max_seconds=max_seconds)
     ens.set_active(np.ones(n, np.int32))
     ens.moments()
     stats = {"n": n, "tol": tol, "max_steps": max_steps, "dtype": dtype,
              "converged": int((r["status"] == "converged").sum()),
              "diverged": int((r["status"] == "diverged").sum()),
              "median_steps": float(np.median(r["steps"])), "gpu_s": round(r["gpu_s"], 2),
+             "unconverged_residual_p50_p90": [float(x) for x in np.percentile(
+                 r["residual"][r["status"] == "max_steps"], [50, 90])]
+             if (r["status"] == "max_steps").any() else None,
              "mlups": round(r["mlups_gpu"], 1)}
     return masks, Re, ens, r, stats
 
@@ -163,7 +167,7 @@ def fields_on_device(pred, masks_dev, u_lid=0.1):
 
 
 def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, dtype="f64", N=256,
-             tile=(64, 4)):
+             tile=(64, 4), max_seconds=None):
     import torch
     masks, Re, tau = S.make_cases(n_eval, seed, N)
     masks_dev = torch.from_numpy(masks).to(net.dev)
@@ -181,7 +185,8 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
             torch.cuda.synchronize()
             t_inf = time.perf_counter() - ti
             ens.init(rho, vel)
-            del pred
+            vel_pred = vel
+            del pred, rho
         else:
             ens.init()
 
@@ -189,7 +194,7 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
             if s % (20 * check) == 0:
                 _log(stage="eval", mode=mode, step=s, active=nact, max_res=rmax,
                      t=round(time.perf_counter() - t0, 1))
-        r = E.run_to_steady(ens, check, tol, max_steps, graph=True, log=log)
+        r = E.run_to_steady(ens, check, tol, max_steps, graph=True, log=log, max_seconds=max_seconds)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ens.set_active(np.ones(n_eval, np.int32))
@@ -204,6 +209,17 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
     dv = (cw["vel"] - cc["vel"]).abs()
     dmax = torch.maximum(dv[:tc], dv[tc:]).view(n_eval, -1).max(dim=1).values.cpu().numpy() / 0.1
     ratio = cc["steps"][both] / np.maximum(cw["steps"][both], 1)
+    # surrogate error against the converged cold field, per case (max over
+    # fluid cells of |u_pred - u| / u_lid, and relative L2)
+    dp = (vel_pred - cc["vel"]).view(2, n_eval, -1)
+    uc = cc["vel"].view(2, n_eval, -1)
+    pred_max = dp.abs().amax(dim=(0, 2)).cpu().numpy() / 0.1
+    pred_l2 = torch.sqrt((dp * dp).sum(dim=(0, 2)) / (uc * uc).sum(dim=(0, 2)).clamp_min(1e-300))
+    pred_l2 = pred_l2.cpu().numpy()
+    per_case = {"Re": Re, "steps_warm": cw["steps"], "steps_cold": cc["steps"],
+                "status_warm": cw["status"].astype(str), "status_cold": cc["status"].astype(str),
+                "pred_err_max": pred_max, "pred_err_l2": pred_l2, "field_diff_max": dmax,
+                "solid_cells": masks.reshape(n_eval, -1).sum(axis=1)}
     summary = {
         "n_eval": n_eval, "tol": tol, "max_steps": max_steps, "dtype": dtype,
         "cold_converged": int((cc["status"] == "converged").sum()),
@@ -217,8 +233,10 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
         "wall_ratio": round(cc["wall_s"] / cw["wall_s"], 3),
         "field_diff_max_over_both": float(dmax[both].max()) if both.any() else None,
         "field_diff_median": float(np.median(dmax[both])) if both.any() else None,
+        "pred_err_l2_median": float(np.median(pred_l2[both])) if both.any() else None,
+        "pred_err_max_median": float(np.median(pred_max[both])) if both.any() else None,
     }
-    return summary, res, (masks, Re)
+    return summary, per_case
 
 
 def main():
@@ -227,6 +245,7 @@ def main():
     p.add_argument("--n-train", type=int, default=4096)
     p.add_argument("--gen-tol", type=float, default=1e-6)
     p.add_argument("--gen-max-steps", type=int, default=60000)
+    p.add_argument("--gen-seconds", type=float, default=None, help="wall budget of the generation")
     p.add_argument("--epochs", type=int, default=20)
     p.add_argument("--batch", type=int, default=16)
     p.add_argument("--lr", type=float, default=2e-3)
@@ -236,16 +255,29 @@ def main():
     p.add_argument("--eval-max-steps", type=int, default=200000)
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--eval-out", default="", help="per-case results (.npz)")
+    p.add_argument("--eval-seconds", type=float, default=None, help="wall budget per start mode")
     a = p.parse_args()
     import torch
     net = S.WarmStartNet(seed=a.seed)
     _log(stage="init", params=net.nparams)
     if a.stage in ("train", "all"):
-        masks, Re, ens, r, stats = generate(a.n_train, a.seed, tol=a.gen_tol, max_steps=a.gen_max_steps)
+        masks, Re, ens, r, stats = generate(a.n_train, a.seed, tol=a.gen_tol, max_steps=a.gen_max_steps,
+                                            max_seconds=a.gen_seconds)
         _log(stage="generate_done", **stats)
         targets = device_targets(ens, a.n_train)
         del ens
         torch.cuda.empty_cache()
+        # a lattice that blew up (BGK at tau near 1/2 with a one-cell gap under
+        # the lid, for instance) has no steady state to learn: drop it
+        keep = torch.isfinite(targets).view(a.n_train, -1).all(dim=1).cpu().numpy()
+        keep &= r["status"] != "diverged"
+        if not keep.all():
+            _log(stage="generate_dropped", count=int((~keep).sum()),
+                 cases=[{"i": int(i), "Re": float(Re[i])} for i in np.nonzero(~keep)[0][:20]])
+            idx = torch.from_numpy(np.nonzero(keep)[0]).to(net.dev)
+            targets = targets.index_select(0, idx).contiguous()
+            masks, Re = masks[keep], Re[keep]
         masks_dev = torch.from_numpy(masks).to(net.dev)
         re_dev = torch.from_numpy(S.re_feature(Re)).to(net.dev)
         hist, t_train = train(net, masks_dev, re_dev, targets, a.epochs, a.batch, a.lr, a.seed,
@@ -259,9 +291,11 @@ def main():
     if a.stage in ("eval", "all"):
         if a.stage == "eval":
             net.load(a.weights)
-        summary, res, _ = evaluate(net, a.n_eval, seed=a.seed + 1, tol=a.tol,
-                                   max_steps=a.eval_max_steps)
+        summary, per_case = evaluate(net, a.n_eval, seed=a.seed + 1, tol=a.tol,
+                                     max_steps=a.eval_max_steps, max_seconds=a.eval_seconds)
         _log(stage="eval_done", **summary)
+        if a.eval_out:
+            np.savez(a.eval_out, **per_case)
 
 
 if __name__ == "__main__":

@@ -86,3 +86,26 @@ def test_invalid_arguments_fail_before_launch():
         abi.check(abi.lib().mfa_pull_collide(ctypes.byref(f), ctypes.byref(f), 2, 1.0, None,
                                              None))
     assert e.value.code in (-1, -2)
+
+
+def test_launch_configuration_tables():
+    """mfa_set_launch / mfa_set_launch_dev validate, store and reset (host-only)."""
+    lat = abi.LATTICE["D3Q27"]
+    default = abi.get_launch(lat)
+    abi.set_launch(lat, 128, 2, 1, 1)
+    assert abi.get_launch(lat) == (128, 2, 1, 1)
+    for bad in ((96, 1, 1, 0), (256, 2, 1, 0), (64, 1, 3, 0), (64, 1, 1, 2)):
+        with pytest.raises(abi.MfaError):
+            abi.set_launch(lat, *bad)
+    abi.set_launch(lat)
+    assert abi.get_launch(lat) == default
+    for dt in ("f32", "f16dev", "bf16dev"):
+        d = abi.get_launch_dev(lat, abi.DTYPE[dt])
+        abi.set_launch_dev(lat, abi.DTYPE[dt], 128, 2, 1, 0)
+        assert abi.get_launch_dev(lat, abi.DTYPE[dt]) == (128, 2, 1, 0)
+        with pytest.raises(abi.MfaError):
+            abi.set_launch_dev(lat, abi.DTYPE[dt], 32, 1, 2, 1)
+        abi.set_launch_dev(lat, abi.DTYPE[dt])
+        assert abi.get_launch_dev(lat, abi.DTYPE[dt]) == d
+    with pytest.raises(abi.MfaError):
+        abi.get_launch_dev(lat, abi.DTYPE["f64"])

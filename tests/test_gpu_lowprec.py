@@ -159,3 +159,50 @@ def test_tgv_error_against_fp64(dev, dt):
     print(f"{dt}: max |u - u_fp64| / u0 = {err:.3e}; mass drift {abs(l1 - l0) / l0:.3e}")
     assert err < TGV_BOUND[dt]
     assert np.isfinite(vlp).all()
+
+
+PAIR_CASES = {
+    # name: (n for 3-D, n for 2-D, boundary set, forced, launches to compare)
+    "periodic_forced": ([40, 10, 7], [40, 17, 1], "periodic", True, [(64, 4, 1), (128, 2, 0)]),
+    "lid_walls": ([24, 10, 7], [24, 17, 1], "lid", False, [(64, 4, 1)]),
+    "multiwave_periodic": ([260, 6, 3], [260, 9, 1], "periodic", False, [(64, 4, 1), (256, 1, 1)]),
+    "multiwave_walls": ([258, 5, 3], [258, 7, 1], "inflow_pressure", True, [(64, 2, 0), (128, 1, 1)]),
+    "odd_nx_falls_back": ([23, 6, 5], [23, 9, 1], "noslip", False, [(64, 4, 1)]),
+}
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+@pytest.mark.parametrize("dt", ["f32", "f16dev", "bf16dev"])
+@pytest.mark.parametrize("case", list(PAIR_CASES))
+def test_paired_kernel_is_bit_identical_to_one_cell_kernel(dev, lat, dt, case):
+    """Two x-adjacent cells per thread (paired loads, cross-lane shifts, extra
+    loads at wavefront and row ends) against one cell per thread."""
+    import torch
+    from tests.gpu_util import bc_tables
+    L = O.Lattice(lat)
+    n3, n2, bcs, forced, launches = PAIR_CASES[case]
+    n = n3 if L.D == 3 else n2
+    rng = np.random.default_rng(11)
+    f64 = L.w[:, None, None, None] * (1 + 0.05 * rng.standard_normal((L.Q, n[2], n[1], n[0])))
+    src = torch.from_numpy(f64).to(dev)
+    a = _dev_tensor(torch, L.Q, n, dt, dev)
+    fa = abi.field(a, n, 0, L.D, abi.DTYPE[dt])
+    abi.convert(abi.field(src, n, 0, L.D), fa, abi.LATTICE[lat])
+    g, _, _ = bc_tables(L, bcs)
+    F = [2e-4, -1e-4, 5e-5][:L.D] if forced else None
+    lid, did = abi.LATTICE[lat], abi.DTYPE[dt]
+    outs = []
+    try:
+        for bx, by, nt in [(64, 4, None)] + launches:
+            b = torch.full_like(a, float("nan"))
+            if nt is None:
+                abi.set_launch_dev(lid, did, bx, by, 1, 0)
+            else:
+                abi.set_launch_dev(lid, did, bx, by, 2, nt)
+            abi.step(fa, abi.field(b, n, 0, L.D, abi.DTYPE[dt]), g, 1.0 / 0.6, F)
+            torch.cuda.synchronize()
+            outs.append(b.view(torch.int16 if b.element_size() == 2 else torch.int32).cpu().numpy())
+    finally:
+        abi.set_launch_dev(lid, did)
+    for k, o in enumerate(outs[1:]):
+        assert np.array_equal(o, outs[0]), (case, launches[k])
