@@ -392,13 +392,13 @@ static void launch_shape(const int n[3], dim3& grid, dim3& block, int grow)
 
 // Launch configuration of the fused step kernel, per lattice index (D2Q9,
 // D3Q15, D3Q19, D3Q27), set by the autotuner through mfa_set_launch.  The
-// defaults are the autotuner's picks on MI355X for D2Q9 128^2, D3Q19 256^3 and
-// D3Q27 192^3 FP64 forced periodic; D3Q15 takes D3Q19's.
+// defaults are the autotuner's picks on MI355X for D2Q9 128^2, D3Q19 256^3
+// FP64 forced periodic and D3Q27 384^3 FP64 periodic; D3Q15 takes D3Q19's.
 static std::mutex g_cfg_mu;
 static const mfa_launch k_default_cfg[4] = {
-    {256, 1, 1, 0}, {256, 1, 1, 1}, {256, 1, 1, 1}, {64, 4, 1, 0}};
+    {256, 1, 1, 0}, {256, 1, 1, 1}, {256, 1, 1, 1}, {128, 1, 2, 0}};
 static mfa_launch g_cfg[4] = {
-    {256, 1, 1, 0}, {256, 1, 1, 1}, {256, 1, 1, 1}, {64, 4, 1, 0}};
+    {256, 1, 1, 0}, {256, 1, 1, 1}, {256, 1, 1, 1}, {128, 1, 2, 0}};
 
 static bool valid_cfg(const mfa_launch& c)
 {

@@ -28,24 +28,31 @@ import numpy as np
 from . import abi
 
 LAYERS = [  # name, cin, cout, stride, relu, input
-    ("e1", 8, 32, 1, True, "x0"),
-    ("e1b", 32, 32, 1, True, "e1"),
-    ("d1", 32, 64, 2, True, "e1b"),
-    ("e2", 64, 64, 1, True, "d1"),
-    ("d2", 64, 128, 2, True, "e2"),
-    ("e3", 128, 128, 1, True, "d2"),
-    ("d3", 128, 128, 2, True, "e3"),
-    ("b1", 128, 128, 1, True, "d3"),
-    ("b2", 128, 128, 1, True, "b1"),
-    ("c3", 128, 128, 1, True, "u3"),
-    ("c3p", 128, 64, 1, True, "c3"),
-    ("c2", 64, 64, 1, True, "u2"),
-    ("c2p", 64, 32, 1, True, "c2"),
-    ("c1", 32, 32, 1, True, "u1"),
-    ("out", 32, 8, 1, False, "c1"),
+    ("e1", 8, 16, 1, True, "x0"),         # 256^2
+    ("e1b", 16, 16, 1, True, "e1"),
+    ("d1", 16, 32, 2, True, "e1b"),       # 128^2
+    ("e2", 32, 32, 1, True, "d1"),
+    ("d2", 32, 64, 2, True, "e2"),        # 64^2
+    ("e3", 64, 64, 1, True, "d2"),
+    ("d3", 64, 96, 2, True, "e3"),        # 32^2
+    ("e4", 96, 96, 1, True, "d3"),
+    ("d4", 96, 120, 2, True, "e4"),       # 16^2
+    ("e5", 120, 120, 1, True, "d4"),
+    ("d5", 120, 120, 2, True, "e5"),      # 8^2: the bottleneck sees the whole cavity
+    ("b1", 120, 120, 1, True, "d5"),
+    ("b2", 120, 120, 1, True, "b1"),
+    ("c5", 120, 96, 1, True, "u5"),       # 16^2
+    ("c4", 96, 64, 1, True, "u4"),        # 32^2
+    ("c3", 64, 32, 1, True, "u3"),        # 64^2
+    ("c2", 32, 16, 1, True, "u2"),        # 128^2
+    ("c1", 16, 16, 1, True, "u1"),        # 256^2
+    ("out", 16, 8, 1, False, "c1"),
 ]
 # additive skips: name -> (upsampled tensor, skip tensor)
-SKIPS = {"u3": ("b2", "e3"), "u2": ("c3p", "e2"), "u1": ("c2p", "e1b")}
+SKIPS = {"u5": ("b2", "e5"), "u4": ("c5", "e4"), "u3": ("c4", "e3"), "u2": ("c3", "e2"),
+         "u1": ("c2", "e1b")}
+# scaled targets: (rho - 1) / (RHO_SCALE u_lid^2), u / u_lid
+RHO_SCALE = 10.0
 # tensors that are ReLU outputs (their gradient is masked by value > 0)
 RELU_OUT = {name for name, _, _, _, relu, _ in LAYERS if relu}
 
@@ -307,13 +314,13 @@ def make_cases(n, seed, N=256, re_range=(100.0, 3200.0), u_lid=0.1):
 
 def targets_from_fields(rho, vel, u_lid=0.1):
     """(n,H,W) rho, (n,2,H,W) u -> (n,H,W,3) float32 scaled targets."""
-    t = np.stack([(rho - 1.0) / u_lid ** 2, vel[:, 0] / u_lid, vel[:, 1] / u_lid], axis=-1)
+    t = np.stack([(rho - 1.0) / (RHO_SCALE * u_lid ** 2), vel[:, 0] / u_lid, vel[:, 1] / u_lid], axis=-1)
     return t.astype(np.float32)
 
 
 def fields_from_prediction(pred, u_lid=0.1):
     """(n,H,W,>=3) scaled prediction -> rho (n,H,W), u (n,2,H,W) float64."""
     p = np.asarray(pred, dtype=np.float64)
-    rho = 1.0 + p[..., 0] * u_lid ** 2
+    rho = 1.0 + p[..., 0] * (RHO_SCALE * u_lid ** 2)
     u = np.stack([p[..., 1], p[..., 2]], axis=1) * u_lid
     return rho, u

@@ -63,7 +63,7 @@ def device_targets(ens, n, N=256, u_lid=0.1):
     rho = ens.rho.view(n, N, N)
     ux = ens.vel[:tc].view(n, N, N)
     uy = ens.vel[tc:].view(n, N, N)
-    t = torch.stack([(rho - 1.0) / u_lid ** 2, ux / u_lid, uy / u_lid], dim=-1)
+    t = torch.stack([(rho - 1.0) / (S.RHO_SCALE * u_lid ** 2), ux / u_lid, uy / u_lid], dim=-1)
     return t.to(torch.float32).contiguous()
 
 
@@ -156,11 +156,19 @@ def predict(net, masks_dev, re_feat_dev, batch=16):
 
 
 def fields_on_device(pred, masks_dev, u_lid=0.1):
-    """rho (n*N*N,), vel (2*n*N*N,) float64 in the ensemble's cell order."""
+    """rho (n*N*N,), vel (2*n*N*N,) float64 in the ensemble's cell order.
+
+    The predicted density is shifted so that each cavity's fluid holds the
+    mass of the cold start (rho = 1 per fluid cell): a closed cavity keeps its
+    initial mass, and its steady velocity scales with it, so a start with any
+    other mass would converge to a different field."""
     import torch
     p = pred.double()
     fluid = (masks_dev == 0).double()
-    rho = (1.0 + p[..., 0] * u_lid ** 2) * fluid + (1 - fluid)
+    drho = p[..., 0] * (S.RHO_SCALE * u_lid ** 2) * fluid
+    nfl = fluid.sum(dim=(1, 2)).clamp_min(1.0)
+    drho = drho - (drho.sum(dim=(1, 2)) / nfl)[:, None, None] * fluid
+    rho = 1.0 + drho
     ux = p[..., 1] * u_lid * fluid
     uy = p[..., 2] * u_lid * fluid
     return rho.reshape(-1).contiguous(), torch.cat([ux.reshape(-1), uy.reshape(-1)]).contiguous()

@@ -28,7 +28,10 @@ def _rand(rng, shape, scale=1.0):
 
 
 SHAPES = [(2, 9, 7, 8, 32, 1), (1, 16, 16, 32, 64, 2), (2, 8, 12, 64, 32, 1),
-          (1, 10, 10, 128, 128, 2), (1, 6, 6, 32, 8, 1), (3, 5, 11, 16, 24, 1)]
+          (1, 10, 10, 128, 128, 2), (1, 6, 6, 32, 8, 1), (3, 5, 11, 16, 24, 1),
+          # the warm-start network's deep layers on small maps
+          (2, 4, 4, 96, 120, 2), (2, 2, 2, 120, 120, 2), (2, 1, 1, 120, 120, 1),
+          (2, 2, 2, 120, 96, 1), (1, 3, 5, 120, 120, 2), (2, 16, 16, 16, 8, 1)]
 
 
 @pytest.mark.parametrize("N,H,W,Cin,Cout,stride", SHAPES)
@@ -87,23 +90,28 @@ def _net_params_np(net):
 
 
 def test_network_gradient_against_float64_reference(dev):
+    """Whole-network weight gradients against float64.  The GPU rounds every
+    activation and data gradient to bf16, so agreement improves with the
+    number of pixels each weight gradient sums over; at 128^2 the deepest
+    layers (8^2 and 4^2 maps) still sum over enough of them."""
     import torch
-    net = S.WarmStartNet(H=32, W=32, seed=3)
+    H = 128
+    net = S.WarmStartNet(H=H, W=H, seed=3)
     rng = np.random.default_rng(0)
-    solid = np.zeros((2, 32, 32), np.uint8)
-    solid[0, 10:16, 4:20] = 1
+    solid = np.zeros((2, H, H), np.uint8)
+    solid[0, 40:64, 16:80] = 1
     re = np.array([0.3, -0.7], np.float32)
-    x0 = torch.empty((2, 32, 32, 8), dtype=torch.bfloat16, device=dev)
+    x0 = torch.empty((2, H, H, 8), dtype=torch.bfloat16, device=dev)
     st = torch.from_numpy(solid).to(dev)
-    S._ck(S.lib().mfa_nn_make_input(S._p(st), S._p(torch.from_numpy(re).to(dev)), S._p(x0), 2, 32,
-                                    32, S._s()))
-    target = rng.standard_normal((2, 32, 32, 3)).astype(np.float32)
+    S._ck(S.lib().mfa_nn_make_input(S._p(st), S._p(torch.from_numpy(re).to(dev)), S._p(x0), 2, H,
+                                    H, S._s()))
+    target = rng.standard_normal((2, H, H, 3)).astype(np.float32)
     tt = torch.from_numpy(target).to(dev)
     out = net.forward(x0)
-    g = torch.empty((2, 32, 32, 8), dtype=torch.bfloat16, device=dev)
-    part = torch.zeros(1024, dtype=torch.float32, device=dev)
+    g = torch.empty((2, H, H, 8), dtype=torch.bfloat16, device=dev)
+    part = torch.zeros((2 * H * H + 255) // 256, dtype=torch.float32, device=dev)
     count = float((solid == 0).sum() * 3)
-    S._ck(S.lib().mfa_nn_mse(S._p(out), S._p(tt), S._p(st), S._p(g), S._p(part), 2 * 32 * 32, 8, 3,
+    S._ck(S.lib().mfa_nn_mse(S._p(out), S._p(tt), S._p(st), S._p(g), S._p(part), 2 * H * H, 8, 3,
                              1.0 / count, S._s()))
     net.zero_grad()
     net.backward(g)
@@ -116,13 +124,16 @@ def test_network_gradient_against_float64_reference(dev):
     grads = R.backward(params, S.LAYERS, S.SKIPS, act, pre, g_ref)
     assert abs(loss_gpu - loss_ref) <= 2e-2 * loss_ref
     assert np.abs(_to_np(out) - o).max() <= 3e-2 * np.abs(o).max()
+    bad = []
     for name, l in net.layers.items():
         gw = l["gw"].cpu().numpy().reshape(-1).astype(np.float64)
         rw = grads[name][0].reshape(-1)
         cos = gw @ rw / (np.linalg.norm(gw) * np.linalg.norm(rw) + 1e-300)
         rel = np.linalg.norm(gw - rw) / (np.linalg.norm(rw) + 1e-300)
         print(f"{name}: cos={cos:.5f} rel={rel:.3e}")
-        assert cos > 0.995 and rel < 0.1, name
+        if not (cos > 0.995 and rel < 0.1):
+            bad.append(name)
+    assert not bad, bad
 
 
 def test_training_reduces_loss(dev):

@@ -1,0 +1,90 @@
+"""What any warm start can gain under the 1e-8 steady criterion, measured
+with the oracle on a small lid cavity (64^2, Re = 100, lid u = 0.1, the
+criterion of muphfasa_amd.ensemble.run_to_steady: relative L2 change of u
+over 1 000 steps below 1e-8).
+
+* Starting from the equilibrium of the EXACT steady (rho, u) still takes a
+  large fraction of the cold start's steps: the populations' non-equilibrium
+  part is missing and has to relax through the slow modes.  This bounds the
+  speed-up of any predictor whose output initialises feq (the warm start
+  here): it is below 3x on this case even for a perfect prediction.
+* A closed cavity keeps its initial mass and its steady velocity scales with
+  it: a start whose mass differs from the cold start's converges to a
+  different field, the same start with its mass restored does not.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+N, RE, U = 64, 100.0, 0.1
+L = O.Lattice("D2Q9")
+n = [N, N]
+TAU = 0.5 + 3 * U * N / RE
+C = L.c[:, :2].astype(float)
+
+
+def _res():
+    fu = np.zeros((6, 3))
+    fu[3, 0] = U
+    return O.resolve_bc(2, [1, 1, 1, 0], np.ones(6), fu)
+
+
+def _fab(rho, u):
+    f = O.new_fab(L, n)
+    v = O.valid(L, f)
+    usq = u[0] ** 2 + u[1] ** 2
+    for q in range(L.Q):
+        cu = C[q, 0] * u[0] + C[q, 1] * u[1]
+        v[q, 0] = L.w[q] * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * usq)
+    O.fill_periodic(L, f, n, [0, 0, 0])
+    O.apply_bc(L, f, n, [0, 0, 0], _res())
+    return f
+
+
+def _to_steady(f, tol=1e-8, max_steps=200000):
+    res = _res()
+    _, u = O.moments(L, f, n)
+    prev = np.asarray(u).copy()
+    steps = 0
+    while steps < max_steps:
+        f = O.run(L, f, n, 1.0 / TAU, None, [0, 0, 0], res, 1000)
+        steps += 1000
+        _, u = O.moments(L, f, n)
+        u = np.asarray(u)
+        r = np.sqrt(((u - prev) ** 2).sum() / (u ** 2).sum())
+        prev = u.copy()
+        if r < tol:
+            break
+    rho, u = O.moments(L, f, n)
+    return steps, np.asarray(rho).reshape(N, N), np.asarray(u).reshape(2, N, N)
+
+
+@pytest.fixture(scope="module")
+def cold():
+    return _to_steady(_fab(np.ones((N, N)), np.zeros((2, N, N))))
+
+
+def test_equilibrium_start_from_exact_steady_fields_is_below_3x(cold):
+    s_cold, rho, u = cold
+    assert 10000 < s_cold < 100000
+    s_warm, _, u_w = _to_steady(_fab(rho, u))
+    print(f"cold {s_cold} steps, feq(exact steady) {s_warm} steps: {s_cold / s_warm:.2f}x")
+    assert s_warm < s_cold                     # it helps ...
+    assert s_cold / s_warm < 3.0               # ... but not 3x, even with a perfect prediction
+    assert np.abs(u_w - u).max() < 1e-6 * U    # and lands on the same field
+
+
+def test_mass_of_the_start_selects_the_steady_field(cold):
+    s_cold, rho, u = cold
+    rng = np.random.default_rng(1)
+    bump = np.outer(np.sin(np.linspace(0, np.pi, N)), np.cos(np.linspace(0, 2 * np.pi, N)))
+    rho_err = rho + 0.3 * np.abs(rho - 1).max() * bump + 1e-3 * rng.uniform()  # wrong field, wrong mass
+    _, _, u_off = _to_steady(_fab(rho_err, u))
+    rho_fix = rho_err - rho_err.mean() + rho.mean()                            # mass restored
+    _, _, u_fix = _to_steady(_fab(rho_fix, u))
+    d_off = np.abs(u_off - u).max() / U
+    d_fix = np.abs(u_fix - u).max() / U
+    print(f"max |u - u_cold| / U: mass off {d_off:.2e}, mass restored {d_fix:.2e}")
+    assert d_off > 1e-4
+    assert d_fix < 1e-5
