@@ -81,3 +81,36 @@ def test_parmparse_last_definition_wins_and_comments():
     assert u["Re"] == 400.0
     u2 = host.inputs_units(text, ["units.Re=100"])     # command-line override wins
     assert u2["Re"] == 100.0
+
+
+def _inputinfo(*args):
+    import subprocess
+    exe = os.path.join(ROOT, "muphfasa_amd", "lib", "muphfasa_inputinfo")
+    return subprocess.run([exe, *args], capture_output=True, text=True, timeout=30)
+
+
+def test_inputinfo_summary_matches_unit_converter():
+    """The inspector reports the solver's own lattice units and step count."""
+    r = _inputinfo(os.path.join(INP, "lid_cavity_128"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    u = host.inputs_units(_read("lid_cavity_128"))
+    tau = float(re.search(r"^\s+tau\s+(\S+)", r.stdout, re.M).group(1))
+    dt = float(re.search(r"^\s+dt\s+(\S+) s", r.stdout, re.M).group(1))
+    assert abs(tau - u["tau"]) <= 1e-5 * u["tau"] and abs(dt - u["dt"]) <= 1e-5 * u["dt"]
+    assert re.search(r"steps\s+10000\s+\(limit: sim.max_step\)", r.stdout)
+    assert "lattice (0.1, 0)" in r.stdout and "accepts" in r.stdout
+
+
+def test_inputinfo_advisories_and_refusals():
+    # Re = 10^4 at 128 cells: tau 0.5038, no admissible u_lat: refinement advice, still accepted
+    r = _inputinfo(os.path.join(INP, "lid_cavity_128"), "units.Re=10000")
+    assert r.returncode == 0 and "[!] tau = 0.5038" in r.stdout and "refine every direction" in r.stdout
+    # tau above the ideal range with a feasible fix: the suggested u_lat lands inside it
+    r = _inputinfo(os.path.join(INP, "lid_cavity_128"), "units.Re=10", "lbm.u_lat=0.05")
+    m = re.search(r"suggest lbm.u_lat = (\S+) \(tau (\S+)\)", r.stdout)
+    assert m and 0.6 <= float(m.group(2)) <= 1.5, r.stdout
+    # the solver's hard limit (u_lat >= 0.3) and an unknown lattice: exit status 1
+    assert _inputinfo(os.path.join(INP, "lid_cavity_128"), "lbm.u_lat=0.35").returncode == 1
+    assert _inputinfo(os.path.join(INP, "periodic_d3q19_256"), "lbm.lattice_type=D3Q7").returncode == 1
+    r = _inputinfo(os.path.join(INP, "lid_cavity_128"), "units.nu_phys=0.1")
+    assert r.returncode == 1 and "Both units.Re and units.nu_phys" in r.stdout
