@@ -80,7 +80,8 @@ typedef struct mfa_geom_bc {
 /* Launch configuration of mfa_step's kernel (per lattice, process-wide):
  * block_x in {64,128,256} cells along x, block_y rows per block
  * (block_x*block_y <= 256), cells_per_thread in {1,2} along z (3-D only),
- * nt_loads: non-temporal population loads.  Chosen by the autotuner
+ * nt_loads: 0 cached population loads, 1 non-temporal, 2 non-temporal only
+ * for populations with c_x = 0.  Chosen by the autotuner
  * (muphfasa_amd/autotune.py); no counterpart in the reference, whose launch
  * shapes are AMReX's. */
 typedef struct mfa_launch {

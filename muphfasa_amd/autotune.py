@@ -15,9 +15,9 @@ import sys
 from . import abi
 
 CANDIDATES_3D = [(bx, by, nc, nt) for bx, by in ((256, 1), (128, 2), (64, 4), (64, 2), (128, 1))
-                 for nc in (1, 2) for nt in (0, 1)]
+                 for nc in (1, 2) for nt in (0, 1, 2)]
 CANDIDATES_2D = [(bx, by, 1, nt) for bx, by in ((256, 1), (128, 2), (64, 4), (64, 2), (64, 1))
-                 for nt in (0, 1)]
+                 for nt in (0, 1, 2)]
 
 
 def tune_step(lattice: str, n, forced: bool = True, periodic: bool = True, steps: int = 20,

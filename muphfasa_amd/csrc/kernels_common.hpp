@@ -153,7 +153,9 @@ __device__ __forceinline__ void for_q(Fn&& fn) { Unroll<L, 0>::run(fn); }
 // Pull the Q populations of cell (x,y,z).  mode: 0 fused (inline wrap and
 // walls), 1 halo (reads ghost cells), 2 unstreamed (the cell's own values).
 // WALLS: some dimension is non-periodic (compile-time, so the periodic
-// kernel carries no boundary code).  NT: non-temporal loads.
+// kernel carries no boundary code).  NT: 0 cached loads, 1 non-temporal
+// loads, 2 non-temporal only where c_x = 0 (rows read x-shifted share cache
+// lines with the neighbouring wavefront, which cached loads let it reuse).
 template <bool NT>
 __device__ __forceinline__ double ld(const double* p)
 {
@@ -161,7 +163,7 @@ __device__ __forceinline__ double ld(const double* p)
     else return *p;
 }
 
-template <class L, int MODE, bool WALLS = true, bool NT = false, class G = Geo>
+template <class L, int MODE, bool WALLS = true, int NT = 0, class G = Geo>
 __device__ __forceinline__ void pull(double (&f)[L::Q], const double* __restrict__ fin, int x, int y,
                                      int z, const G& g, const View& v)
 {
@@ -169,7 +171,7 @@ __device__ __forceinline__ void pull(double (&f)[L::Q], const double* __restrict
     if constexpr (MODE == 2) {
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            f[q] = ld<NT>(&fin[q * v.sq + self]);
+            f[q] = ld<NT != 0>(&fin[q * v.sq + self]);
         });
         return;
     } else {
@@ -178,7 +180,7 @@ __device__ __forceinline__ void pull(double (&f)[L::Q], const double* __restrict
             constexpr int q = decltype(qc)::value;
             const long long off = s.o[0][L::c(q, 0) + 1] + s.o[1][L::c(q, 1) + 1] +
                                   s.o[2][L::c(q, 2) + 1];
-            f[q] = ld<NT>(&fin[q * v.sq + off]);
+            f[q] = ld<NT == 1 || (NT == 2 && L::c(q, 0) == 0)>(&fin[q * v.sq + off]);
         });
         if constexpr (MODE == 0 && WALLS) {
             const int p[3] = {x, y, z};

@@ -38,7 +38,7 @@ int fail(int code, const std::string& msg)
 // grid = (ceil(nx/bx), ceil(ny/by), ceil(nz/NC)).  All NC*Q loads of a
 // thread are issued before the first collision so they are in flight
 // together.
-template <class L, bool FORCED, int MODE, bool WALLS, int NC, bool NT>
+template <class L, bool FORCED, int MODE, bool WALLS, int NC, int NT>
 __global__ __launch_bounds__(256) void k_step(const double* __restrict__ fin,
                                               double* __restrict__ fout, const Geo g,
                                               const View vi, const View vo, const double omega,
@@ -404,10 +404,10 @@ static bool valid_cfg(const mfa_launch& c)
 {
     const bool bx_ok = c.block_x == 64 || c.block_x == 128 || c.block_x == 256;
     return bx_ok && c.block_y >= 1 && c.block_x * c.block_y <= 256 && (c.cells_per_thread == 1 ||
-           c.cells_per_thread == 2) && (c.nt_loads == 0 || c.nt_loads == 1);
+           c.cells_per_thread == 2) && (c.nt_loads >= 0 && c.nt_loads <= 2);
 }
 
-template <class L, bool FORCED, bool WALLS, int NC, bool NT>
+template <class L, bool FORCED, bool WALLS, int NC, int NT>
 static void launch_fused(dim3 grid, dim3 block, hipStream_t st, const double* a, double* b,
                          const Geo& g, const View& vi, const View& vo, double omega, double F0,
                          double F1, double F2)
@@ -421,12 +421,21 @@ static void launch_fused_cfg(const mfa_launch& c, dim3 grid, dim3 block, hipStre
                              const double* a, double* b, const Geo& g, const View& vi,
                              const View& vo, double omega, double F0, double F1, double F2)
 {
+    auto go = [&](auto nc, auto nt) {
+        launch_fused<L, FORCED, WALLS, decltype(nc)::value, decltype(nt)::value>(grid, block, st, a, b, g, vi,
+                                                                                 vo, omega, F0, F1, F2);
+    };
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I0 = std::integral_constant<int, 0>;
     if (c.cells_per_thread == 2) {
-        if (c.nt_loads) launch_fused<L, FORCED, WALLS, 2, true>(grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
-        else launch_fused<L, FORCED, WALLS, 2, false>(grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        if (c.nt_loads == 1) go(I2{}, I1{});
+        else if (c.nt_loads == 2) go(I2{}, I2{});
+        else go(I2{}, I0{});
     } else {
-        if (c.nt_loads) launch_fused<L, FORCED, WALLS, 1, true>(grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
-        else launch_fused<L, FORCED, WALLS, 1, false>(grid, block, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        if (c.nt_loads == 1) go(I1{}, I1{});
+        else if (c.nt_loads == 2) go(I1{}, I2{});
+        else go(I1{}, I0{});
     }
 }
 
@@ -460,8 +469,8 @@ static int launch_step(const mfa_field* fin, mfa_field* fout, const Geo& g, doub
         dim3 grid, block;
         launch_shape(g.n, grid, block, 0);
         grid.z = g.n[2];
-        if (force) hipLaunchKernelGGL((k_step<L, true, 1, false, 1, false>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
-        else hipLaunchKernelGGL((k_step<L, false, 1, false, 1, false>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        if (force) hipLaunchKernelGGL((k_step<L, true, 1, false, 1, 0>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
+        else hipLaunchKernelGGL((k_step<L, false, 1, false, 1, 0>), grid, block, 0, st, a, b, g, vi, vo, omega, F0, F1, F2);
     }
     MFA_HIP_CHECK(hipGetLastError());
     return MFA_OK;

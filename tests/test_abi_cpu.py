@@ -94,7 +94,7 @@ def test_launch_configuration_tables():
     default = abi.get_launch(lat)
     abi.set_launch(lat, 128, 2, 1, 1)
     assert abi.get_launch(lat) == (128, 2, 1, 1)
-    for bad in ((96, 1, 1, 0), (256, 2, 1, 0), (64, 1, 3, 0), (64, 1, 1, 2)):
+    for bad in ((96, 1, 1, 0), (256, 2, 1, 0), (64, 1, 3, 0), (64, 1, 1, 3)):
         with pytest.raises(abi.MfaError):
             abi.set_launch(lat, *bad)
     abi.set_launch(lat)

@@ -170,3 +170,31 @@ def test_empty_and_mismatched_boxes_are_errors(dev):
     fz.n[0] = 0
     with pytest.raises(abi.MfaError):
         abi.step(fz, fz, g, 1.0)
+
+
+LAUNCHES = [(256, 1, 1, 0), (128, 2, 2, 1), (64, 4, 2, 2), (64, 1, 1, 2), (128, 1, 2, 0)]
+
+
+@pytest.mark.parametrize("lat", LATS)
+@pytest.mark.parametrize("bc", ["periodic", "lid", "inflow_pressure"])
+def test_every_launch_configuration_matches_oracle(dev, lat, bc):
+    """Block shapes, two cells per thread along z (odd n_z leaves a partial
+    column) and the three load policies all give the oracle's bits."""
+    import torch
+    L = O.Lattice(lat)
+    n = [70, 9, 7] if L.D == 3 else [70, 13]
+    g, ref, per = bc_tables(L, bc)
+    F = [2e-5, -1e-5, 3e-6][: L.D]
+    f0 = random_state(L, n, seed=zlib.crc32(f"launch/{lat}/{bc}".encode()))
+    _, expect = _oracle_step(L, f0, n, per, ref, 1.0 / 0.8, F)
+    lid = abi.LATTICE[lat]
+    tin, fin = valid_only(L, n, dev, f0)
+    try:
+        for cfg in LAUNCHES:                # (2-D runs one cell per thread whatever cfg says)
+            abi.set_launch(lid, *cfg)
+            tout, fout = valid_only(L, n, dev)
+            abi.step(fin, fout, g, 1.0 / 0.8, F)
+            torch.cuda.synchronize()
+            assert np.array_equal(to_host(tout), O.valid(L, expect)), cfg
+    finally:
+        abi.set_launch(lid)
