@@ -36,6 +36,9 @@ def parse():
     p.add_argument("--no-copy-probe", action="store_true",
                    help="skip the streaming-copy ceiling measurement (same bytes as one step)")
     p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--layout", default="row", choices=["row", "component"],
+                   help="population order in HBM: row-interleaved (the engine's own "
+                        "allocation) or the MultiFab's component-major planes")
     return p.parse_args()
 
 
@@ -101,9 +104,11 @@ def main():
     gen.manual_seed(args.seed)
     rho = torch.ones((1, n, n, n), dtype=torch.float64, device=dev)
     vel = (torch.rand((3, n, n, n), dtype=torch.float64, device=dev, generator=gen) * 2 - 1) * 1e-3
-    fa_t = torch.empty((19, n, n, n), dtype=torch.float64, device=dev)
+    shape = abi.row_shape(19, [n] * 3, 0, 3) if args.layout == "row" else (19, n, n, n)
+    fa_t = torch.empty(shape, dtype=torch.float64, device=dev)
     fb_t = torch.empty_like(fa_t)
-    fa, fb = abi.field(fa_t, [n] * 3, 0, 3), abi.field(fb_t, [n] * 3, 0, 3)
+    fa = abi.field(fa_t, [n] * 3, 0, 3, layout=args.layout)
+    fb = abi.field(fb_t, [n] * 3, 0, 3, layout=args.layout)
     stream = torch.cuda.current_stream()
     abi.init_equilibrium(abi.field(rho, [n] * 3, 0, 3), abi.field(vel, [n] * 3, 0, 3), fa,
                          lattice, F, stream)
@@ -177,7 +182,9 @@ def main():
         "config": {"workload": f"D3Q19 BGK periodic {n}^3, FP64 populations, tau=0.8, "
                                f"Guo force (1e-6,0,0), {args.steps} timed steps",
                    "lattice": "D3Q19", "cells": [n, n, n], "tau": tau, "force": F,
-                   "storage": "SoA [q][z][y][x], no halo (periodic wrap inline)"},
+                   "storage": ("SoA rows, [z][y][q][x] (the Q rows of one (y,z) side by side)"
+                               if args.layout == "row" else "SoA planes, [q][z][y][x] (MultiFab order)")
+                              + ", no halo (periodic wrap inline)"},
         "hbm_roofline_pct": round(100.0 * achieved / HBM_PEAK_GBS, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -42,6 +42,14 @@ int mfa_ensemble_set_grid(mfa_ensemble* e, int blocks);
  * solid cells get the rest equilibrium. */
 int mfa_ensemble_init(mfa_ensemble* e, void* f, const unsigned char* masks, const double* rho,
                       const double* vel, void* stream);
+/* Warm start: fa = feq(rho, u) as mfa_ensemble_init, then one step fa -> fb,
+ * then fa = (fa + fb) / 2 (fb is scratch).  A plain equilibrium start from
+ * fields that are not the lattice's own steady state carries a part that no
+ * run to a residual criterion removes: it converges to a field about 6e-4
+ * times the start's error away from the cold start's (measured on a D2Q9
+ * lid cavity); the average of two consecutive states does not (2.6e-11). */
+int mfa_ensemble_init_warm(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks,
+                           const double* rho, const double* vel, void* stream);
 int mfa_ensemble_step(mfa_ensemble* e, const void* fin, void* fout, const unsigned char* masks,
                       void* stream);
 /* nsteps steps alternating fa->fb->fa...; use_graph captures them in one

@@ -11,11 +11,16 @@
  *     mfa_last_error().
  *
  * Storage (the "MultiFab population storage" of one box): structure of
- * arrays, component-major.  Element (x,y,z,q) of an mfa_field lives at
+ * arrays.  Element (x,y,z,q) of an mfa_field lives at
  *     data + x + y*sy + z*sz + q*sq            (elements, not bytes)
  * with x,y,z measured from the valid lower corner, so ghost cells are
  * addressed with coordinates -1 and n_d.  ngrow says how many ghost layers
- * the allocation holds around the valid box (0 or 1).
+ * the allocation holds around the valid box (0 or 1).  Any strides whose
+ * valid cells do not overlap are accepted: the MultiFab's component-major
+ * order (sq = e0*e1*e2, one plane per population) and the row-interleaved
+ * order (sq = e0, sy = ncomp*e0: the Q rows of one (y,z) side by side, the
+ * engine's own allocation, about 5 % faster for the step on MI355X) run the
+ * same kernels and give identical results.
  */
 #ifndef MFA_LBM_H
 #define MFA_LBM_H

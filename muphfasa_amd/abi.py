@@ -141,26 +141,43 @@ def stream_ptr(stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def field(t, n, ngrow: int, lattice_dim: int, dtype: int = 0) -> Field:
-    """mfa_field over a contiguous tensor of shape (ncomp, e2, e1, e0) whose
-    extents are n_d + 2*ngrow in the lattice's dimensions (z has no ghost in
-    2-D) or larger (padding)."""
+def field(t, n, ngrow: int, lattice_dim: int, dtype: int = 0, layout: str = "component") -> Field:
+    """mfa_field over a contiguous tensor whose extents are n_d + 2*ngrow in
+    the lattice's dimensions (z has no ghost in 2-D) or larger (padding).
+
+    layout "component": shape (ncomp, e2, e1, e0), the MultiFab's order (one
+    plane per component).  layout "row": shape (e2, e1, ncomp, e0), the Q
+    rows of one (y, z) side by side (sq = e0, sy = ncomp*e0)."""
     assert t.is_contiguous()
-    ncomp, e2, e1, e0 = t.shape
+    if layout == "component":
+        ncomp, e2, e1, e0 = t.shape
+        sy, sz, sq = e0, e0 * e1, e0 * e1 * e2
+    elif layout == "row":
+        e2, e1, ncomp, e0 = t.shape
+        sy, sz, sq = ncomp * e0, ncomp * e0 * e1, e0
+    else:
+        raise ValueError(f"unknown layout {layout!r}")
     n = list(n) + [1] * (3 - len(n))
     gz = ngrow if lattice_dim == 3 else 0
     assert e0 >= n[0] + 2 * ngrow and e1 >= n[1] + 2 * ngrow and e2 >= n[2] + 2 * gz
     f = Field()
     esz = t.element_size()
-    off = ngrow + ngrow * e0 + gz * e0 * e1
+    off = ngrow + ngrow * sy + gz * sz
     f.data = t.data_ptr() + off * esz
     for d in range(3):
         f.n[d] = n[d]
     f.ngrow = ngrow
-    f.sy, f.sz, f.sq = e0, e0 * e1, e0 * e1 * e2
+    f.sy, f.sz, f.sq = sy, sz, sq
     f.ncomp = ncomp
     f.dtype = dtype
     return f
+
+
+def row_shape(Q: int, n, ngrow: int, lattice_dim: int):
+    """Tensor shape of a row-interleaved population field (for field(..., layout="row"))."""
+    n = list(n) + [1] * (3 - len(n))
+    gz = ngrow if lattice_dim == 3 else 0
+    return (n[2] + 2 * gz, n[1] + 2 * ngrow, Q, n[0] + 2 * ngrow)
 
 
 def step(fin: Field, fout: Field, g: GeomBC, omega: float, force=None, stream=None):

@@ -21,16 +21,18 @@ CANDIDATES_2D = [(bx, by, 1, nt) for bx, by in ((256, 1), (128, 2), (64, 4), (64
 
 
 def tune_step(lattice: str, n, forced: bool = True, periodic: bool = True, steps: int = 20,
-              candidates=None, apply: bool = True, dev="cuda:0"):
+              candidates=None, apply: bool = True, dev="cuda:0", layout: str = "row"):
+    """layout: population order of the timed fields ("row" = the engine's
+    row-interleaved allocation, "component" = the MultiFab's planes)."""
     import torch
     lid = abi.LATTICE[lattice]
     D, Q = abi.LATTICE_DQ[lid]
     n = list(n) + [1] * (3 - len(n))
-    a = torch.empty((Q, n[2], n[1], n[0]), dtype=torch.float64, device=dev)
+    shape = abi.row_shape(Q, n, 0, D) if layout == "row" else (Q, n[2], n[1], n[0])
+    a = torch.empty(shape, dtype=torch.float64, device=dev)
     b = torch.empty_like(a)
-    w = {9: 4 / 9, 15: 2 / 9, 19: 1 / 3, 27: 8 / 27}[Q]
-    a.fill_(w * 0 + 1.0 / Q)
-    fa, fb = abi.field(a, n, 0, D), abi.field(b, n, 0, D)
+    a.fill_(1.0 / Q)
+    fa, fb = abi.field(a, n, 0, D, layout=layout), abi.field(b, n, 0, D, layout=layout)
     names = ["Periodic"] * (2 * D) if periodic else ["NoSlip"] * (2 * D)
     g = abi.resolve_bc(lid, [abi.BC[x] for x in names])
     F = [1e-6, 0.0, 0.0][:D] if forced else None
@@ -54,7 +56,7 @@ def tune_step(lattice: str, n, forced: bool = True, periodic: bool = True, steps
         cells = n[0] * n[1] * n[2]
         out.append({"cfg": list(cfg), "ms": ms, "mlups": cells / ms / 1e3,
                     "GBs": cells * 2 * Q * 8 / ms / 1e6})
-        print(json.dumps({"lattice": lattice, "n": n, **out[-1]}), flush=True)
+        print(json.dumps({"lattice": lattice, "n": n, "layout": layout, **out[-1]}), flush=True)
     out.sort(key=lambda r: r["ms"])
     if apply:
         abi.set_launch(lid, *out[0]["cfg"])
@@ -132,14 +134,19 @@ def copy_ceiling(nbytes: int = 2 * 1024 ** 3, reps: int = 10, dev="cuda:0"):
 
 
 if __name__ == "__main__":
-    # python -m muphfasa_amd.autotune LATTICE [DTYPE] NX NY [NZ]
+    # python -m muphfasa_amd.autotune LATTICE [DTYPE] NX NY [NZ] [--layout component]
     args = sys.argv[1:]
+    layout = "row"
+    if "--layout" in args:
+        i = args.index("--layout")
+        layout = args[i + 1]
+        del args[i:i + 2]
     lat = args.pop(0) if args else "D3Q19"
     dtype = args.pop(0) if args and not args[0].isdigit() else "f64"
     size = [int(v) for v in args] or [256, 256, 256]
     print(json.dumps({"copy_ceiling_GBs": copy_ceiling()}), flush=True)
     if dtype == "f64":
-        best = tune_step(lat, size)
+        best = tune_step(lat, size, layout=layout)
     else:
         best = tune_step_dev(lat, dtype, size)
     print(json.dumps({"best": best[0]}), flush=True)

@@ -202,6 +202,29 @@ __global__ __launch_bounds__(256) void k_batch_init(T* __restrict__ fout, const 
     }
 }
 
+// fa = (fa + fb) / 2 over the cells of every active tile: the average of two
+// consecutive states of a warm start (see mfa_ensemble_init_warm).
+template <class L, class T>
+__global__ __launch_bounds__(256) void k_batch_average(T* __restrict__ fa, const T* __restrict__ fb,
+                                                       const Tile* __restrict__ tiles, int ntiles,
+                                                       const LatInfo* __restrict__ lats)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        const LatInfo li = lats[tl.lat];
+        const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
+        if (x >= li.nx || y >= li.ny) continue;
+        const long long sq = (long long)li.nx * li.ny;
+        const long long self = li.pop_off + x + (long long)y * li.nx;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const long long i = self + q * sq;
+            if constexpr (std::is_same<T, double>::value) fa[i] = 0.5 * (fa[i] + fb[i]);
+            else fa[i] = from_f<T>(0.5f * (to_f(fa[i]) + to_f(fb[i])));
+        });
+    }
+}
+
 // Per-tile partial sums of |u - u_prev|^2 and |u|^2 over fluid cells.
 template <bool MASKED>
 __global__ __launch_bounds__(256) void k_batch_residual(const double* __restrict__ vel,
@@ -423,6 +446,24 @@ int mfa_ensemble_init(mfa_ensemble* e, void* f, const unsigned char* masks, cons
             hipLaunchKernelGGL((k_batch_init<L, T, true>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (T*)f, masks, e->d_tiles, e->ntiles, e->d_lats, rho, vel, e->total_cells);
         else
             hipLaunchKernelGGL((k_batch_init<L, T, false>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (T*)f, masks, e->d_tiles, e->ntiles, e->d_lats, rho, vel, e->total_cells);
+        ENS_HIP(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_ensemble_init_warm(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks,
+                           const double* rho, const double* vel, void* stream)
+{
+    if (!e || !fa || !fb || fa == fb) return ens_fail(MFA_ERR_INVALID, "bad populations");
+    int rc = mfa_ensemble_init(e, fa, masks, rho, vel, stream);
+    if (rc == MFA_OK) rc = mfa_ensemble_step(e, fa, fb, masks, stream);
+    if (rc != MFA_OK || !e->ntiles) return rc;
+    return with_type(e->dtype, [&](auto tv) {
+        using T = decltype(tv);
+        using L = Lattice<MFA_D2Q9>;
+        const dim3 block(e->tx, e->ty);
+        hipLaunchKernelGGL((k_batch_average<L, T>), dim3(grid_for(e)), block, 0, (hipStream_t)stream,
+                           (T*)fa, (const T*)fb, e->d_tiles, e->ntiles, e->d_lats);
         ENS_HIP(hipGetLastError());
         return (int)MFA_OK;
     });

@@ -41,12 +41,13 @@ def lib():
         L.mfa_ensemble_set_active.argtypes = [vp, ip]
         L.mfa_ensemble_set_grid.argtypes = [vp, ctypes.c_int]
         L.mfa_ensemble_init.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.mfa_ensemble_init_warm.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.mfa_ensemble_step.argtypes = [vp, vp, vp, vp, vp]
         L.mfa_ensemble_run.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
         L.mfa_ensemble_moments.argtypes = [vp, vp, vp, vp, vp, vp]
         L.mfa_ensemble_residual.argtypes = [vp, vp, vp, vp, dp, vp]
         for n in ("mfa_ensemble_info", "mfa_ensemble_set_active", "mfa_ensemble_set_grid",
-                  "mfa_ensemble_init", "mfa_ensemble_step", "mfa_ensemble_run",
+                  "mfa_ensemble_init", "mfa_ensemble_init_warm", "mfa_ensemble_step", "mfa_ensemble_run",
                   "mfa_ensemble_moments", "mfa_ensemble_residual"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
@@ -128,9 +129,16 @@ class Ensemble:
     def stream(self):
         return abi.stream_ptr()
 
-    def init(self, rho=None, vel=None):
-        _check(lib().mfa_ensemble_init(self.h, _p(self.fa), _p(self.mask), _p(rho), _p(vel),
-                                       self.stream()))
+    def init(self, rho=None, vel=None, average=False):
+        """f = feq(rho, u) (rest state when both are None).  average=True is
+        the warm start of mfa_ensemble_init_warm: one step, then the mean of
+        the two consecutive states."""
+        if average:
+            _check(lib().mfa_ensemble_init_warm(self.h, _p(self.fa), _p(self.fb), _p(self.mask),
+                                                _p(rho), _p(vel), self.stream()))
+        else:
+            _check(lib().mfa_ensemble_init(self.h, _p(self.fa), _p(self.mask), _p(rho), _p(vel),
+                                           self.stream()))
 
     def set_grid(self, blocks):
         _check(lib().mfa_ensemble_set_grid(self.h, blocks))

@@ -172,8 +172,10 @@ SimulationImpl::SimulationImpl(const InputParser& params) : m_params(params)
         // periodicity comes from bc.lo_type only (src/Geometry/GeometryHandler.cpp:40-47)
         m_geom.periodic[d] = d < D ? (names[2 * d] == "Periodic") : 1;
     }
-    m_fa.define(m_geom.n, m_Q, 0, D);
-    m_fb.define(m_geom.n, m_Q, 0, D);
+    // populations in the row-interleaved order (the step streams it faster);
+    // moments stay in the MultiFab's planes, which plotfiles write as is
+    m_fa.define(m_geom.n, m_Q, 0, D, MFA_F64, Layout::Rows);
+    m_fb.define(m_geom.n, m_Q, 0, D, MFA_F64, Layout::Rows);
     m_rho.define(m_geom.n, 1, 0, D);
     m_vel.define(m_geom.n, D, 0, D);
     hip_check(hipStreamCreate(&m_stream), "hipStreamCreate");

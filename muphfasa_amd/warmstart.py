@@ -174,25 +174,36 @@ def fields_on_device(pred, masks_dev, u_lid=0.1):
     return rho.reshape(-1).contiguous(), torch.cat([ux.reshape(-1), uy.reshape(-1)]).contiguous()
 
 
+MODES = ("warm", "warm_plain", "cold")
+
+
 def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, dtype="f64", N=256,
-             tile=(64, 4), max_seconds=None):
+             tile=(64, 4), max_seconds=None, modes=MODES):
+    """Runs the held-out cases to the steady criterion from each start:
+      cold       rest state (rho = 1, u = 0), the plain method
+      warm_plain feq of the network's prediction (mass restored)
+      warm       the same followed by one step and the mean of the two
+                 consecutive states (mfa_ensemble_init_warm)
+    and compares steps, wall time and converged fields with the cold start."""
     import torch
     masks, Re, tau = S.make_cases(n_eval, seed, N)
     masks_dev = torch.from_numpy(masks).to(net.dev)
     re_dev = torch.from_numpy(S.re_feature(Re)).to(net.dev)
     res = {}
-    for mode in ("warm", "cold"):
+    vel_pred = None
+    order = [m for m in modes if m != "cold"] + (["cold"] if "cold" in modes else [])
+    for mode in order:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ens = E.Ensemble([(N, N)] * n_eval, 1.0 / tau, dtype=dtype, tile=tile, masks=list(masks))
         t_inf = 0.0
-        if mode == "warm":
+        if mode != "cold":
             ti = time.perf_counter()
             pred = predict(net, masks_dev, re_dev)
             rho, vel = fields_on_device(pred, masks_dev)
             torch.cuda.synchronize()
             t_inf = time.perf_counter() - ti
-            ens.init(rho, vel)
+            ens.init(rho, vel, average=(mode == "warm"))
             vel_pred = vel
             del pred, rho
         else:
@@ -207,43 +218,55 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
         wall = time.perf_counter() - t0
         ens.set_active(np.ones(n_eval, np.int32))
         ens.moments()
-        vel = ens.vel.clone()
         res[mode] = {"steps": r["steps"], "status": r["status"], "wall_s": wall, "gpu_s": r["gpu_s"],
-                     "inference_s": t_inf, "vel": vel}
+                     "inference_s": t_inf, "vel": ens.vel.clone(),
+                     "lattice_updates": r["lattice_updates"]}
+        _log(stage="eval_mode_done", mode=mode, wall_s=round(wall, 2),
+             converged=int((r["status"] == "converged").sum()),
+             steps_total=int(r["steps"].sum()))
         del ens
-    cw, cc = res["warm"], res["cold"]
-    both = (cw["status"] == "converged") & (cc["status"] == "converged")
     tc = n_eval * N * N
-    dv = (cw["vel"] - cc["vel"]).abs()
-    dmax = torch.maximum(dv[:tc], dv[tc:]).view(n_eval, -1).max(dim=1).values.cpu().numpy() / 0.1
-    ratio = cc["steps"][both] / np.maximum(cw["steps"][both], 1)
-    # surrogate error against the converged cold field, per case (max over
-    # fluid cells of |u_pred - u| / u_lid, and relative L2)
-    dp = (vel_pred - cc["vel"]).view(2, n_eval, -1)
-    uc = cc["vel"].view(2, n_eval, -1)
-    pred_max = dp.abs().amax(dim=(0, 2)).cpu().numpy() / 0.1
-    pred_l2 = torch.sqrt((dp * dp).sum(dim=(0, 2)) / (uc * uc).sum(dim=(0, 2)).clamp_min(1e-300))
-    pred_l2 = pred_l2.cpu().numpy()
-    per_case = {"Re": Re, "steps_warm": cw["steps"], "steps_cold": cc["steps"],
-                "status_warm": cw["status"].astype(str), "status_cold": cc["status"].astype(str),
-                "pred_err_max": pred_max, "pred_err_l2": pred_l2, "field_diff_max": dmax,
-                "solid_cells": masks.reshape(n_eval, -1).sum(axis=1)}
-    summary = {
-        "n_eval": n_eval, "tol": tol, "max_steps": max_steps, "dtype": dtype,
-        "cold_converged": int((cc["status"] == "converged").sum()),
-        "warm_converged": int((cw["status"] == "converged").sum()),
-        "both_converged": int(both.sum()),
-        "cold_steps_total": int(cc["steps"].sum()), "warm_steps_total": int(cw["steps"].sum()),
-        "steps_ratio_median": float(np.median(ratio)) if both.any() else None,
-        "steps_ratio_mean": float(np.mean(ratio)) if both.any() else None,
-        "cold_wall_s": round(cc["wall_s"], 2), "warm_wall_s": round(cw["wall_s"], 2),
-        "warm_inference_s": round(cw["inference_s"], 3),
-        "wall_ratio": round(cc["wall_s"] / cw["wall_s"], 3),
-        "field_diff_max_over_both": float(dmax[both].max()) if both.any() else None,
-        "field_diff_median": float(np.median(dmax[both])) if both.any() else None,
-        "pred_err_l2_median": float(np.median(pred_l2[both])) if both.any() else None,
-        "pred_err_max_median": float(np.median(pred_max[both])) if both.any() else None,
-    }
+    cc = res.get("cold")
+    summary = {"n_eval": n_eval, "tol": tol, "max_steps": max_steps, "dtype": dtype,
+               "check": check}
+    per_case = {"Re": Re, "solid_cells": masks.reshape(n_eval, -1).sum(axis=1)}
+    for mode, rm in res.items():
+        summary[f"{mode}_converged"] = int((rm["status"] == "converged").sum())
+        summary[f"{mode}_steps_total"] = int(rm["steps"].sum())
+        summary[f"{mode}_wall_s"] = round(rm["wall_s"], 2)
+        summary[f"{mode}_lattice_updates"] = int(rm["lattice_updates"])
+        per_case[f"steps_{mode}"] = rm["steps"]
+        per_case[f"status_{mode}"] = rm["status"].astype(str)
+    if cc is not None:
+        uc = cc["vel"].view(2, n_eval, -1)
+        if vel_pred is not None:
+            # surrogate error against the converged cold field, per case
+            dp = (vel_pred - cc["vel"]).view(2, n_eval, -1)
+            per_case["pred_err_max"] = dp.abs().amax(dim=(0, 2)).cpu().numpy() / 0.1
+            per_case["pred_err_l2"] = torch.sqrt(
+                (dp * dp).sum(dim=(0, 2)) / (uc * uc).sum(dim=(0, 2)).clamp_min(1e-300)).cpu().numpy()
+        for mode, rm in res.items():
+            if mode == "cold":
+                continue
+            both = (rm["status"] == "converged") & (cc["status"] == "converged")
+            dv = (rm["vel"] - cc["vel"]).abs()
+            dmax = torch.maximum(dv[:tc], dv[tc:]).view(n_eval, -1).max(dim=1).values.cpu().numpy() / 0.1
+            per_case[f"field_diff_max_{mode}"] = dmax
+            ratio = cc["steps"][both] / np.maximum(rm["steps"][both], 1)
+            summary[f"{mode}_both_converged"] = int(both.sum())
+            summary[f"{mode}_steps_ratio_median"] = float(np.median(ratio)) if both.any() else None
+            summary[f"{mode}_steps_ratio_mean"] = float(np.mean(ratio)) if both.any() else None
+            summary[f"{mode}_steps_ratio_of_sums"] = (
+                float(cc["steps"][both].sum() / max(rm["steps"][both].sum(), 1)) if both.any() else None)
+            summary[f"{mode}_wall_ratio"] = round(cc["wall_s"] / rm["wall_s"], 3)
+            summary[f"{mode}_lu_ratio"] = round(cc["lattice_updates"] / max(rm["lattice_updates"], 1), 3)
+            summary[f"{mode}_inference_s"] = round(rm["inference_s"], 3)
+            summary[f"{mode}_field_diff_max"] = float(dmax[both].max()) if both.any() else None
+            summary[f"{mode}_field_diff_median"] = float(np.median(dmax[both])) if both.any() else None
+            summary[f"{mode}_field_diff_p90"] = float(np.percentile(dmax[both], 90)) if both.any() else None
+        if "pred_err_l2" in per_case:
+            summary["pred_err_l2_median"] = float(np.median(per_case["pred_err_l2"]))
+            summary["pred_err_max_median"] = float(np.median(per_case["pred_err_max"]))
     return summary, per_case
 
 
@@ -265,6 +288,8 @@ def main():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--eval-out", default="", help="per-case results (.npz)")
     p.add_argument("--eval-seconds", type=float, default=None, help="wall budget per start mode")
+    p.add_argument("--modes", default=",".join(MODES),
+                   help="starts to evaluate (comma list of warm, warm_plain, cold)")
     a = p.parse_args()
     import torch
     net = S.WarmStartNet(seed=a.seed)
@@ -300,7 +325,8 @@ def main():
         if a.stage == "eval":
             net.load(a.weights)
         summary, per_case = evaluate(net, a.n_eval, seed=a.seed + 1, tol=a.tol,
-                                     max_steps=a.eval_max_steps, max_seconds=a.eval_seconds)
+                                     max_steps=a.eval_max_steps, max_seconds=a.eval_seconds,
+                                     modes=tuple(a.modes.split(",")))
         _log(stage="eval_done", **summary)
         if a.eval_out:
             np.savez(a.eval_out, **per_case)

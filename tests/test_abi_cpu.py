@@ -109,3 +109,43 @@ def test_launch_configuration_tables():
         assert abi.get_launch_dev(lat, abi.DTYPE[dt]) == d
     with pytest.raises(abi.MfaError):
         abi.get_launch_dev(lat, abi.DTYPE["f64"])
+
+
+def test_overlapping_strides_are_rejected_before_launch():
+    """The stride check admits any non-overlapping order (component-major or
+    row-interleaved) and rejects views whose valid cells would alias; the
+    rejection happens before any device work."""
+    g = abi.resolve_bc(2, [abi.BC["Periodic"]] * 6)
+    n = 8
+    for sy, sz, sq in ((n, n * n, n * n * n - 1),           # component planes overlap
+                       (19 * n - 1, 19 * n * n, n),         # rows overlap (row-interleaved)
+                       (n, n * n, n // 2),                  # population rows overlap
+                       (n - 1, n * n, n * n * n)):          # y rows overlap
+        f = abi.Field()
+        f.data = 256
+        f.n[0] = f.n[1] = f.n[2] = n
+        f.sy, f.sz, f.sq, f.ncomp, f.dtype = sy, sz, sq, 19, 0
+        with pytest.raises(abi.MfaError) as e:
+            abi.check(abi.lib().mfa_step(ctypes.byref(f), ctypes.byref(f), ctypes.byref(g), 1.0,
+                                         None, None))
+        assert e.value.code == -1 and "strides" in str(e.value), (sy, sz, sq)
+
+
+def test_every_header_declaration_is_exported():
+    """Every function declared in include/*.h is exported by the built
+    library that implements it (mfa_host.h: the C++ host; the others: the HIP
+    engine).  Only loads and looks up symbols; nothing runs on a device."""
+    decl = re.compile(r"^\s*(?:const\s+)?[A-Za-z_]\w*\s*\*?\s*\*?\s*((?:mfa|muphfasa)_\w+)\s*\(",
+                      re.M)
+    libs = {"mfa_host.h": os.path.join(ROOT, "muphfasa_amd", "lib", "libmuphfasa_host.so")}
+    seen = 0
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        names = set(decl.findall(open(os.path.join(ROOT, "include", h)).read()))
+        assert names, h
+        L = abi.load_library(libs[h]) if h in libs else abi.lib()
+        missing = [n for n in sorted(names) if not hasattr(L, n)]
+        assert not missing, (h, missing)
+        seen += len(names)
+    assert seen >= 50

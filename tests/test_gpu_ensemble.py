@@ -126,3 +126,40 @@ def test_fp32_member_close_to_fp64(dev):
         out[dt] = ens.lattice_fields(0)[1]
     err = np.abs(out["f32"] - out["f64"]).max() / 0.1
     assert err < 1e-5, err
+
+
+def test_warm_init_average_matches_oracle(dev):
+    """mfa_ensemble_init_warm: feq of the given fields (rest state in solid
+    cells), one masked step, then the mean of the two states, against the
+    oracle's equilibrium initialisation and obstacle step."""
+    import torch
+    L = O.Lattice("D2Q9")
+    sizes = [(30, 26), (20, 22)]
+    masks = [np.zeros((26, 30), np.uint8), np.zeros((22, 20), np.uint8)]
+    masks[0][8:14, 10:19] = 1
+    masks[1][3:6, 12:16] = 1
+    omegas = [1.0 / 0.6, 1.0 / 0.53]
+    rng = np.random.default_rng(8)
+    rhos, vels = [], []
+    for (nx, ny), m in zip(sizes, masks):
+        r = 1.0 + 0.01 * rng.standard_normal((ny, nx))
+        v = 0.05 * rng.standard_normal((2, ny, nx))
+        r[m == 1] = 1.0
+        v[:, m == 1] = 0.0
+        rhos.append(r)
+        vels.append(v)
+    ens = E.Ensemble(sizes, omegas, masks=masks)
+    rho_d = torch.from_numpy(np.concatenate([r.ravel() for r in rhos])).to(dev)
+    vel_d = torch.from_numpy(np.concatenate([np.concatenate([v[0].ravel() for v in vels]),
+                                             np.concatenate([v[1].ravel() for v in vels])])).to(dev)
+    ens.init(rho_d, vel_d, average=True)
+    torch.cuda.synchronize()
+    fu = np.zeros((6, 3))
+    fu[3, 0] = 0.1
+    res = O.resolve_bc(2, [1, 1, 1, 0], np.ones(6), fu)
+    for l, (nx, ny) in enumerate(sizes):
+        f0, _, _ = O.init_equilibrium(L, rhos[l][None], vels[l][:, None], [nx, ny])
+        O.apply_bc(L, f0, [nx, ny], [0, 0, 0], res)
+        f1 = O.run_masked(L, f0, [nx, ny], omegas[l], [0, 0], res, masks[l][None], 1)
+        expect = 0.5 * (O.valid(L, f0) + O.valid(L, f1))
+        assert np.array_equal(ens.lattice_populations(l), expect), l
