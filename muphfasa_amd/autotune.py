@@ -70,7 +70,7 @@ CANDIDATES_DEV = [(bx, by, nc, nt) for bx, by in ((256, 1), (128, 2), (64, 4), (
 
 
 def tune_step_dev(lattice: str, dtype: str, n, forced: bool = False, steps: int = 20,
-                  candidates=None, apply: bool = True, dev="cuda:0"):
+                  candidates=None, apply: bool = True, dev="cuda:0", layout: str = "row"):
     """The same search for reduced storage (FP32 / FP16 / BF16 deviations):
     block shape, one cell or two x-adjacent cells per thread, non-temporal
     loads of the pairs."""
@@ -79,9 +79,11 @@ def tune_step_dev(lattice: str, dtype: str, n, forced: bool = False, steps: int 
     D, Q = abi.LATTICE_DQ[lid]
     n = list(n) + [1] * (3 - len(n))
     tdt = {"f32": torch.float32, "f16dev": torch.float16, "bf16dev": torch.bfloat16}[dtype]
-    a = torch.zeros((Q, n[2], n[1], n[0]), dtype=tdt, device=dev)
+    shape = abi.row_shape(Q, n, 0, D) if layout == "row" else (Q, n[2], n[1], n[0])
+    a = torch.zeros(shape, dtype=tdt, device=dev)
     b = torch.zeros_like(a)
-    fa, fb = abi.field(a, n, 0, D, did), abi.field(b, n, 0, D, did)
+    fa = abi.field(a, n, 0, D, did, layout=layout)
+    fb = abi.field(b, n, 0, D, did, layout=layout)
     g = abi.resolve_bc(lid, [abi.BC["Periodic"]] * (2 * D))
     F = [1e-6, 0.0, 0.0][:D] if forced else None
     out = []
@@ -103,7 +105,8 @@ def tune_step_dev(lattice: str, dtype: str, n, forced: bool = False, steps: int 
         cells = n[0] * n[1] * n[2]
         out.append({"cfg": list(cfg), "ms": ms, "mlups": cells / ms / 1e3,
                     "GBs": cells * 2 * Q * a.element_size() / ms / 1e6})
-        print(json.dumps({"lattice": lattice, "dtype": dtype, "n": n, **out[-1]}), flush=True)
+        print(json.dumps({"lattice": lattice, "dtype": dtype, "n": n, "layout": layout, **out[-1]}),
+              flush=True)
     out.sort(key=lambda r: r["ms"])
     if apply:
         abi.set_launch_dev(lid, did, *out[0]["cfg"])
@@ -148,5 +151,5 @@ if __name__ == "__main__":
     if dtype == "f64":
         best = tune_step(lat, size, layout=layout)
     else:
-        best = tune_step_dev(lat, dtype, size)
+        best = tune_step_dev(lat, dtype, size, layout=layout)
     print(json.dumps({"best": best[0]}), flush=True)

@@ -32,8 +32,11 @@ struct LatInfo {
     long long cell_off;  // cell offset (masks, moments)
 };
 
+// near: the tile or its one-cell rim holds a solid cell of the mask (set by
+// mfa_ensemble_init from the masks it is given; 1 until then).  Tiles with
+// near == 0 skip every mask read: their cells' pulls cannot touch a solid.
 struct Tile {
-    int lat, x0, y0, pad;
+    int lat, x0, y0, near;
 };
 
 struct BcTable {
@@ -89,9 +92,10 @@ __global__ __launch_bounds__(256) void k_batch_step(const T* __restrict__ fin, T
         T* b = fout + li.pop_off;
         const long long self = x + (long long)y * li.nx;
         const unsigned char* mask = MASKED ? masks + li.cell_off : nullptr;
+        const bool near = MASKED && tl.near;
         const GeoRef g = geo_for<L>(li, bc);
         if constexpr (std::is_same<T, double>::value) {
-            if (MASKED && mask[self]) {
+            if (near && mask[self]) {
                 for_q<L>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
                     b[q * v.sq + self] = L::w(q);
@@ -100,11 +104,11 @@ __global__ __launch_bounds__(256) void k_batch_step(const T* __restrict__ fin, T
             }
             double f[L::Q];
             pull<L, 0, true, false>(f, a, x, y, 0, g, v);
-            if constexpr (MASKED)
+            if (near)
                 obstacle_bb<L>(f, mask, x, y, li.nx, li.ny, [&](int m) { return a[m * v.sq + self]; });
             collide_store<L, false>(f, b, self, v, li.omega, 0.0, 0.0, 0.0);
         } else {
-            if (MASKED && mask[self]) {
+            if (near && mask[self]) {
                 for_q<L>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
                     b[q * v.sq + self] = from_f<T>(0.0f);
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(256) void k_batch_step(const T* __restrict__ fin, T
             }
             float d[L::Q];
             pull_dev<L, true>(d, a, x, y, 0, g, v);
-            if constexpr (MASKED)
+            if (near)
                 obstacle_bb<L>(d, mask, x, y, li.nx, li.ny, [&](int m) { return to_f(a[m * v.sq + self]); });
             collide_store_dev<L, false>(d, b, self, v, (float)li.omega, 0.f, 0.f, 0.f);
         }
@@ -140,7 +144,8 @@ __global__ __launch_bounds__(256) void k_batch_moments(const T* __restrict__ fin
         const long long self = x + (long long)y * li.nx;
         const long long cell = li.cell_off + self;
         const unsigned char* mask = MASKED ? masks + li.cell_off : nullptr;
-        if (MASKED && mask[self]) {
+        const bool near = MASKED && tl.near;
+        if (near && mask[self]) {
             rho[cell] = 1.0;
             vel[cell] = 0.0;
             vel[total_cells + cell] = 0.0;
@@ -150,12 +155,12 @@ __global__ __launch_bounds__(256) void k_batch_moments(const T* __restrict__ fin
         double f[L::Q];
         if constexpr (std::is_same<T, double>::value) {
             pull<L, 0, true, false>(f, a, x, y, 0, g, v);
-            if constexpr (MASKED)
+            if (near)
                 obstacle_bb<L>(f, mask, x, y, li.nx, li.ny, [&](int m) { return a[m * v.sq + self]; });
         } else {
             float d[L::Q];
             pull_dev<L, true>(d, a, x, y, 0, g, v);
-            if constexpr (MASKED)
+            if (near)
                 obstacle_bb<L>(d, mask, x, y, li.nx, li.ny, [&](int m) { return to_f(a[m * v.sq + self]); });
             for_q<L>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
@@ -202,6 +207,29 @@ __global__ __launch_bounds__(256) void k_batch_init(T* __restrict__ fout, const 
     }
 }
 
+// near flag of each tile: does the tile or its one-cell rim hold a solid cell?
+__global__ __launch_bounds__(256) void k_tile_near(const unsigned char* __restrict__ masks,
+                                                   const Tile* __restrict__ tiles, int ntiles,
+                                                   const LatInfo* __restrict__ lats, int tx, int ty,
+                                                   int* __restrict__ near)
+{
+    const int hx = tx + 2, hy = ty + 2;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        const LatInfo li = lats[tl.lat];
+        int any = 0;
+        for (int i = threadIdx.x; i < hx * hy; i += blockDim.x) {
+            const int x = tl.x0 - 1 + i % hx, y = tl.y0 - 1 + i / hx;
+            if (x >= 0 && x < li.nx && y >= 0 && y < li.ny &&
+                masks[li.cell_off + x + (long long)y * li.nx])
+                any = 1;
+        }
+        any = __syncthreads_or(any);
+        if (threadIdx.x == 0) near[t] = any;
+        __syncthreads();
+    }
+}
+
 // fa = (fa + fb) / 2 over the cells of every active tile: the average of two
 // consecutive states of a warm start (see mfa_ensemble_init_warm).
 template <class L, class T>
@@ -243,7 +271,7 @@ __global__ __launch_bounds__(256) void k_batch_residual(const double* __restrict
         double a = 0.0, b = 0.0;
         if (x < li.nx && y < li.ny) {
             const long long cell = li.cell_off + x + (long long)y * li.nx;
-            if (!(MASKED && masks[cell])) {
+            if (!(MASKED && tl.near && masks[cell])) {
                 const double u0 = vel[cell], u1 = vel[total_cells + cell];
                 const double d0 = u0 - vel_prev[cell], d1 = u1 - vel_prev[total_cells + cell];
                 a = d0 * d0 + d1 * d1;
@@ -290,6 +318,9 @@ struct mfa_ensemble {
     BcTable* d_bc = nullptr;
     double* d_part = nullptr;
     int ntiles = 0, tiles_cap = 0;
+    // near flag of every tile of every lattice (active or not), in the order
+    // rebuild_tiles enumerates them; empty until masks have been seen
+    std::vector<int> near;
 };
 
 static thread_local std::string g_ens_err;
@@ -309,11 +340,13 @@ static int ens_fail(int code, const std::string& m)
 static int rebuild_tiles(mfa_ensemble* e)
 {
     e->tiles.clear();
-    for (int l = 0; l < e->nlat; ++l) {
-        if (!e->active[l]) continue;
+    size_t k = 0;
+    const int near_default = e->masked ? 1 : 0;
+    for (int l = 0; l < e->nlat; ++l)
         for (int y0 = 0; y0 < e->lats[l].ny; y0 += e->ty)
-            for (int x0 = 0; x0 < e->lats[l].nx; x0 += e->tx) e->tiles.push_back(Tile{l, x0, y0, 0});
-    }
+            for (int x0 = 0; x0 < e->lats[l].nx; x0 += e->tx, ++k)
+                if (e->active[l])
+                    e->tiles.push_back(Tile{l, x0, y0, k < e->near.size() ? e->near[k] : near_default});
     e->ntiles = (int)e->tiles.size();
     if (e->ntiles > e->tiles_cap) {
         if (e->d_tiles) (void)hipFree(e->d_tiles);
@@ -325,6 +358,38 @@ static int rebuild_tiles(mfa_ensemble* e)
     if (e->ntiles)
         ENS_HIP(hipMemcpy(e->d_tiles, e->tiles.data(), sizeof(Tile) * e->ntiles, hipMemcpyHostToDevice));
     return MFA_OK;
+}
+
+// Near flags of every tile from the masks (synchronises the stream), then the
+// active tile list again with the flags in it.
+static int compute_near(mfa_ensemble* e, const unsigned char* masks, hipStream_t st)
+{
+    std::vector<Tile> all;
+    for (int l = 0; l < e->nlat; ++l)
+        for (int y0 = 0; y0 < e->lats[l].ny; y0 += e->ty)
+            for (int x0 = 0; x0 < e->lats[l].nx; x0 += e->tx) all.push_back(Tile{l, x0, y0, 1});
+    Tile* d_all = nullptr;
+    int* d_near = nullptr;
+    const int n = (int)all.size();
+    ENS_HIP(hipMalloc((void**)&d_all, sizeof(Tile) * n));
+    if (hipMalloc((void**)&d_near, sizeof(int) * n) != hipSuccess) {
+        (void)hipFree(d_all);
+        return ens_fail(MFA_ERR_HIP, "ensemble: device allocation failed");
+    }
+    std::vector<int> near(n, 1);
+    hipError_t err = hipMemcpyAsync(d_all, all.data(), sizeof(Tile) * n, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess) {
+        hipLaunchKernelGGL(k_tile_near, dim3(n < 4096 ? n : 4096), dim3(256), 0, st, masks, d_all, n,
+                           e->d_lats, e->tx, e->ty, d_near);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipMemcpyAsync(near.data(), d_near, sizeof(int) * n, hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    (void)hipFree(d_all);
+    (void)hipFree(d_near);
+    if (err != hipSuccess) return ens_fail(MFA_ERR_HIP, std::string("ensemble: near flags: ") + hipGetErrorString(err));
+    e->near.swap(near);
+    return rebuild_tiles(e);
 }
 
 static int grid_for(const mfa_ensemble* e)
@@ -437,6 +502,10 @@ int mfa_ensemble_init(mfa_ensemble* e, void* f, const unsigned char* masks, cons
 {
     if (!e || !f) return ens_fail(MFA_ERR_INVALID, "null argument");
     if (e->masked && !masks) return ens_fail(MFA_ERR_INVALID, "ensemble was created with masks");
+    if (e->masked) {
+        const int rc = compute_near(e, masks, (hipStream_t)stream);
+        if (rc != MFA_OK) return rc;
+    }
     if (!e->ntiles) return MFA_OK;
     return with_type(e->dtype, [&](auto tv) {
         using T = decltype(tv);

@@ -33,7 +33,7 @@ def tgv_fields(torch, N, u0, dev):
 
 
 def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32", "f16dev", "bf16dev"),
-        dev="cuda:0"):
+        dev="cuda:0", layout="row"):
     import torch
     L = N / (2 * math.pi)
     nu = u0 * L / Re
@@ -45,10 +45,11 @@ def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32"
     ref_vel = {}
     for st in storages:
         tdt = getattr(torch, TORCH_DT[st])
-        a = torch.empty((27, N, N, N), dtype=tdt, device=dev)
+        shape = abi.row_shape(27, n, 0, 3) if layout == "row" else (27, N, N, N)
+        a = torch.empty(shape, dtype=tdt, device=dev)
         b = torch.empty_like(a)
-        fa = abi.field(a, n, 0, 3, abi.DTYPE[st])
-        fb = abi.field(b, n, 0, 3, abi.DTYPE[st])
+        fa = abi.field(a, n, 0, 3, abi.DTYPE[st], layout=layout)
+        fb = abi.field(b, n, 0, 3, abi.DTYPE[st], layout=layout)
         rho, vel = tgv_fields(torch, N, u0, dev)
         fr, fv = abi.field(rho, n, 0, 3), abi.field(vel, n, 0, 3)
         abi.init_equilibrium(fr, fv, fa, 3)
@@ -65,7 +66,7 @@ def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32"
             e0.record()
             for _ in range(k):
                 abi.step(fa, fb, g, omega)
-                a, b, fa, fb = b, a, fb, fa
+                fa, fb = fb, fa
             e1.record()
             torch.cuda.synchronize()
             t_kernel += e0.elapsed_time(e1) * 1e-3
@@ -96,7 +97,8 @@ def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32"
                           "ke_final": ke[-1][1]}), flush=True)
         del a, b
         torch.cuda.empty_cache()
-    return {"N": N, "u0": u0, "Re": Re, "tau": tau, "steps": steps, "results": results}
+    return {"N": N, "u0": u0, "Re": Re, "tau": tau, "steps": steps, "layout": layout,
+            "results": results}
 
 
 def main():
@@ -105,8 +107,11 @@ def main():
     p.add_argument("--steps", type=int, default=5000)
     p.add_argument("--sample", type=int, default=500)
     p.add_argument("--out", default="")
+    p.add_argument("--layout", default="row", choices=["row", "component"])
+    p.add_argument("--storages", default="f64,f32,f16dev,bf16dev")
     a = p.parse_args()
-    r = run(a.n, steps=a.steps, sample=a.sample)
+    r = run(a.n, steps=a.steps, sample=a.sample, storages=tuple(a.storages.split(",")),
+            layout=a.layout)
     if a.out:
         with open(a.out, "w") as fh:
             json.dump(r, fh)

@@ -265,8 +265,9 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
             summary[f"{mode}_field_diff_median"] = float(np.median(dmax[both])) if both.any() else None
             summary[f"{mode}_field_diff_p90"] = float(np.percentile(dmax[both], 90)) if both.any() else None
         if "pred_err_l2" in per_case:
-            summary["pred_err_l2_median"] = float(np.median(per_case["pred_err_l2"]))
-            summary["pred_err_max_median"] = float(np.median(per_case["pred_err_max"]))
+            # nan-aware: a lattice that blew up has no field to compare with
+            summary["pred_err_l2_median"] = float(np.nanmedian(per_case["pred_err_l2"]))
+            summary["pred_err_max_median"] = float(np.nanmedian(per_case["pred_err_max"]))
     return summary, per_case
 
 
@@ -277,6 +278,8 @@ def main():
     p.add_argument("--gen-tol", type=float, default=1e-6)
     p.add_argument("--gen-max-steps", type=int, default=60000)
     p.add_argument("--gen-seconds", type=float, default=None, help="wall budget of the generation")
+    p.add_argument("--keep-residual", type=float, default=1e-3,
+                   help="train only on cases whose last residual is below this")
     p.add_argument("--epochs", type=int, default=20)
     p.add_argument("--batch", type=int, default=16)
     p.add_argument("--lr", type=float, default=2e-3)
@@ -305,6 +308,9 @@ def main():
         # the lid, for instance) has no steady state to learn: drop it
         keep = torch.isfinite(targets).view(a.n_train, -1).all(dim=1).cpu().numpy()
         keep &= r["status"] != "diverged"
+        # a flow still changing by more than keep_residual per check interval
+        # (an unsteady one, behind an obstacle at high Re) is no steady state
+        keep &= r["residual"] < a.keep_residual
         if not keep.all():
             _log(stage="generate_dropped", count=int((~keep).sum()),
                  cases=[{"i": int(i), "Re": float(Re[i])} for i in np.nonzero(~keep)[0][:20]])

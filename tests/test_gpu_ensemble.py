@@ -64,15 +64,21 @@ def test_member_matches_oracle_and_moments(dev):
         assert np.array_equal(r, rho[0]) and np.array_equal(v, vel[:, 0])
 
 
-def test_masked_member_matches_oracle(dev):
+@pytest.mark.parametrize("tile", [(64, 4), (16, 16), (32, 2)])
+def test_masked_member_matches_oracle(dev, tile):
+    """Obstacles inside tiles, on tile edges and only in a tile's one-cell
+    rim (the near flag must catch pulls across tile edges); a lattice with no
+    solid at all runs the unmasked path."""
     import torch
     L = O.Lattice("D2Q9")
-    sizes = [(30, 26), (20, 20)]
-    masks = [np.zeros((26, 30), np.uint8), np.zeros((20, 20), np.uint8)]
+    sizes = [(30, 26), (20, 20), (40, 12)]
+    masks = [np.zeros((26, 30), np.uint8), np.zeros((20, 20), np.uint8),
+             np.zeros((12, 40), np.uint8)]
     masks[0][8:14, 10:19] = 1
+    masks[0][20, 16] = 1          # a lone solid cell on a tile row edge
     masks[1][3:5, 15:17] = 1
-    omegas = [1.0 / 0.6, 1.0 / 0.8]
-    ens = E.Ensemble(sizes, omegas, masks=masks)
+    omegas = [1.0 / 0.6, 1.0 / 0.8, 1.0 / 0.7]
+    ens = E.Ensemble(sizes, omegas, masks=masks, tile=tile)
     ens.init()
     ens.run(150)
     torch.cuda.synchronize()
