@@ -309,11 +309,11 @@ static void shape(const int n[3], dim3& grid, dim3& block)
 // Launch configuration of the reduced-storage step, per lattice and storage
 // type (index dtype - MFA_F32).  cells_per_thread 2 selects k_step_dev_x2
 // (block_x then counts threads, i.e. cell pairs); nt_loads applies to it.
-// Defaults: the autotuner's picks on MI355X for D3Q27 384^3 (FP32: pairs in
-// 64x2 blocks, 5.22 TB/s; FP16 / BF16: one cell per thread in 128x2 blocks,
-// 5.09 TB/s, where pairs reach only 4.0 TB/s), used for every lattice.
+// Defaults: the autotuner's picks on MI355X for D3Q27 384^3 in the
+// row-interleaved order (FP32: pairs in 64x4 blocks, 5.60 TB/s; FP16 / BF16:
+// one cell per thread in 128x1 blocks, 5.70 TB/s), used for every lattice.
 static std::mutex g_dev_mu;
-static const mfa_launch k_dev_default[3] = {{64, 2, 2, 0}, {128, 2, 1, 0}, {128, 2, 1, 0}};
+static const mfa_launch k_dev_default[3] = {{64, 4, 2, 0}, {128, 1, 1, 0}, {128, 1, 1, 0}};
 static mfa_launch g_dev_cfg[4][3] = {
     {k_dev_default[0], k_dev_default[1], k_dev_default[2]},
     {k_dev_default[0], k_dev_default[1], k_dev_default[2]},

@@ -33,7 +33,9 @@ def tgv_fields(torch, N, u0, dev):
 
 
 def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32", "f16dev", "bf16dev"),
-        dev="cuda:0", layout="row"):
+        dev="cuda:0", layout="row", tune=False):
+    """tune: time every launch candidate for each storage type on this box
+    first (muphfasa_amd.autotune) and run with the fastest."""
     import torch
     L = N / (2 * math.pi)
     nu = u0 * L / Re
@@ -43,7 +45,17 @@ def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32"
     g = abi.resolve_bc(3, [abi.BC["Periodic"]] * 6)
     results = {}
     ref_vel = {}
+    tuned = {}
     for st in storages:
+        if tune:
+            from . import autotune
+            if st == "f64":
+                best = autotune.tune_step("D3Q27", n, forced=False, layout=layout)[0]
+            else:
+                best = autotune.tune_step_dev("D3Q27", st, n, layout=layout)[0]
+            tuned[st] = best
+            print(json.dumps({"storage": st, "tuned": best}), flush=True)
+            torch.cuda.empty_cache()
         tdt = getattr(torch, TORCH_DT[st])
         shape = abi.row_shape(27, n, 0, 3) if layout == "row" else (27, N, N, N)
         a = torch.empty(shape, dtype=tdt, device=dev)
@@ -84,7 +96,8 @@ def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32"
         m1 = out[0].item()
         bytes_lu = 2 * 27 * a.element_size()
         mlups = N ** 3 * steps / t_kernel / 1e6
-        results[st] = {"mlups": round(mlups, 1), "bytes_per_lu": bytes_lu,
+        launch = abi.get_launch(3) if st == "f64" else abi.get_launch_dev(3, abi.DTYPE[st])
+        results[st] = {"mlups": round(mlups, 1), "bytes_per_lu": bytes_lu, "launch": list(launch),
                        "GBs": round(mlups * bytes_lu / 1e3, 1), "ke": ke,
                        "l2_u_err_vs_f64": errs, "mass_rel_drift": abs(m1 - m0) / m0,
                        "finite": bool(torch.isfinite(vel).all().item())}
@@ -109,9 +122,10 @@ def main():
     p.add_argument("--out", default="")
     p.add_argument("--layout", default="row", choices=["row", "component"])
     p.add_argument("--storages", default="f64,f32,f16dev,bf16dev")
+    p.add_argument("--tune", action="store_true", help="autotune the launch per storage first")
     a = p.parse_args()
     r = run(a.n, steps=a.steps, sample=a.sample, storages=tuple(a.storages.split(",")),
-            layout=a.layout)
+            layout=a.layout, tune=a.tune)
     if a.out:
         with open(a.out, "w") as fh:
             json.dump(r, fh)

@@ -31,10 +31,10 @@ def _log(**kw):
 
 
 def generate(n, seed, dtype="f32", tol=1e-6, max_steps=60000, check=1000, N=256, tile=(64, 4),
-             max_seconds=None):
+             max_seconds=None, cases=None):
     """Returns masks (n,N,N) uint8, Re (n,), fields on device (rho, vel) and
-    the run statistics."""
-    masks, Re, tau = S.make_cases(n, seed, N)
+    the run statistics.  cases: (masks, Re, tau) instead of S.make_cases."""
+    masks, Re, tau = cases if cases is not None else S.make_cases(n, seed, N)
     ens = E.Ensemble([(N, N)] * n, 1.0 / tau, dtype=dtype, tile=tile, masks=list(masks))
     ens.init()
     t0 = time.time()
@@ -178,7 +178,7 @@ MODES = ("warm", "warm_plain", "cold")
 
 
 def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, dtype="f64", N=256,
-             tile=(64, 4), max_seconds=None, modes=MODES):
+             tile=(64, 4), max_seconds=None, modes=MODES, cases=None):
     """Runs the held-out cases to the steady criterion from each start:
       cold       rest state (rho = 1, u = 0), the plain method
       warm_plain feq of the network's prediction (mass restored)
@@ -186,7 +186,7 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
                  consecutive states (mfa_ensemble_init_warm)
     and compares steps, wall time and converged fields with the cold start."""
     import torch
-    masks, Re, tau = S.make_cases(n_eval, seed, N)
+    masks, Re, tau = cases if cases is not None else S.make_cases(n_eval, seed, N)
     masks_dev = torch.from_numpy(masks).to(net.dev)
     re_dev = torch.from_numpy(S.re_feature(Re)).to(net.dev)
     res = {}

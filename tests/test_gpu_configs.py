@@ -90,6 +90,10 @@ def test_periodic_forced_d3q19_32cubed_parity(dev):
 
 
 def test_periodic_forced_d3q19_256cubed_invariants(dev):
+    """The bench workload at full size in both population orders: FP64 mass
+    conserved, x momentum grows by exactly N^3 F per step, and the
+    row-interleaved run (the bench's order) is bit-identical to the
+    component-major one after the same steps."""
     import torch
     N = 256
     n = [N, N, N]
@@ -101,23 +105,31 @@ def test_periodic_forced_d3q19_256cubed_invariants(dev):
     rho = torch.ones((1, N, N, N), dtype=torch.float64, device=dev)
     vel = (torch.rand((3, N, N, N), dtype=torch.float64, device=dev, generator=gen) * 2 - 1) * 1e-3
     fr, fv = abi.field(rho, n, 0, 3), abi.field(vel, n, 0, 3)
-    ta = torch.empty((19, N, N, N), dtype=torch.float64, device=dev)
-    tb = torch.empty_like(ta)
-    fa, fb = abi.field(ta, n, 0, 3), abi.field(tb, n, 0, 3)
-    abi.init_equilibrium(fr, fv, fa, 2, F)
-    out = torch.zeros(8, dtype=torch.float64, device=dev)
-    abi.reduce(fa, 2, abi.RED["mass"], out.data_ptr())
-    abi.reduce(fa, 2, abi.RED["momentum"], out[1:].data_ptr())
-    torch.cuda.synchronize()
-    m0, p0 = out[0].item(), out[1:4].cpu().numpy().copy()
     steps = 50
-    ta, fa = _run_gpu(g, ta, fa, tb, fb, omega, F, steps)
-    abi.reduce(fa, 2, abi.RED["mass"], out.data_ptr())
-    abi.reduce(fa, 2, abi.RED["momentum"], out[1:].data_ptr())
-    torch.cuda.synchronize()
-    m1, p1 = out[0].item(), out[1:4].cpu().numpy()
-    assert abs(m1 - m0) / m0 < 1e-13
-    gain = steps * N ** 3 * F[0]
-    assert abs((p1[0] - p0[0]) - gain) / gain < 1e-9
-    assert abs(p1[1] - p0[1]) < 1e-9 and abs(p1[2] - p0[2]) < 1e-9
-    assert torch.isfinite(ta).all().item()
+    final = {}
+    for layout in ("component", "row"):
+        print(f"case d3q19 256^3 layout={layout}", flush=True)
+        shape = (19, N, N, N) if layout == "component" else abi.row_shape(19, n, 0, 3)
+        ta = torch.empty(shape, dtype=torch.float64, device=dev)
+        tb = torch.empty_like(ta)
+        fa, fb = abi.field(ta, n, 0, 3, layout=layout), abi.field(tb, n, 0, 3, layout=layout)
+        r0, v0 = rho.clone(), vel.clone()
+        abi.init_equilibrium(abi.field(r0, n, 0, 3), abi.field(v0, n, 0, 3), fa, 2, F)
+        out = torch.zeros(8, dtype=torch.float64, device=dev)
+        abi.reduce(fa, 2, abi.RED["mass"], out.data_ptr())
+        abi.reduce(fa, 2, abi.RED["momentum"], out[1:].data_ptr())
+        torch.cuda.synchronize()
+        m0, p0 = out[0].item(), out[1:4].cpu().numpy().copy()
+        ta, fa = _run_gpu(g, ta, fa, tb, fb, omega, F, steps)
+        abi.reduce(fa, 2, abi.RED["mass"], out.data_ptr())
+        abi.reduce(fa, 2, abi.RED["momentum"], out[1:].data_ptr())
+        torch.cuda.synchronize()
+        m1, p1 = out[0].item(), out[1:4].cpu().numpy()
+        assert abs(m1 - m0) / m0 < 1e-13
+        gain = steps * N ** 3 * F[0]
+        assert abs((p1[0] - p0[0]) - gain) / gain < 1e-9
+        assert abs(p1[1] - p0[1]) < 1e-9 and abs(p1[2] - p0[2]) < 1e-9
+        assert torch.isfinite(ta).all().item()
+        final[layout] = ta if layout == "component" else ta.permute(2, 0, 1, 3)
+        del tb
+    assert torch.equal(final["component"], final["row"])

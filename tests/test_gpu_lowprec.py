@@ -206,3 +206,37 @@ def test_paired_kernel_is_bit_identical_to_one_cell_kernel(dev, lat, dt, case):
         abi.set_launch_dev(lid, did)
     for k, o in enumerate(outs[1:]):
         assert np.array_equal(o, outs[0]), (case, launches[k])
+
+
+@pytest.mark.parametrize("lat", ["D3Q19", "D3Q27"])
+@pytest.mark.parametrize("dt", ["f32", "f16dev", "bf16dev"])
+def test_row_interleaved_order_gives_the_same_bits(dev, lat, dt):
+    """The reduced-storage kernels (paired and one-cell launches) read and
+    write the row-interleaved order through strides: the same step from the
+    same stored deviations gives identical bits in both orders."""
+    import torch
+    L = O.Lattice(lat)
+    lid, did = abi.LATTICE[lat], abi.DTYPE[dt]
+    n = [24, 10, 7]
+    rng = np.random.default_rng(17)
+    f64 = L.w[:, None, None, None] * (1 + 0.05 * rng.standard_normal((L.Q, n[2], n[1], n[0])))
+    src = torch.from_numpy(f64).to(dev)
+    a = _dev_tensor(torch, L.Q, n, dt, dev)
+    abi.convert(abi.field(src, n, 0, 3), abi.field(a, n, 0, 3, did), lid)
+    ar = a.permute(1, 2, 0, 3).contiguous()            # (z, y, q, x)
+    g = abi.resolve_bc(lid, [abi.BC["Periodic"]] * 6)
+    F = [2e-4, -1e-4, 5e-5]
+    try:
+        for cfg in ((64, 2, 2, 1), (128, 2, 1, 0)):
+            abi.set_launch_dev(lid, did, *cfg)
+            b = torch.empty_like(a)
+            abi.step(abi.field(a, n, 0, 3, did), abi.field(b, n, 0, 3, did), g, 1.0 / 0.6, F)
+            br = torch.empty_like(ar)
+            abi.step(abi.field(ar, n, 0, 3, did, layout="row"),
+                     abi.field(br, n, 0, 3, did, layout="row"), g, 1.0 / 0.6, F)
+            torch.cuda.synchronize()
+            assert torch.equal(br.permute(2, 0, 1, 3).contiguous().view(torch.int16 if dt != "f32"
+                                                                       else torch.int32),
+                               b.view(torch.int16 if dt != "f32" else torch.int32)), cfg
+    finally:
+        abi.set_launch_dev(lid, did)
