@@ -7,7 +7,8 @@
  * every lattice uses the reference's per-cell step (the same kernels as
  * mfa_step) and one shared resolved boundary table (mfa_resolve_bc).
  * Populations of lattice l live at element offset pop_off[l] of one buffer,
- * component-major [q][y][x]; per-cell arrays (masks: uint8, 1 = solid; rho;
+ * row-interleaved [y][q][x] (the 9 rows of one y side by side, element
+ * (x,y,q) at pop_off + x + y*9*nx + q*nx); per-cell arrays (masks: uint8, 1 = solid; rho;
  * vel as two planes of total_cells) at cell_off[l].  The ensemble object
  * owns only its small metadata; population and field buffers belong to the
  * caller.  Status codes as in mfa_lbm.h; message from mfa_ensemble_error().

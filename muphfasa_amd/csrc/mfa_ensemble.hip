@@ -3,7 +3,8 @@
 // whole ensemble.
 //
 // Each lattice l (nx_l x ny_l cells, its own omega) stores its populations
-// contiguously, component-major ([q][y][x]) at pop_off_l.  The work is a
+// contiguously at pop_off_l in the row-interleaved order ([y][q][x]: the Q
+// rows of one y side by side).  The work is a
 // flat list of tiles (lattice, x0, y0) of TX x TY cells, one cell per thread;
 // a fixed grid of blocks walks the list with a block stride (persistent
 // blocks, no atomics, deterministic).  Retired (converged) lattices are
@@ -61,6 +62,16 @@ __device__ __forceinline__ GeoRef geo_for(const LatInfo& li, const BcTable* bc)
     return g;
 }
 
+// Population view of one lattice: row-interleaved, the Q rows of one y side
+// by side ([y][q][x]), so that a tile's stores of all Q populations land in
+// one contiguous run per row (the single-lattice engine streams this order
+// 8-14 % faster than component planes for D2Q9).
+template <class L>
+__device__ __forceinline__ View pop_view(const LatInfo& li)
+{
+    return View{(long long)L::Q * li.nx, (long long)L::Q * li.nx * li.ny, (long long)li.nx};
+}
+
 // Half-way bounce-back on solid obstacle cells: a population pulled from a
 // solid neighbour is the cell's own population pointing into it.
 template <class L, class V, class Ld>
@@ -87,15 +98,16 @@ __global__ __launch_bounds__(256) void k_batch_step(const T* __restrict__ fin, T
         const LatInfo li = lats[tl.lat];
         const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
         if (x >= li.nx || y >= li.ny) continue;
-        const View v{li.nx, (long long)li.nx * li.ny, (long long)li.nx * li.ny};
+        const View v = pop_view<L>(li);
         const T* a = fin + li.pop_off;
         T* b = fout + li.pop_off;
-        const long long self = x + (long long)y * li.nx;
+        const long long self = x + (long long)y * v.sy;      // population index
+        const long long cell = x + (long long)y * li.nx;     // mask index
         const unsigned char* mask = MASKED ? masks + li.cell_off : nullptr;
         const bool near = MASKED && tl.near;
         const GeoRef g = geo_for<L>(li, bc);
         if constexpr (std::is_same<T, double>::value) {
-            if (near && mask[self]) {
+            if (near && mask[cell]) {
                 for_q<L>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
                     b[q * v.sq + self] = L::w(q);
@@ -108,7 +120,7 @@ __global__ __launch_bounds__(256) void k_batch_step(const T* __restrict__ fin, T
                 obstacle_bb<L>(f, mask, x, y, li.nx, li.ny, [&](int m) { return a[m * v.sq + self]; });
             collide_store<L, false>(f, b, self, v, li.omega, 0.0, 0.0, 0.0);
         } else {
-            if (near && mask[self]) {
+            if (near && mask[cell]) {
                 for_q<L>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
                     b[q * v.sq + self] = from_f<T>(0.0f);
@@ -139,13 +151,13 @@ __global__ __launch_bounds__(256) void k_batch_moments(const T* __restrict__ fin
         const LatInfo li = lats[tl.lat];
         const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
         if (x >= li.nx || y >= li.ny) continue;
-        const View v{li.nx, (long long)li.nx * li.ny, (long long)li.nx * li.ny};
+        const View v = pop_view<L>(li);
         const T* a = fin + li.pop_off;
-        const long long self = x + (long long)y * li.nx;
-        const long long cell = li.cell_off + self;
+        const long long self = x + (long long)y * v.sy;      // population index
+        const long long cell = li.cell_off + x + (long long)y * li.nx;
         const unsigned char* mask = MASKED ? masks + li.cell_off : nullptr;
         const bool near = MASKED && tl.near;
-        if (near && mask[self]) {
+        if (near && mask[cell - li.cell_off]) {
             rho[cell] = 1.0;
             vel[cell] = 0.0;
             vel[total_cells + cell] = 0.0;
@@ -189,9 +201,9 @@ __global__ __launch_bounds__(256) void k_batch_init(T* __restrict__ fout, const 
         const LatInfo li = lats[tl.lat];
         const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
         if (x >= li.nx || y >= li.ny) continue;
-        const long long sq = (long long)li.nx * li.ny;
-        const long long self = x + (long long)y * li.nx;
-        const long long cell = li.cell_off + self;
+        const View v = pop_view<L>(li);
+        const long long self = x + (long long)y * v.sy;
+        const long long cell = li.cell_off + x + (long long)y * li.nx;
         const bool solid = MASKED && masks[cell];
         const double r = (rho && !solid) ? rho[cell] : 1.0;
         const double u0 = (vel && !solid) ? vel[cell] : 0.0;
@@ -201,8 +213,8 @@ __global__ __launch_bounds__(256) void k_batch_init(T* __restrict__ fout, const 
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const double fe = feq<L, q>(r, u0, u1, 0.0, usq);
-            if constexpr (std::is_same<T, double>::value) b[q * sq + self] = fe;
-            else b[q * sq + self] = from_f<T>((float)(fe - L::w(q)));
+            if constexpr (std::is_same<T, double>::value) b[q * v.sq + self] = fe;
+            else b[q * v.sq + self] = from_f<T>((float)(fe - L::w(q)));
         });
     }
 }
@@ -242,11 +254,11 @@ __global__ __launch_bounds__(256) void k_batch_average(T* __restrict__ fa, const
         const LatInfo li = lats[tl.lat];
         const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
         if (x >= li.nx || y >= li.ny) continue;
-        const long long sq = (long long)li.nx * li.ny;
-        const long long self = li.pop_off + x + (long long)y * li.nx;
+        const View v = pop_view<L>(li);
+        const long long self = li.pop_off + x + (long long)y * v.sy;
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const long long i = self + q * sq;
+            const long long i = self + q * v.sq;
             if constexpr (std::is_same<T, double>::value) fa[i] = 0.5 * (fa[i] + fb[i]);
             else fa[i] = from_f<T>(0.5f * (to_f(fa[i]) + to_f(fb[i])));
         });

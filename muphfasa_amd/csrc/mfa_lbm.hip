@@ -407,17 +407,17 @@ static void launch_shape(const int n[3], dim3& grid, dim3& block, int grow)
 
 // Launch configuration of the fused step kernel, per lattice index (D2Q9,
 // D3Q15, D3Q19, D3Q27), set by the autotuner through mfa_set_launch.  The
-// defaults are the autotuner's picks on MI355X for D2Q9 128^2, D3Q19 256^3
-// FP64 forced periodic in the row-interleaved order (block 128x2, one cell
-// per thread, non-temporal loads for the c_x = 0 rows: 0.844 ms against
-// 0.876 ms for 256x1 with every load non-temporal) and D3Q27 384^3 FP64
-// periodic in the same order (the same shape: 4.22 ms, 5.80 TB/s); D3Q15
-// takes D3Q19's.
+// defaults are the autotuner's picks on MI355X in the row-interleaved order:
+// block 128x2, one cell per thread, non-temporal loads for the c_x = 0 rows
+// (the x-shifted rows stay cached: neighbouring wavefronts share their
+// lines).  D3Q19 256^3 FP64 forced periodic: 0.844 ms against 0.876 ms for
+// 256x1 with every load non-temporal; D3Q27 384^3: 4.22 ms, 5.80 TB/s;
+// D2Q9 8192^2: 1.49 ms, 6.48 TB/s.  D3Q15 takes D3Q19's.
 static std::mutex g_cfg_mu;
 static const mfa_launch k_default_cfg[4] = {
-    {256, 1, 1, 0}, {128, 2, 1, 2}, {128, 2, 1, 2}, {128, 2, 1, 2}};
+    {128, 2, 1, 2}, {128, 2, 1, 2}, {128, 2, 1, 2}, {128, 2, 1, 2}};
 static mfa_launch g_cfg[4] = {
-    {256, 1, 1, 0}, {128, 2, 1, 2}, {128, 2, 1, 2}, {128, 2, 1, 2}};
+    {128, 2, 1, 2}, {128, 2, 1, 2}, {128, 2, 1, 2}, {128, 2, 1, 2}};
 
 static bool valid_cfg(const mfa_launch& c)
 {

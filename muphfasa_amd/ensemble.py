@@ -185,9 +185,12 @@ class Ensemble:
         return rho, np.stack([u, v])
 
     def lattice_populations(self, l):
+        """Populations of lattice l as (9, 1, ny, nx), component-major like the
+        oracle's fabs (the engine stores them row-interleaved, [y][q][x])."""
         nx, ny = self.sizes[l]
         p0 = self.pop_off[l]
-        return self.fa[p0:p0 + 9 * nx * ny].double().cpu().numpy().reshape(9, 1, ny, nx)
+        rows = self.fa[p0:p0 + 9 * nx * ny].double().cpu().numpy().reshape(ny, 9, nx)
+        return np.ascontiguousarray(rows.transpose(1, 0, 2)).reshape(9, 1, ny, nx)
 
 
 def run_to_steady(ens: Ensemble, check=1000, tol=1e-8, max_steps=200000, graph=True, log=None,
