@@ -24,7 +24,8 @@ mfa_sim* mfa_sim_create(const char* inputs_text, int noverrides, const char* con
 void mfa_sim_destroy(mfa_sim* s);
 
 int mfa_sim_initialize(mfa_sim* s);                 /* ISimulation::initialize */
-int mfa_sim_advance(mfa_sim* s, int nsteps);        /* nsteps x ISimulation::advance */
+int mfa_sim_advance(mfa_sim* s, int nsteps);        /* nsteps x ISimulation::advance (replayed
+                                                       from captured HIP graphs of 64 steps) */
 int mfa_sim_update_macroscopic(mfa_sim* s, int step);
 int mfa_sim_write_output(mfa_sim* s, int step);
 int mfa_sim_sync(mfa_sim* s);

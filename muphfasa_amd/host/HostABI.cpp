@@ -51,9 +51,7 @@ int mfa_sim_initialize(mfa_sim* s) { return guarded([&] { s->sim->initialize(); 
 
 int mfa_sim_advance(mfa_sim* s, int nsteps)
 {
-    return guarded([&] {
-        for (int i = 0; i < nsteps; ++i) s->sim->advance();
-    });
+    return guarded([&] { s->sim->advanceSteps(nsteps); });
 }
 
 int mfa_sim_update_macroscopic(mfa_sim* s, int step)
@@ -136,11 +134,19 @@ int mfa_run_main(int argc, const char* const* argv)
         const int max_step = params.maxStep();
         const double stop_time_lat = sim->units().toLatticeTime(params.stopTime());
         std::printf("%g\n", stop_time_lat);
-        // max_step + 1 iterations, as in the reference's loop
-        for (int step = sim->startStep(); step <= max_step; ++step) {
+        // max_step + 1 iterations of advance(); writeOutput(step), as in the
+        // reference's loop.  Runs of steps that write nothing and stay inside
+        // the stop time go to advanceSteps (graph replay) in one call.
+        int step = sim->startStep();
+        while (step <= max_step) {
             if (stop_time_lat > 0 && stop_time_lat < sim->time()) break;
-            sim->advance();
-            sim->writeOutput(step);
+            int k = 1;
+            while (step + k <= max_step && !sim->outputDue(step + k - 1) &&
+                   !(stop_time_lat > 0 && stop_time_lat < sim->time() + k))
+                ++k;
+            sim->advanceSteps(k);
+            sim->writeOutput(step + k - 1);
+            step += k;
         }
         hip_check(hipStreamSynchronize(sim->stream()), "sync");
     });

@@ -94,11 +94,19 @@ public:
     explicit SimulationImpl(const InputParser& params);
     ~SimulationImpl() override
     {
+        for (auto& g : m_graph)
+            if (g) (void)hipGraphExecDestroy(g);
         if (m_stream) (void)hipStreamDestroy(m_stream);
     }
 
     void initialize() override;
     void advance() override;
+    void advanceSteps(int n) override;
+    bool outputDue(int step) const noexcept override
+    {
+        return (m_params.plotInterval() > 0 && step % m_params.plotInterval() == 0) ||
+               (m_params.checkpointInterval() > 0 && step % m_params.checkpointInterval() == 0);
+    }
     void updateMacroscopic(int step) override;
     void writeOutput(int step) override;
 
@@ -138,6 +146,9 @@ private:
     int m_start_step = 0;
     int m_macro_update_step = -1;
     hipStream_t m_stream = nullptr;
+    // graphs of kGraphSteps steps starting from m_fa (0) and from m_fb (1)
+    static constexpr int kGraphSteps = 64;
+    hipGraphExec_t m_graph[2] = {nullptr, nullptr};
 };
 
 SimulationImpl::SimulationImpl(const InputParser& params) : m_params(params)
@@ -260,6 +271,40 @@ void SimulationImpl::advance()
               "advance");
     std::swap(m_fin, m_fout);
     m_time += 1.0;
+}
+
+void SimulationImpl::advanceSteps(int n)
+{
+    // Whole graphs first: kGraphSteps (even) steps leave the state in the
+    // buffer they started from, so m_fin / m_fout keep their roles.
+    while (n >= kGraphSteps) {
+        const int which = (m_fin == &m_fa) ? 0 : 1;
+        if (!m_graph[which]) {
+            hipGraph_t graph = nullptr;
+            hip_check(hipStreamBeginCapture(m_stream, hipStreamCaptureModeThreadLocal), "capture");
+            DeviceField* a = m_fin;
+            DeviceField* b = m_fout;
+            int rc = MFA_OK;
+            for (int s = 0; s < kGraphSteps && rc == MFA_OK; ++s) {
+                rc = mfa_step(a->view(), b->view(), &m_bc, m_omega,
+                              m_has_forcing ? m_force.data() : nullptr, m_stream);
+                std::swap(a, b);
+            }
+            const hipError_t ce = hipStreamEndCapture(m_stream, &graph);
+            if (rc != MFA_OK || ce != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                mfa_check(rc, "advanceSteps (capture)");
+                hip_check(ce, "hipStreamEndCapture");
+            }
+            const hipError_t ie = hipGraphInstantiate(&m_graph[which], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            hip_check(ie, "hipGraphInstantiate");
+        }
+        hip_check(hipGraphLaunch(m_graph[which], m_stream), "hipGraphLaunch");
+        m_time += kGraphSteps;
+        n -= kGraphSteps;
+    }
+    while (n-- > 0) advance();
 }
 
 void SimulationImpl::updateMacroscopic(int step)

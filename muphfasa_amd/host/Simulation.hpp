@@ -52,6 +52,12 @@ public:
     virtual const Geometry& geom() const noexcept = 0;
 
     // engine additions
+    // n x advance(), replayed from captured HIP graphs of an even number of
+    // steps where n allows (the same kernels and results as n advance()
+    // calls, without a host launch per step)
+    virtual void advanceSteps(int n) = 0;
+    // writeOutput(step) would write a plotfile or a checkpoint
+    virtual bool outputDue(int step) const noexcept = 0;
     virtual int lattice() const noexcept = 0;
     virtual const mfa_geom_bc& boundary() const noexcept = 0;
     virtual const std::array<double, 3>& forceLattice() const noexcept = 0;
