@@ -60,8 +60,11 @@ struct ConvShape {
 };
 
 // ---------------------------------------------------------------- forward --
-// Block: 4 waves.  WC waves along output channels (1 or 2), 4/WC along pixels.
-template <int WC>
+// Block: 4 waves.  WC waves along output channels (1 or 2), 4/WC along pixels;
+// a wave computes 16*TI pixels x 16*TJ channels (TI x TJ MFMA tiles per
+// 32-deep slice of k).  TI = 4, TJ = 1 serves layers with 16 or fewer output
+// channels without idle MFMA columns.
+template <int WC, int TI, int TJ>
 __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, const bf16* __restrict__ w,
                                                   const float* __restrict__ bias, bf16* __restrict__ y,
                                                   float* __restrict__ yf, const ConvShape s, const int relu)
@@ -70,16 +73,16 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, co
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wp = wave % WP, wc = wave / WP;
     const long long P = (long long)s.N * s.Ho * s.Wo;
-    const long long p0 = (long long)blockIdx.x * (32 * WP) + 32 * wp;
-    const int c0 = blockIdx.y * (32 * WC) + 32 * wc;
+    const long long p0 = (long long)blockIdx.x * (16 * TI * WP) + 16 * TI * wp;
+    const int c0 = blockIdx.y * (16 * TJ * WC) + 16 * TJ * wc;
     const int K = 9 * s.Cin;
     const int r = lane & 15, kg = lane >> 4;
 
     // pixel coordinates of the two A rows this lane feeds
-    int pn[2], py[2], px[2];
-    bool pv[2];
+    int pn[TI], py[TI], px[TI];
+    bool pv[TI];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < TI; ++i) {
         const long long p = p0 + 16 * i + r;
         pv[i] = p < P;
         const long long pp = pv[i] ? p : 0;
@@ -88,38 +91,38 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, co
         py[i] = (int)(t % s.Ho);
         pn[i] = (int)(t / s.Ho);
     }
-    f32x4 acc[2][2];
+    f32x4 acc[TI][TJ];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < TJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     for (int k0 = 0; k0 < K; k0 += 32) {
         const int k = k0 + 8 * kg;
-        bf16x8 a[2], b[2];
+        bf16x8 a[TI], b[TJ];
         const bool kv = k < K;
         const int tap = kv ? k / s.Cin : 0, ci = kv ? k % s.Cin : 0;
         const int ky = tap / 3, kx = tap % 3;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < TI; ++i) {
             const int yy = py[i] * s.stride + ky - 1, xx = px[i] * s.stride + kx - 1;
             const bool ok = kv && pv[i] && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W;
             a[i] = ok ? load8(x + (((long long)pn[i] * s.H + yy) * s.W + xx) * s.Cin + ci) : zero8();
         }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < TJ; ++j) {
             const int co = c0 + 16 * j + r;
             b[j] = (kv && co < s.Cout) ? load8(w + (long long)co * K + k) : zero8();
         }
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < TI; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+            for (int j = 0; j < TJ; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < TJ; ++j) {
             const int co = c0 + 16 * j + r;
             if (co >= s.Cout) continue;
             const float bv = bias ? bias[co] : 0.f;
@@ -139,7 +142,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, co
 // dX[p_in][ci] = sum over (tap, co) dY[p_out][co] * WT[tap][ci][co], with
 // p_out = ((y_in + 1 - ky) / stride, (x_in + 1 - kx) / stride) when exact.
 // Epilogue: dX = (accumulate ? dX : 0) + value * (mask ? (mask > 0) : 1).
-template <int WC>
+// Wave tile as in the forward kernel: 16*TI pixels x 16*TJ input channels.
+template <int WC, int TI, int TJ>
 __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ dy, const bf16* __restrict__ wt,
                                                        bf16* __restrict__ dx, const bf16* __restrict__ mask,
                                                        const ConvShape s, const int accumulate)
@@ -148,14 +152,14 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wp = wave % WP, wc = wave / WP;
     const long long P = (long long)s.N * s.H * s.W;
-    const long long p0 = (long long)blockIdx.x * (32 * WP) + 32 * wp;
-    const int c0 = blockIdx.y * (32 * WC) + 32 * wc;
+    const long long p0 = (long long)blockIdx.x * (16 * TI * WP) + 16 * TI * wp;
+    const int c0 = blockIdx.y * (16 * TJ * WC) + 16 * TJ * wc;
     const int K = 9 * s.Cout;
     const int r = lane & 15, kg = lane >> 4;
-    int pn[2], py[2], px[2];
-    bool pv[2];
+    int pn[TI], py[TI], px[TI];
+    bool pv[TI];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < TI; ++i) {
         const long long p = p0 + 16 * i + r;
         pv[i] = p < P;
         const long long pp = pv[i] ? p : 0;
@@ -164,19 +168,19 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ 
         py[i] = (int)(t % s.H);
         pn[i] = (int)(t / s.H);
     }
-    f32x4 acc[2][2];
+    f32x4 acc[TI][TJ];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < TJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < K; k0 += 32) {
         const int k = k0 + 8 * kg;
         const bool kv = k < K;
         const int tap = kv ? k / s.Cout : 0, co = kv ? k % s.Cout : 0;
         const int ky = tap / 3, kx = tap % 3;
-        bf16x8 a[2], b[2];
+        bf16x8 a[TI], b[TJ];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < TI; ++i) {
             const int ty = py[i] + 1 - ky, tx = px[i] + 1 - kx;
             bool ok = kv && pv[i] && ty >= 0 && tx >= 0;
             int yo = 0, xo = 0;
@@ -188,19 +192,19 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ 
             a[i] = ok ? load8(dy + (((long long)pn[i] * s.Ho + yo) * s.Wo + xo) * s.Cout + co) : zero8();
         }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < TJ; ++j) {
             const int ci = c0 + 16 * j + r;
             b[j] = (kv && ci < s.Cin) ? load8(wt + ((long long)tap * s.Cin + ci) * s.Cout + co) : zero8();
         }
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < TI; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+            for (int j = 0; j < TJ; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < TJ; ++j) {
             const int ci = c0 + 16 * j + r;
             if (ci >= s.Cin) continue;
 #pragma unroll
@@ -219,16 +223,20 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ 
 // -------------------------------------------------------- weight gradient --
 // Block tile: 64 output channels x 64 columns of k; each block reduces one
 // contiguous range of output pixels (split over blockIdx.z) and adds its
-// FP32 partial into dW (and dbias) atomically.  Per 32-pixel slice the dY
-// and im2col(X) tiles are staged in LDS transposed ([row][pixel]) so each
-// lane's 8 pixels are one 16-byte LDS read.
+// FP32 partial into dW (and dbias) atomically.  Per stage of kWgStage
+// pixels the dY and im2col(X) tiles are staged in LDS transposed
+// ([row][pixel]) so each lane's 8 pixels are one 16-byte LDS read; with 128
+// pixels a stage each wave issues 16 MFMAs between two barriers.
+constexpr int kWgStage = 128;
+
 __global__ __launch_bounds__(256) void k_conv_bwd_weight(const bf16* __restrict__ x,
                                                          const bf16* __restrict__ dy,
                                                          float* __restrict__ dw, float* __restrict__ db,
                                                          const ConvShape s, const int pix_per_block)
 {
-    __shared__ __attribute__((aligned(16))) bf16 sA[64][40];  // [co][p]  (32 + pad)
-    __shared__ __attribute__((aligned(16))) bf16 sB[64][40];  // [k][p]
+    constexpr int PS = kWgStage;
+    __shared__ __attribute__((aligned(16))) bf16 sA[64][PS + 8];  // [co][p]  (+ pad)
+    __shared__ __attribute__((aligned(16))) bf16 sB[64][PS + 8];  // [k][p]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = wave & 1, wcl = wave >> 1;
     const int co0 = blockIdx.x * 64, kk0 = blockIdx.y * 64;
@@ -243,52 +251,59 @@ __global__ __launch_bounds__(256) void k_conv_bwd_weight(const bf16* __restrict_
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     float bsum = 0.f;
-    // staging assignment: 256 threads; each loads one 8-channel group of one pixel
-    const int sp = threadIdx.x >> 3;   // pixel within the 32-slice
-    const int sg = threadIdx.x & 7;    // group of 8 rows (64 rows = 8 groups)
-    for (long long ps = pbeg; ps < pend; ps += 32) {
-        const long long p = ps + sp;
-        const bool pv = p < pend;
-        int n = 0, yo = 0, xo = 0;
-        if (pv) {
-            xo = (int)(p % s.Wo);
-            const long long t = p / s.Wo;
-            yo = (int)(t % s.Ho);
-            n = (int)(t / s.Ho);
-        }
-        // dY slice -> sA[co][p]
-        {
-            const int co = co0 + 8 * sg;
-            bf16x8 v = (pv && co < s.Cout) ? load8(dy + p * s.Cout + co) : zero8();
+    for (long long ps = pbeg; ps < pend; ps += PS) {
+        // staging: PS pixels x 8 groups of 8 rows per operand, PS*8/256 items per thread
 #pragma unroll
-            for (int e = 0; e < 8; ++e) sA[8 * sg + e][sp] = v[e];
-        }
-        // im2col(X) slice -> sB[k][p]
-        {
-            const int k = kk0 + 8 * sg;
-            bf16x8 v = zero8();
-            if (pv && k < K) {
-                const int tap = k / s.Cin, ci = k % s.Cin;
-                const int yy = yo * s.stride + tap / 3 - 1, xx = xo * s.stride + tap % 3 - 1;
-                if (yy >= 0 && yy < s.H && xx >= 0 && xx < s.W)
-                    v = load8(x + (((long long)n * s.H + yy) * s.W + xx) * s.Cin + ci);
+        for (int m = 0; m < PS * 8 / 256; ++m) {
+            const int item = threadIdx.x + 256 * m;
+            const int sp = item >> 3;           // pixel within the stage
+            const int sg = item & 7;            // group of 8 rows
+            const long long p = ps + sp;
+            const bool pv = p < pend;
+            int n = 0, yo = 0, xo = 0;
+            if (pv) {
+                xo = (int)(p % s.Wo);
+                const long long t = p / s.Wo;
+                yo = (int)(t % s.Ho);
+                n = (int)(t / s.Ho);
             }
+            // dY slice -> sA[co][p]
+            {
+                const int co = co0 + 8 * sg;
+                const bf16x8 v = (pv && co < s.Cout) ? load8(dy + p * s.Cout + co) : zero8();
 #pragma unroll
-            for (int e = 0; e < 8; ++e) sB[8 * sg + e][sp] = v[e];
+                for (int e = 0; e < 8; ++e) sA[8 * sg + e][sp] = v[e];
+            }
+            // im2col(X) slice -> sB[k][p]
+            {
+                const int k = kk0 + 8 * sg;
+                bf16x8 v = zero8();
+                if (pv && k < K) {
+                    const int tap = k / s.Cin, ci = k % s.Cin;
+                    const int yy = yo * s.stride + tap / 3 - 1, xx = xo * s.stride + tap % 3 - 1;
+                    if (yy >= 0 && yy < s.H && xx >= 0 && xx < s.W)
+                        v = load8(x + (((long long)n * s.H + yy) * s.W + xx) * s.Cin + ci);
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) sB[8 * sg + e][sp] = v[e];
+            }
         }
         __syncthreads();
-        bf16x8 a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = *(const bf16x8*)&sA[32 * wr + 16 * i + r][8 * kg];
+        for (int kk = 0; kk < PS / 32; ++kk) {
+            bf16x8 a[2], b[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = *(const bf16x8*)&sB[32 * wcl + 16 * j + r][8 * kg];
+            for (int i = 0; i < 2; ++i) a[i] = *(const bf16x8*)&sA[32 * wr + 16 * i + r][32 * kk + 8 * kg];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) b[j] = *(const bf16x8*)&sB[32 * wcl + 16 * j + r][32 * kk + 8 * kg];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+        }
         if (db && blockIdx.y == 0 && threadIdx.x < 64) {
 #pragma unroll 8
-            for (int q = 0; q < 32; ++q) bsum += (float)sA[threadIdx.x][q];
+            for (int q = 0; q < PS; ++q) bsum += (float)sA[threadIdx.x][q];
         }
         __syncthreads();
     }
@@ -468,12 +483,16 @@ int mfa_nn_conv3x3(const void* x, const void* w, const float* bias, void* y, flo
         return nn_fail(-1, "mfa_nn_conv3x3: channels must be multiples of 8, stride 1 or 2");
     ConvShape s{N, H, W, Cin, Cout, stride, (H + stride - 1) / stride, (W + stride - 1) / stride};
     const long long P = (long long)N * s.Ho * s.Wo;
-    if (Cout <= 32) {
+    // block tiles (pixels x channels): 256 x 16, 128 x 32, 64 x 64
+    if (Cout <= 16) {
+        dim3 grid((unsigned)((P + 255) / 256), (Cout + 15) / 16);
+        hipLaunchKernelGGL((k_conv_fwd<1, 4, 1>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)w, bias, (bf16*)y, y_f32, s, relu);
+    } else if (Cout <= 32) {
         dim3 grid((unsigned)((P + 127) / 128), (Cout + 31) / 32);
-        hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)w, bias, (bf16*)y, y_f32, s, relu);
+        hipLaunchKernelGGL((k_conv_fwd<1, 2, 2>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)w, bias, (bf16*)y, y_f32, s, relu);
     } else {
         dim3 grid((unsigned)((P + 63) / 64), (Cout + 63) / 64);
-        hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)w, bias, (bf16*)y, y_f32, s, relu);
+        hipLaunchKernelGGL((k_conv_fwd<2, 2, 2>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)w, bias, (bf16*)y, y_f32, s, relu);
     }
     NN_HIP(hipGetLastError());
     return 0;
@@ -487,12 +506,15 @@ int mfa_nn_conv3x3_bwd_data(const void* dy, const void* wt, void* dx, const void
         return nn_fail(-1, "mfa_nn_conv3x3_bwd_data: channels must be multiples of 8, stride 1 or 2");
     ConvShape s{N, H, W, Cin, Cout, stride, (H + stride - 1) / stride, (W + stride - 1) / stride};
     const long long P = (long long)N * H * W;
-    if (Cin <= 32) {
+    if (Cin <= 16) {
+        dim3 grid((unsigned)((P + 255) / 256), (Cin + 15) / 16);
+        hipLaunchKernelGGL((k_conv_bwd_data<1, 4, 1>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (const bf16*)wt, (bf16*)dx, (const bf16*)mask, s, accumulate);
+    } else if (Cin <= 32) {
         dim3 grid((unsigned)((P + 127) / 128), (Cin + 31) / 32);
-        hipLaunchKernelGGL(k_conv_bwd_data<1>, grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (const bf16*)wt, (bf16*)dx, (const bf16*)mask, s, accumulate);
+        hipLaunchKernelGGL((k_conv_bwd_data<1, 2, 2>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (const bf16*)wt, (bf16*)dx, (const bf16*)mask, s, accumulate);
     } else {
         dim3 grid((unsigned)((P + 63) / 64), (Cin + 63) / 64);
-        hipLaunchKernelGGL(k_conv_bwd_data<2>, grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (const bf16*)wt, (bf16*)dx, (const bf16*)mask, s, accumulate);
+        hipLaunchKernelGGL((k_conv_bwd_data<2, 2, 2>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (const bf16*)wt, (bf16*)dx, (const bf16*)mask, s, accumulate);
     }
     NN_HIP(hipGetLastError());
     return 0;
@@ -507,10 +529,10 @@ int mfa_nn_conv3x3_bwd_weight(const void* x, const void* dy, float* dw, float* d
     ConvShape s{N, H, W, Cin, Cout, stride, (H + stride - 1) / stride, (W + stride - 1) / stride};
     const long long P = (long long)N * s.Ho * s.Wo;
     const int tiles = ((Cout + 63) / 64) * ((9 * Cin + 63) / 64);
-    // enough pixel slices to fill the chip (~2048 blocks), 32-aligned
+    // enough pixel slices to fill the chip (~2048 blocks), whole LDS stages
     long long per = (P * tiles + 2047) / 2048;
-    per = ((per + 31) / 32) * 32;
-    if (per < 256) per = 256;
+    per = ((per + kWgStage - 1) / kWgStage) * kWgStage;
+    if (per < 2 * kWgStage) per = 2 * kWgStage;
     const int nz = (int)((P + per - 1) / per);
     dim3 grid((Cout + 63) / 64, (9 * Cin + 63) / 64, nz);
     hipLaunchKernelGGL(k_conv_bwd_weight, grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
