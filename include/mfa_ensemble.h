@@ -40,7 +40,9 @@ int mfa_ensemble_set_active(mfa_ensemble* e, const int* active);
 /* Number of persistent blocks (0 = one block per tile). */
 int mfa_ensemble_set_grid(mfa_ensemble* e, int blocks);
 /* f = feq(rho, u) of the given per-cell fields (NULL = rest state rho=1,u=0);
- * solid cells get the rest equilibrium. */
+ * solid cells get the rest equilibrium.  With masks it also records which
+ * tiles lie next to a solid (the other tiles skip every mask read from then
+ * on), which synchronises the stream; the masks must not change afterwards. */
 int mfa_ensemble_init(mfa_ensemble* e, void* f, const unsigned char* masks, const double* rho,
                       const double* vel, void* stream);
 /* Warm start: fa = feq(rho, u) as mfa_ensemble_init, then one step fa -> fb,
