@@ -1,13 +1,17 @@
 #!/usr/bin/env python3
 """Headline benchmark: D3Q19 BGK periodic box 256^3, FP64, tau=0.8, Guo body
 force F=(1e-6,0,0), rho=1 and seeded uniform noise u in [-1e-3,1e-3], one
-lattice per GPU.  A step is one mfa_step launch (pull + moments + BGK + Guo,
-periodic wrap inline) over the whole 256^3 box with inputs resident in HBM.
+lattice per GPU, inputs resident in HBM.  A step is one time step of the
+whole box (pull + moments + BGK + Guo, periodic wrap inline).  By default the
+steps run in pairs through mfa_step2 (two steps per pass, the intermediate
+state kept in LDS; bit-identical to two mfa_step calls); --kernel single runs
+one mfa_step launch per step.
 
-Prints one JSON line with MLUPS, the roofline of the step kernel (HBM bytes
-per lattice update = 2*19*8 = 304, one read and one write of every
-population) and the CPU baseline (the oracle, timed on this host's cores on a
-bounded number of steps of the same workload).
+Prints one JSON line with MLUPS, the roofline of the step kernel (algorithmic
+bytes per lattice update = 2*19*8 = 304, one read and one write of every
+population per step), the HBM traffic the counters measured for it, and the
+CPU baseline (the oracle, timed on this host's cores on a bounded number of
+steps of the same workload).
 """
 from __future__ import annotations
 
@@ -36,6 +40,8 @@ def parse():
     p.add_argument("--no-copy-probe", action="store_true",
                    help="skip the streaming-copy ceiling measurement (same bytes as one step)")
     p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--kernel", default="pair", choices=["pair", "single"],
+                   help="pair: two steps per launch (mfa_step2); single: one (mfa_step)")
     p.add_argument("--layout", default="row", choices=["row", "component"],
                    help="population order in HBM: row-interleaved (the engine's own "
                         "allocation) or the MultiFab's component-major planes")
@@ -70,8 +76,11 @@ def cpu_baseline(n, steps_hint, seconds):
                       f"(= {mlups * 304 / 1e3:.1f} GB/s algorithmic)"}
 
 
-def traffic_from_profiles(n):
-    path = os.path.join(ROOT, "profiles", f"step_d3q19_{n}_f64_traffic.json")
+def traffic_from_profiles(n, kernel):
+    """HBM bytes per launch of the timed kernel, from the committed counter
+    summary of the same workload (FETCH_SIZE + WRITE_SIZE, calibrated)."""
+    name = f"step2_d3q19_{n}_f64_traffic.json" if kernel == "pair" else f"step_d3q19_{n}_f64_traffic.json"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     try:
@@ -121,19 +130,31 @@ def main():
     mass0 = out[0].item()
     jx0 = out[1].item()
 
-    cur, nxt = fa, fb
-    for _ in range(args.warmup):
-        abi.step(cur, nxt, g, omega, F, stream)
-        cur, nxt = nxt, cur
+    pair = args.kernel == "pair"
+
+    def advance(k, cur, nxt):
+        """k time steps; pairs through mfa_step2, an odd last one through mfa_step."""
+        if pair:
+            for _ in range(k // 2):
+                abi.step2(cur, nxt, g, omega, F, stream)
+                cur, nxt = nxt, cur
+            if k % 2:
+                abi.step(cur, nxt, g, omega, F, stream)
+                cur, nxt = nxt, cur
+        else:
+            for _ in range(k):
+                abi.step(cur, nxt, g, omega, F, stream)
+                cur, nxt = nxt, cur
+        return cur, nxt
+
+    cur, nxt = advance(args.warmup, fa, fb)
 
     torch.cuda.synchronize()
     t_begin = time.perf_counter()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
-    for _ in range(args.steps):
-        abi.step(cur, nxt, g, omega, F, stream)
-        cur, nxt = nxt, cur
+    cur, nxt = advance(args.steps, cur, nxt)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_begin
@@ -150,7 +171,8 @@ def main():
     mlups = N3 / (ms_per_step * 1e-3) / 1e6
     bytes_per_lu = 2 * 19 * 8
     achieved = N3 * bytes_per_lu / (ms_per_step * 1e-3) / 1e9
-    traffic = traffic_from_profiles(n)
+    steps_per_launch = 2 if pair else 1
+    traffic = traffic_from_profiles(n, args.kernel)
 
     copy = None
     if not args.no_copy_probe:
@@ -180,7 +202,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (rho=1, u seeded uniform noise in [-1e-3,1e-3])",
         "config": {"workload": f"D3Q19 BGK periodic {n}^3, FP64 populations, tau=0.8, "
-                               f"Guo force (1e-6,0,0), {args.steps} timed steps",
+                               f"Guo force (1e-6,0,0), {args.steps} timed steps"
+                               + (", two steps per launch" if pair else ", one step per launch"),
                    "lattice": "D3Q19", "cells": [n, n, n], "tau": tau, "force": F,
                    "storage": ("SoA rows, [z][y][q][x] (the Q rows of one (y,z) side by side)"
                                if args.layout == "row" else "SoA planes, [q][z][y][x] (MultiFab order)")
@@ -189,13 +212,20 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "k_step<D3Q19,forced,fused>",
-                     "algorithmic_bytes_per_launch": N3 * bytes_per_lu},
+                     "kernel": "k_step2<D3Q19,forced>" if pair else "k_step<D3Q19,forced,fused>",
+                     "steps_per_launch": steps_per_launch,
+                     "algorithmic_bytes_per_launch": steps_per_launch * N3 * bytes_per_lu,
+                     # the HBM rate the counters imply: traffic per launch over launch time
+                     "hbm_GBs_measured_traffic": (round(traffic / (steps_per_launch * ms_per_step
+                                                                   * 1e-3) / 1e9, 1)
+                                                  if traffic else None)},
         "mass_rel_drift": abs(mass1 - mass0) / mass0,
         "momentum_growth_over_N3F": jx_growth,
         "copy_ceiling_GBs": copy,
         "frac_of_copy_ceiling": round(achieved / copy["nt"], 4) if copy else None,
-        "launch": abi.get_launch(lattice),
+        "launch": "mfa_step2: 16x16 columns, 64 z-planes per block, LDS ring of 3 planes, "
+                  "XCD-contiguous tile order"
+                  if pair else abi.get_launch(lattice),
         "wall_s_timed": round(wall, 4),
         "cpu_baseline": cpu,
     }

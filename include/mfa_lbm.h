@@ -116,6 +116,18 @@ int mfa_resolve_bc(int lattice, const int face_type[6], const double face_rho[6]
 int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g,
              double omega, const double* force, void* stream);
 
+/* Two steps in one pass: fout = the state after mfa_step(fin -> tmp) and
+ * mfa_step(tmp -> fout), bit for bit, without tmp.  The intermediate state
+ * stays in LDS (temporal blocking), so a pair of steps reads fin about 1.3
+ * times and writes fout once instead of reading and writing twice.  For
+ * callers that advance two steps with nothing in between, e.g. two
+ * SimulationImpl::advance() calls (src/Simulation/SimulationSetup.cpp:119-139)
+ * between outputs.  Fully periodic 3-D boxes (D3Q15, D3Q19, D3Q27) in FP64;
+ * anything else returns MFA_ERR_UNSUPPORTED and the caller uses mfa_step.
+ * No counterpart in the reference. */
+int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g,
+              double omega, const double* force, void* stream);
+
 /* Set (cfg != NULL) or reset to the built-in default (cfg == NULL) the
  * launch configuration mfa_step uses for one lattice; query it. */
 int mfa_set_launch(int lattice, const mfa_launch* cfg);

@@ -20,7 +20,7 @@ BC = {"MovingWall": 0, "NoSlip": 1, "Inflow": 2, "Outflow": 3, "PressureOutlet":
 DTYPE = {"f64": 0, "f32": 1, "f16dev": 2, "bf16dev": 3}
 RED = {"mass": 0, "momentum": 1, "comp_sum": 2}
 
-EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_pull_collide",
+EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_step2", "mfa_pull_collide",
            "mfa_fill_periodic", "mfa_apply_bc", "mfa_moments", "mfa_init_equilibrium",
            "mfa_reduce", "mfa_residual", "mfa_set_launch", "mfa_get_launch", "mfa_set_launch_dev",
            "mfa_get_launch_dev", "mfa_stream_copy",
@@ -86,6 +86,7 @@ def lib():
         L.mfa_last_error.restype = ctypes.c_char_p
         L.mfa_resolve_bc.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp, dp, G]
         L.mfa_step.argtypes = [F, F, G, ctypes.c_double, dp, vp]
+        L.mfa_step2.argtypes = [F, F, G, ctypes.c_double, dp, vp]
         L.mfa_pull_collide.argtypes = [F, F, ctypes.c_int, ctypes.c_double, dp, vp]
         L.mfa_fill_periodic.argtypes = [F, G, vp]
         L.mfa_apply_bc.argtypes = [F, G, vp]
@@ -183,6 +184,12 @@ def row_shape(Q: int, n, ngrow: int, lattice_dim: int):
 def step(fin: Field, fout: Field, g: GeomBC, omega: float, force=None, stream=None):
     check(lib().mfa_step(ctypes.byref(fin), ctypes.byref(fout), ctypes.byref(g), omega,
                          _dbl(force), stream_ptr(stream)))
+
+
+def step2(fin: Field, fout: Field, g: GeomBC, omega: float, force=None, stream=None):
+    """Two steps fin -> fout in one pass (fully periodic 3-D FP64 boxes)."""
+    check(lib().mfa_step2(ctypes.byref(fin), ctypes.byref(fout), ctypes.byref(g), omega,
+                          _dbl(force), stream_ptr(stream)))
 
 
 def pull_collide(fin: Field, fout: Field, lattice: int, omega: float, force=None, stream=None):

@@ -231,12 +231,13 @@ __device__ __forceinline__ void moments(const double (&f)[L::Q], const double F[
 }
 
 
-// BGK (+Guo) collision of pulled populations and store:
-// src/LBM/Collision.cpp:24-53 (plain) and :62-92 (forced).
-template <class L, bool FORCED>
-__device__ __forceinline__ void collide_store(const double (&f)[L::Q], double* __restrict__ fout,
-                                              long long self, const View& vo, const double omega,
-                                              const double F0, const double F1, const double F2)
+// BGK (+Guo) collision of pulled populations: src/LBM/Collision.cpp:24-53
+// (plain) and :62-92 (forced).  `emit(q, value)` receives each collided
+// population, so the step kernels store them to HBM and the two-step kernel
+// to LDS through one copy of the arithmetic.
+template <class L, bool FORCED, class Emit>
+__device__ __forceinline__ void collide(const double (&f)[L::Q], const double omega, const double F0,
+                                        const double F1, const double F2, Emit&& emit)
 {
     const double F[3] = {F0, F1, F2};
     double rho, u0, u1, u2;
@@ -253,6 +254,18 @@ __device__ __forceinline__ void collide_store(const double (&f)[L::Q], double* _
         const double fe = feq<L, q>(rho, u0, u1, u2, usq);
         double out = omw * f[q] + omega * fe;
         if constexpr (FORCED) out = out + guo<L, q>(u0, u1, u2, F0, F1, F2, pref, uF);
+        emit(qc, out);
+    });
+}
+
+// ... and store to fout with non-temporal stores.
+template <class L, bool FORCED>
+__device__ __forceinline__ void collide_store(const double (&f)[L::Q], double* __restrict__ fout,
+                                              long long self, const View& vo, const double omega,
+                                              const double F0, const double F1, const double F2)
+{
+    collide<L, FORCED>(f, omega, F0, F1, F2, [&](auto qc, double out) {
+        constexpr int q = decltype(qc)::value;
         __builtin_nontemporal_store(out, &fout[q * vo.sq + self]);
     });
 }

@@ -284,10 +284,15 @@ void SimulationImpl::advanceSteps(int n)
             hip_check(hipStreamBeginCapture(m_stream, hipStreamCaptureModeThreadLocal), "capture");
             DeviceField* a = m_fin;
             DeviceField* b = m_fout;
+            const double* F = m_has_forcing ? m_force.data() : nullptr;
+            // fully periodic 3-D boxes take the steps in pairs (mfa_step2,
+            // bit-identical to two mfa_step calls)
+            const bool pairs = m_geom.dim == 3 && m_geom.periodic[0] && m_geom.periodic[1] &&
+                               m_geom.periodic[2];
             int rc = MFA_OK;
-            for (int s = 0; s < kGraphSteps && rc == MFA_OK; ++s) {
-                rc = mfa_step(a->view(), b->view(), &m_bc, m_omega,
-                              m_has_forcing ? m_force.data() : nullptr, m_stream);
+            for (int s = 0; s < kGraphSteps && rc == MFA_OK; s += pairs ? 2 : 1) {
+                rc = pairs ? mfa_step2(a->view(), b->view(), &m_bc, m_omega, F, m_stream)
+                           : mfa_step(a->view(), b->view(), &m_bc, m_omega, F, m_stream);
                 std::swap(a, b);
             }
             const hipError_t ce = hipStreamEndCapture(m_stream, &graph);

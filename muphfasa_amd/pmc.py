@@ -36,11 +36,11 @@ def _pick(d, needle):
 
 
 def summarize(fetch_csv, write_csv, n, Q=19, elem=8, step="k_step", copy="k_copy",
-              copy_bytes=None, commands=()):
+              copy_bytes=None, commands=(), steps_per_launch=1, workload=None):
     fe = per_kernel(fetch_csv, "FETCH_SIZE")
     wr = per_kernel(write_csv, "WRITE_SIZE")
     ks, kc = _pick(fe, step), _pick(fe, copy)
-    algo = 2 * Q * elem * n ** 3
+    algo = steps_per_launch * 2 * Q * elem * n ** 3
     cb = copy_bytes if copy_bytes is not None else Q * elem * n ** 3
     mean = lambda v: sum(v) / len(v)
     # factor = known bytes / counter bytes for the copy kernel
@@ -49,7 +49,8 @@ def summarize(fetch_csv, write_csv, n, Q=19, elem=8, step="k_step", copy="k_copy
     rd = mean(fe[ks]) * 1024 * ff
     wt = mean(wr[_pick(wr, step)]) * 1024 * wf
     return {
-        "kernel": ks, "workload": f"D3Q19 {n}^3 FP64 forced periodic (bench.py)",
+        "kernel": ks, "workload": workload or f"D3Q19 {n}^3 FP64 forced periodic (bench.py)",
+        "steps_per_launch": steps_per_launch,
         "fetch_size_kib_per_dispatch": mean(fe[ks]),
         "write_size_kib_per_dispatch": mean(wr[_pick(wr, step)]),
         "dispatches": len(fe[ks]),
@@ -87,11 +88,15 @@ def main():
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--out", default="")
     p.add_argument("--command", action="append", default=[])
+    p.add_argument("--step", default="k_step", help="name fragment of the measured kernel")
+    p.add_argument("--steps-per-launch", type=int, default=1)
+    p.add_argument("--workload", default=None)
     a = p.parse_args()
     if a.stats:
         print(stats_table(a.stats, a.top))
         return
-    r = summarize(a.fetch_csv, a.write_csv, a.n, commands=a.command)
+    r = summarize(a.fetch_csv, a.write_csv, a.n, step=a.step, commands=a.command,
+                  steps_per_launch=a.steps_per_launch, workload=a.workload)
     s = json.dumps(r, indent=1)
     if a.out:
         with open(a.out, "w") as fh:

@@ -133,3 +133,31 @@ def test_periodic_forced_d3q19_256cubed_invariants(dev):
         final[layout] = ta if layout == "component" else ta.permute(2, 0, 1, 3)
         del tb
     assert torch.equal(final["component"], final["row"])
+
+
+def test_step2_equals_two_steps_at_256cubed(dev):
+    """The bench workload: one mfa_step2 pass equals two mfa_step calls bit
+    for bit at the full size, in the row-interleaved order."""
+    import torch
+    N = 256
+    n = [N, N, N]
+    F = [1e-6, 0.0, 0.0]
+    omega = 1.0 / 0.8
+    g = abi.resolve_bc(2, [abi.BC["Periodic"]] * 6)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(11)
+    rho = torch.ones((1, N, N, N), dtype=torch.float64, device=dev)
+    vel = (torch.rand((3, N, N, N), dtype=torch.float64, device=dev, generator=gen) * 2 - 1) * 1e-3
+    shape = abi.row_shape(19, n, 0, 3)
+    ta = torch.empty(shape, dtype=torch.float64, device=dev)
+    fa = abi.field(ta, n, 0, 3, layout="row")
+    abi.init_equilibrium(abi.field(rho, n, 0, 3), abi.field(vel, n, 0, 3), fa, 2, F)
+    del rho, vel
+    tb = torch.empty_like(ta)
+    tc = torch.empty_like(ta)
+    fb, fc = abi.field(tb, n, 0, 3, layout="row"), abi.field(tc, n, 0, 3, layout="row")
+    abi.step(fa, fb, g, omega, F)
+    abi.step(fb, fc, g, omega, F)
+    abi.step2(fa, fb, g, omega, F)
+    torch.cuda.synchronize()
+    assert torch.equal(tb, tc)

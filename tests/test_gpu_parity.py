@@ -254,3 +254,53 @@ def test_row_interleaved_layout_matches_oracle(dev, lat, bc):
     torch.cuda.synchronize()
     assert np.array_equal(to_host(r)[0], rho_e)
     assert np.array_equal(to_host(v), vel_e)
+
+
+STEP2_SIZES = [[13, 7, 6], [64, 3, 2], [40, 33, 70], [1, 5, 3]]
+
+
+@pytest.mark.parametrize("lat", ["D3Q15", "D3Q19", "D3Q27"])
+@pytest.mark.parametrize("si", range(len(STEP2_SIZES)))
+@pytest.mark.parametrize("forced", [False, True])
+@pytest.mark.parametrize("layout", ["component", "row"])
+def test_step2_equals_two_oracle_steps(dev, lat, si, forced, layout):
+    """mfa_step2 (two steps per pass, intermediate state in LDS) gives the
+    oracle's bits after two steps: boxes smaller than its tile, not a
+    multiple of it, and more z-planes than one block marches (70 > 64)."""
+    import torch
+    L = O.Lattice(lat)
+    n = STEP2_SIZES[si]
+    print(f"case step2 lattice={lat} n={n} forced={forced} layout={layout}", flush=True)
+    g, ref, per = bc_tables(L, "periodic")
+    F = [2e-5, -1e-5, 3e-6] if forced else None
+    omega = 1.0 / 0.8
+    f0 = random_state(L, n, seed=zlib.crc32(f"step2/{lat}/{si}".encode()))
+    _, mid = _oracle_step(L, f0, n, per, ref, omega, F)
+    _, expect = _oracle_step(L, mid, n, per, ref, omega, F)
+    if layout == "row":
+        tin = to_dev(_to_rows(O.valid(L, f0)), dev)
+    else:
+        tin = to_dev(np.ascontiguousarray(O.valid(L, f0)), dev)
+    tout = torch.full_like(tin, float("nan"))
+    abi.step2(abi.field(tin, n, 0, 3, layout=layout), abi.field(tout, n, 0, 3, layout=layout),
+              g, omega, F)
+    torch.cuda.synchronize()
+    got = _from_rows(to_host(tout)) if layout == "row" else to_host(tout)
+    assert np.array_equal(got, O.valid(L, expect))
+
+
+def test_step2_refuses_walls_and_2d(dev):
+    import torch
+    L = O.Lattice("D3Q19")
+    g, _, _ = bc_tables(L, "lid")
+    a = torch.zeros((19, 4, 4, 4), dtype=torch.float64, device=dev)
+    b = torch.zeros_like(a)
+    with pytest.raises(abi.MfaError) as e:
+        abi.step2(abi.field(a, [4, 4, 4], 0, 3), abi.field(b, [4, 4, 4], 0, 3), g, 1.0)
+    assert e.value.code == -2
+    L2 = O.Lattice("D2Q9")
+    g2, _, _ = bc_tables(L2, "periodic")
+    a2 = torch.zeros((9, 1, 4, 4), dtype=torch.float64, device=dev)
+    with pytest.raises(abi.MfaError) as e:
+        abi.step2(abi.field(a2, [4, 4], 0, 2), abi.field(a2.clone(), [4, 4], 0, 2), g2, 1.0)
+    assert e.value.code == -2
