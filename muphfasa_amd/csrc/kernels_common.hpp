@@ -15,6 +15,10 @@ namespace mfa {
 
 int fail(int code, const std::string& msg);
 
+// Accelerator complex dies of an MI355X.  The hardware deals consecutive
+// workgroups of a launch round-robin to them; each has its own L2.
+constexpr unsigned kXcds = 8;
+
 #define MFA_HIP_CHECK(expr)                                                         \
     do {                                                                            \
         hipError_t e_ = (expr);                                                     \

@@ -315,8 +315,6 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double* __re
 // cell the arithmetic is that of k_step (pull, collide), so two-step results
 // equal two single steps bit for bit.  The next plane's pulls are issued
 // before the current plane's second step, so they are in flight meanwhile.
-constexpr unsigned kXcds = 8;               // accelerator complex dies of an MI355X
-
 template <class L>
 struct Step2Tile {                          // LDS: 3 planes x Q x (TX+2)(TY+2) doubles
     static constexpr int TX = 16, TY = TX;
@@ -419,6 +417,8 @@ int init_dev(mfa_field* rho, mfa_field* vel, mfa_field* f, const Geo& g, int lat
 int convert_dev(const mfa_field* src, mfa_field* dst, const Geo& g, int lat, hipStream_t st);
 int reduce_dev(const mfa_field* f, const Geo& g, int lat, int what, double* part, int nblocks,
                hipStream_t st);
+int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, double omega,
+              const double* force, int TZ, hipStream_t st);
 
 // -------------------------------------------------------------- helpers ---
 static int lattice_dim(int lat) { return lat == MFA_D2Q9 ? 2 : 3; }
@@ -728,22 +728,27 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
     const int Q = lattice_q(lat);
     if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_step2: unknown lattice");
     int rc;
-    if ((rc = check_field(fin, Q, 0, "mfa_step2 fin")) || (rc = check_field(fout, Q, 0, "mfa_step2 fout")))
+    if ((rc = check_field(fin, Q, 0, "mfa_step2 fin", true)) ||
+        (rc = check_field(fout, Q, 0, "mfa_step2 fout", true)))
         return rc;
     if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
         return fail(MFA_ERR_INVALID, "mfa_step2: fin/fout boxes differ");
     if (fin->data == fout->data) return fail(MFA_ERR_INVALID, "mfa_step2: fin aliases fout");
-    // D3Q27's 27 populations leave LDS room for 12x12 columns only, and the
-    // pass then runs slower than two single steps (384^3: 5.91 against 4.25
-    // ms per step), so it is not offered.
-    if (lat != MFA_D3Q15 && lat != MFA_D3Q19)
-        return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: D3Q15 and D3Q19 only (use mfa_step)");
+    if (fin->dtype != fout->dtype) return fail(MFA_ERR_INVALID, "mfa_step2: fin/fout storage types differ");
+    if (lattice_dim(lat) != 3) return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: 3-D lattices only (use mfa_step)");
     const Geo geo = geo_of(*fin, g, lat);
     if (geo.walls)
         return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: fully periodic boxes only (use mfa_step)");
+    const int TZ = geo.n[2] < kStep2Planes ? geo.n[2] : kStep2Planes;
+    if (fin->dtype != MFA_F64)
+        return step2_dev(fin, fout, geo, lat, omega, force, TZ, (hipStream_t)stream);
+    // In FP64, D3Q27's 27 populations leave LDS room for 12x12 columns only,
+    // and the pass then runs slower than two single steps (384^3: 5.91
+    // against 4.25 ms per step), so it is not offered.
+    if (lat != MFA_D3Q15 && lat != MFA_D3Q19)
+        return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: FP64 D3Q15 and D3Q19 only (use mfa_step)");
     const View vi = view_of(*fin), vo = view_of(*fout);
     const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0, F2 = force ? force[2] : 0.0;
-    const int TZ = geo.n[2] < kStep2Planes ? geo.n[2] : kStep2Planes;
     auto run = [&](auto Lc) {
         using L = decltype(Lc);
         using T = Step2Tile<L>;

@@ -16,11 +16,15 @@
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "kernels_lowprec.hpp"
 
 namespace mfa {
 
+// One cell per thread.  blockDim.x is a multiple of 64 (checked by the
+// host), so a wavefront lies in one row and y is the same on all its lanes:
+// the pull-source rows and the output row are then wave-uniform addresses.
 template <class L, bool FORCED, bool WALLS, class T>
 __global__ __launch_bounds__(256) void k_step_dev(const T* __restrict__ fin, T* __restrict__ fout,
                                                   const Geo g, const View vi, const View vo,
@@ -28,12 +32,12 @@ __global__ __launch_bounds__(256) void k_step_dev(const T* __restrict__ fin, T* 
                                                   const float F2)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int y = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
     const int z = blockIdx.z;
     if (x >= g.n[0] || y >= g.n[1]) return;
     float d[L::Q];
-    pull_dev<L, WALLS>(d, fin, x, y, z, g, vi);
-    collide_store_dev<L, FORCED>(d, fout, x + y * vo.sy + z * vo.sz, vo, omega, F0, F1, F2);
+    pull_dev_rows<L, WALLS>(d, fin, x, y, z, g, vi);
+    collide_store_dev_row<L, FORCED>(d, fout + y * vo.sy + z * vo.sz, x, vo, omega, F0, F1, F2);
 }
 
 // Storage word of T, and two of them loaded or stored as one access.
@@ -78,7 +82,8 @@ __global__ __launch_bounds__(256) void k_step_dev_x2(const T* __restrict__ fin, 
     using U2 = typename Word<T>::U2;
     const U* __restrict__ in = (const U*)fin;
     const int x0 = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    // a wavefront lies in one row (blockDim.x is a multiple of 64)
+    const int y = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
     const int z = blockIdx.z;
     const int nx = g.n[0];
     if (y >= g.n[1]) return;                 // whole wavefronts only
@@ -89,28 +94,31 @@ __global__ __launch_bounds__(256) void k_step_dev_x2(const T* __restrict__ fin, 
     const int xr = x0 + 2 >= nx ? (g.per[0] ? 0 : nx - 1) : x0 + 2;
     const bool edge_l = active && lane == 0;
     const bool edge_r = active && (lane == 63 || x0 + 2 >= nx);
-    const Src s = make_src<L::D, false>(x0, y, z, g, vi);
+    const unsigned b0 = (unsigned)x0 * (unsigned)sizeof(U), bl = (unsigned)xl * (unsigned)sizeof(U),
+                   br = (unsigned)xr * (unsigned)sizeof(U);
+    long long rows[3][3];
+    source_rows<L::D>(rows, y, z, g, vi);
     float da[L::Q], db[L::Q];
     for_q<L>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int cx = L::c(q, 0);
-        const long long row = q * vi.sq + s.o[1][L::c(q, 1) + 1] + s.o[2][L::c(q, 2) + 1];
+        constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::D == 3 ? L::c(q, 2) : 0;
+        const U* row = in + q * vi.sq + rows[cy + 1][cz + 1];
         U2 p = {0, 0};
         if (active) {
-            if constexpr (NT) p = __builtin_nontemporal_load((const U2*)&in[row + x0]);
-            else p = *(const U2*)&in[row + x0];
+            if constexpr (NT) p = __builtin_nontemporal_load(at_bytes((const U2*)row, b0));
+            else p = *at_bytes((const U2*)row, b0);
         }
         if constexpr (cx == 0) {
             da[q] = word_to_f<T>(p.x);
             db[q] = word_to_f<T>(p.y);
         } else if constexpr (cx == 1) {      // sources x0-1, x0
             U left = (U)__shfl_up((unsigned int)p.y, 1, 64);
-            if (edge_l) left = in[row + xl];
+            if (edge_l) left = *at_bytes(row, bl);
             da[q] = word_to_f<T>(left);
             db[q] = word_to_f<T>(p.x);
         } else {                             // sources x0+1, x0+2
             U right = (U)__shfl_down((unsigned int)p.x, 1, 64);
-            if (edge_r) right = in[row + xr];
+            if (edge_r) right = *at_bytes(row, br);
             da[q] = word_to_f<T>(p.y);
             db[q] = word_to_f<T>(right);
         }
@@ -123,13 +131,132 @@ __global__ __launch_bounds__(256) void k_step_dev_x2(const T* __restrict__ fin, 
     float oa[L::Q], ob[L::Q];
     collide_dev<L, FORCED>(da, oa, omega, F0, F1, F2);
     collide_dev<L, FORCED>(db, ob, omega, F0, F1, F2);
-    U* __restrict__ out = (U*)fout;
-    const long long self = x0 + y * vo.sy + z * vo.sz;
+    U* __restrict__ orow = (U*)fout + y * vo.sy + z * vo.sz;
     for_q<L>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const U2 w = {f_to_word<T>(oa[q]), f_to_word<T>(ob[q])};
-        __builtin_nontemporal_store(w, (U2*)&out[q * vo.sq + self]);
+        __builtin_nontemporal_store(w, at_bytes((U2*)(orow + q * vo.sq), b0));
     });
+}
+
+// Two steps per pass on reduced storage: the temporal blocking of k_step2
+// (mfa_lbm.hip) for stored deviations.  A block owns a TX x TY column of
+// output cells and marches through its z-range.  Phase A pulls and collides
+// the column grown by one cell in x and y one plane ahead; phase B pulls the
+// second step from a ring of LDS planes and writes it to HBM.  The
+// intermediate state is rounded to the storage type before it enters LDS,
+// exactly as a single step stores it, so the pair equals two k_step_dev
+// launches bit for bit, and it takes sizeof(T) bytes of LDS per population.
+// Fully periodic 3-D boxes.
+// RING: LDS planes of the ring.  With three, the plane phase A overwrites
+// was read by the previous plane's phase B, so a second barrier per plane
+// guards it; with four it was last read two planes back, behind the one
+// barrier every plane has.
+template <int TX_, int TY_, int RING_>
+struct Step2DevTile {
+    static constexpr int TX = TX_, TY = TY_, RING = RING_;
+    static constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
+    static constexpr int THREADS = (EC + 63) / 64 * 64;
+    static_assert(RING == 3 || RING == 4, "ring of 3 or 4 planes");
+};
+
+// The tile of a lattice and storage type: RING planes where they fit in the
+// 160 KiB of LDS of a CU, three otherwise.
+template <class L, class T, int TX, int TY, int RING>
+using Step2DevFit = std::conditional_t<(RING * L::Q * (TX + 2) * (TY + 2) * sizeof(T) <= 160 * 1024),
+                                       Step2DevTile<TX, TY, RING>, Step2DevTile<TX, TY, 3>>;
+
+template <class L, bool FORCED, class T, class TL>
+__global__ __launch_bounds__(TL::THREADS) void k_step2_dev(const T* __restrict__ fin, T* __restrict__ fout,
+                                                           const Geo g, const View vi, const View vo,
+                                                           const float omega, const float F0,
+                                                           const float F1, const float F2, const int TZ)
+{
+    using U = typename Word<T>::U;
+    __shared__ U lds[TL::RING][L::Q][TL::EC];
+    const int tid = threadIdx.x;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    // one contiguous run of tiles (x fastest) per XCD, as in k_step2
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned per = nb / kXcds, rem = nb % kXcds, k = b % kXcds;
+    const unsigned t = k * per + (k < rem ? k : rem) + b / kXcds;
+    const int bx = (int)(t % gridDim.x), by = (int)((t / gridDim.x) % gridDim.y);
+    const int bz = (int)(t / (gridDim.x * gridDim.y));
+    const int X0 = bx * TL::TX, Y0 = by * TL::TY, Z0 = bz * TZ;
+    const int Z1 = Z0 + TZ < nz ? Z0 + TZ : nz;
+    // phase A: one cell of the grown column (periodic images, any box size).
+    // Its pull sources in the plane are fixed for the whole march: byte
+    // offsets (x' + y' sy) * sizeof(T) for each (c_x, c_y), which the host
+    // has checked fit in 32 bits; the planes z - c_z are wave-uniform bases.
+    const bool inA = tid < TL::EC;
+    const int exA = tid % TL::EX, eyA = tid / TL::EX;
+    const int gx = ((X0 - 1 + exA) % nx + nx) % nx;
+    const int gy = ((Y0 - 1 + eyA) % ny + ny) % ny;
+    unsigned srcA[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int sx = (gx - (a - 1) + nx) % nx, sy = (gy - (b - 1) + ny) % ny;
+            srcA[a][b] = (unsigned)(sx + sy * vi.sy) * (unsigned)sizeof(T);
+        }
+    // phase B: one output cell
+    const int ox = tid % TL::TX, oy = tid / TL::TX;
+    const bool inB = tid < TL::TX * TL::TY && X0 + ox < nx && Y0 + oy < ny;
+    const unsigned dstB = inB ? (unsigned)((X0 + ox) + (Y0 + oy) * vo.sy) * (unsigned)sizeof(T) : 0u;
+    float P[L::Q];
+    auto slot = [&](int z) { return (z - Z0 + TL::RING) % TL::RING; };
+    auto loadA = [&](int z) {
+        if (!inA) return;
+        const T* plane[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) plane[c] = fin + (long long)(((z - (c - 1)) % nz + nz) % nz) * vi.sz;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
+            P[q] = to_f(*at_bytes(plane[cz + 1] + q * vi.sq, srcA[cx + 1][cy + 1]));
+        });
+    };
+    auto storeA = [&](int z) {
+        if (!inA) return;
+        const int sl = slot(z);
+        float o[L::Q];
+        collide_dev<L, FORCED>(P, o, omega, F0, F1, F2);
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            lds[sl][q][tid] = f_to_word<T>(o[q]);
+        });
+    };
+    loadA(Z0 - 1);
+    storeA(Z0 - 1);
+    loadA(Z0);
+    storeA(Z0);
+    loadA(Z0 + 1);
+    for (int z = Z0; z < Z1; ++z) {
+        storeA(z + 1);
+        if (z + 2 <= Z1) loadA(z + 2);
+        __syncthreads();
+        if (inB) {
+            float d[L::Q];
+            for_q<L>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
+                d[q] = word_to_f<T>(lds[slot(z - cz)][q][(ox + 1 - cx) + TL::EX * (oy + 1 - cy)]);
+            });
+            float o[L::Q];
+            collide_dev<L, FORCED>(d, o, omega, F0, F1, F2);
+            T* const orow = fout + (long long)z * vo.sz;
+            for_q<L>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const T sv = from_f<T>(o[q]);
+                T* p = at_bytes(orow + q * vo.sq, dstB);
+                if constexpr (sizeof(T) == 4) __builtin_nontemporal_store(sv, p);
+                else *p = sv;
+            });
+        }
+        if constexpr (TL::RING == 3) __syncthreads();
+    }
 }
 
 // Moments of reduced storage, evaluated in FP64 from the dequantised
@@ -386,6 +513,50 @@ int step_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doubl
             MFA_HIP_CHECK(hipGetLastError());
             return (int)MFA_OK;
         });
+    });
+}
+
+// Two steps in one pass (k_step2_dev); the caller has checked a fully
+// periodic 3-D box.  TZ: z-planes per block.  FP32 storage only: with FP16
+// and BF16 words the pass moves half the bytes but runs the same arithmetic,
+// which then bounds it, and it measured slower than two single steps on
+// MI355X (D3Q27 384^3 forced: 1.27 against 1.09 ms per step at its best tile).
+// Tiles (measured, D3Q27 384^3 / D3Q19 256^3 FP32 forced): D3Q27 32 x 8 with
+// a four-plane ring, 1.50 against 2.21 ms per step (147 KiB of LDS); D3Q15 and
+// D3Q19 64 x 4 with four planes, 0.330 against 0.462 ms.
+int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, double omega,
+              const double* force, int TZ, hipStream_t st)
+{
+    if (fin->dtype != MFA_F32)
+        return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: reduced storage in FP32 only (use mfa_step)");
+    const View vi{fin->sy, fin->sz, fin->sq}, vo{fout->sy, fout->sz, fout->sq};
+    // in-plane byte offsets are 32-bit in the kernel
+    for (const mfa_field* f : {fin, (const mfa_field*)fout})
+        if (((long long)(f->n[0] - 1) + (long long)(f->n[1] - 1) * f->sy + 1) * 4 > 0xffffffffLL)
+            return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: a z-plane spans more than 4 GiB (use mfa_step)");
+    return with_lattice_dev(lat, [&](auto Lc) {
+        using L = decltype(Lc);
+        if constexpr (L::D != 3) {
+            return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: 3-D lattices only");
+        } else {
+            using T = float;
+            using TL = std::conditional_t<L::Q == 27, Step2DevFit<L, T, 32, 8, 4>, Step2DevFit<L, T, 64, 4, 4>>;
+            const dim3 grid((g.n[0] + TL::TX - 1) / TL::TX, (g.n[1] + TL::TY - 1) / TL::TY,
+                            (g.n[2] + TZ - 1) / TZ);
+            const float om = (float)omega;
+            const float F0 = force ? (float)force[0] : 0.f, F1 = force ? (float)force[1] : 0.f,
+                        F2 = force ? (float)force[2] : 0.f;
+            const T* a = (const T*)fin->data;
+            T* b = (T*)fout->data;
+            if (force)
+                hipLaunchKernelGGL((k_step2_dev<L, true, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g,
+                                   vi, vo, om, F0, F1, F2, TZ);
+            else
+                hipLaunchKernelGGL((k_step2_dev<L, false, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g,
+                                   vi, vo, om, F0, F1, F2, TZ);
+            MFA_HIP_CHECK(hipGetLastError());
+            return (int)MFA_OK;
+        }
     });
 }
 

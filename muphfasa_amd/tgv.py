@@ -32,10 +32,23 @@ def tgv_fields(torch, N, u0, dev):
     return rho, vel
 
 
+def _pairs_offered(fa, fb, g, omega):
+    """Whether mfa_step2 (two steps per launch) takes this lattice and
+    storage; it returns MFA_ERR_UNSUPPORTED where it would run slower."""
+    try:
+        abi.step2(fa, fb, g, omega)
+        return True
+    except abi.MfaError as e:
+        if e.code != -2:
+            raise
+        return False
+
+
 def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32", "f16dev", "bf16dev"),
-        dev="cuda:0", layout="row", tune=False):
+        dev="cuda:0", layout="row", tune=False, pairs=True):
     """tune: time every launch candidate for each storage type on this box
-    first (muphfasa_amd.autotune) and run with the fastest."""
+    first (muphfasa_amd.autotune) and run with the fastest.  pairs: take the
+    steps two per launch (mfa_step2) where the library offers it."""
     import torch
     L = N / (2 * math.pi)
     nu = u0 * L / Re
@@ -64,6 +77,7 @@ def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32"
         fb = abi.field(b, n, 0, 3, abi.DTYPE[st], layout=layout)
         rho, vel = tgv_fields(torch, N, u0, dev)
         fr, fv = abi.field(rho, n, 0, 3), abi.field(vel, n, 0, 3)
+        use_pairs = pairs and _pairs_offered(fa, fb, g, omega)
         abi.init_equilibrium(fr, fv, fa, 3)
         out = torch.zeros(8, dtype=torch.float64, device=dev)
         abi.reduce(fa, 3, abi.RED["mass"], out.data_ptr())
@@ -76,7 +90,11 @@ def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32"
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            for _ in range(k):
+            if use_pairs:
+                for _ in range(k // 2):
+                    abi.step2(fa, fb, g, omega)
+                    fa, fb = fb, fa
+            for _ in range(k % 2 if use_pairs else k):
                 abi.step(fa, fb, g, omega)
                 fa, fb = fb, fa
             e1.record()
@@ -98,6 +116,7 @@ def run(N=384, u0=0.05, Re=800.0, steps=5000, sample=500, storages=("f64", "f32"
         mlups = N ** 3 * steps / t_kernel / 1e6
         launch = abi.get_launch(3) if st == "f64" else abi.get_launch_dev(3, abi.DTYPE[st])
         results[st] = {"mlups": round(mlups, 1), "bytes_per_lu": bytes_lu, "launch": list(launch),
+                       "steps_per_launch": 2 if use_pairs else 1,
                        "GBs": round(mlups * bytes_lu / 1e3, 1), "ke": ke,
                        "l2_u_err_vs_f64": errs, "mass_rel_drift": abs(m1 - m0) / m0,
                        "finite": bool(torch.isfinite(vel).all().item())}
@@ -123,9 +142,11 @@ def main():
     p.add_argument("--layout", default="row", choices=["row", "component"])
     p.add_argument("--storages", default="f64,f32,f16dev,bf16dev")
     p.add_argument("--tune", action="store_true", help="autotune the launch per storage first")
+    p.add_argument("--kernel", default="pair", choices=["pair", "single"],
+                   help="pair: two steps per launch where mfa_step2 is offered")
     a = p.parse_args()
     r = run(a.n, steps=a.steps, sample=a.sample, storages=tuple(a.storages.split(",")),
-            layout=a.layout, tune=a.tune)
+            layout=a.layout, tune=a.tune, pairs=a.kernel == "pair")
     if a.out:
         with open(a.out, "w") as fh:
             json.dump(r, fh)

@@ -2,7 +2,7 @@
 collision).
 
 * One step against a float32 reference of the deviation-form BGK+Guo update
-  written here in numpy (same inputs: the dequantised stored deviations),
+  restated in numpy in tests/lowprec_ref.py (same inputs: the dequantised stored deviations),
   to within a few units of the storage type's last place.
 * Round trip FP64 -> reduced -> FP64 within half an ulp of the storage type.
 * Multi-step error against the FP64 engine on a 3-D Taylor-Green vortex with
@@ -13,6 +13,7 @@ import pytest
 
 from muphfasa_amd import abi
 from oracle import oracle as O
+from tests.lowprec_ref import ref_step_f32 as _ref_step_f32
 
 pytestmark = pytest.mark.gpu
 
@@ -25,44 +26,6 @@ FLOOR = {"f32": 0.0, "f16dev": 2.0 ** -25, "bf16dev": 0.0}
 
 def _dev_tensor(torch, Q, n, dt, dev):
     return torch.empty((Q, n[2], n[1], n[0]), dtype=getattr(torch, TORCH_DT[dt]), device=dev)
-
-
-def _ref_step_f32(L, d, omega, F, periodic=True):
-    """float32 deviation-form pull + BGK (+Guo) on a periodic box."""
-    f32 = np.float32
-    Q, D = L.Q, L.D
-    pulled = np.empty_like(d)
-    for q in range(Q):
-        c = L.c[q]
-        pulled[q] = np.roll(d[q], shift=(c[2], c[1], c[0]), axis=(0, 1, 2))
-    drho = np.zeros(d.shape[1:], f32)
-    j = [np.zeros(d.shape[1:], f32) for _ in range(3)]
-    for q in range(Q):
-        drho = drho + pulled[q]
-        for a in range(D):
-            if L.c[q, a] > 0:
-                j[a] = j[a] + pulled[q]
-            elif L.c[q, a] < 0:
-                j[a] = j[a] - pulled[q]
-    rho = f32(1) + drho
-    inv = f32(1) / rho
-    Fv = [f32(F[a]) if (F is not None and a < len(F)) else f32(0) for a in range(3)]
-    u = [((j[a] + f32(0.5) * Fv[a]) if F is not None else j[a]) * inv for a in range(3)]
-    if D == 2:
-        u[2] = np.zeros_like(u[0])
-    usq = u[0] * u[0] + u[1] * u[1] + (u[2] * u[2] if D == 3 else 0)
-    om = f32(omega)
-    out = np.empty_like(d)
-    uF = u[0] * Fv[0] + u[1] * Fv[1] + u[2] * Fv[2]
-    for q in range(Q):
-        cu = sum(f32(L.c[q, a]) * u[a] for a in range(D))
-        fe = f32(L.w[q]) * (drho + rho * (f32(3) * cu + f32(4.5) * cu * cu - f32(1.5) * usq))
-        o = (f32(1) - om) * pulled[q] + om * fe
-        if F is not None:
-            cF = sum(f32(L.c[q, a]) * Fv[a] for a in range(D))
-            o = o + (f32(1) - f32(0.5) * om) * f32(L.w[q]) * f32(3) * (cF + f32(3) * cu * cF - uF)
-        out[q] = o
-    return out
 
 
 @pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
@@ -240,3 +203,69 @@ def test_row_interleaved_order_gives_the_same_bits(dev, lat, dt):
                                b.view(torch.int16 if dt != "f32" else torch.int32)), cfg
     finally:
         abi.set_launch_dev(lid, did)
+
+
+STEP2_SIZES = [[13, 7, 6], [40, 33, 70], [1, 5, 3], [64, 16, 64], [130, 9, 5]]
+
+
+@pytest.mark.parametrize("lat", ["D3Q15", "D3Q19", "D3Q27"])
+@pytest.mark.parametrize("dt", ["f32"])
+@pytest.mark.parametrize("si", range(len(STEP2_SIZES)))
+@pytest.mark.parametrize("forced", [False, True])
+def test_step2_equals_two_single_steps(dev, lat, dt, si, forced):
+    """FP32-storage mfa_step2 (intermediate state rounded to FP32 in LDS)
+    against two mfa_step calls, bit for bit: boxes smaller than its tiles
+    (64 x 4, and 32 x 8 for D3Q27), not a multiple of them, and more
+    z-planes than one block marches (70 > 64), in both population orders."""
+    import torch
+    L = O.Lattice(lat)
+    lid, did = abi.LATTICE[lat], abi.DTYPE[dt]
+    n = STEP2_SIZES[si]
+    print(f"case step2_dev lattice={lat} dtype={dt} n={n} forced={forced}", flush=True)
+    rng = np.random.default_rng(23 + si)
+    f64 = L.w[:, None, None, None] * (1 + 0.05 * rng.standard_normal((L.Q, n[2], n[1], n[0])))
+    src = torch.from_numpy(f64).to(dev)
+    a = _dev_tensor(torch, L.Q, n, dt, dev)
+    abi.convert(abi.field(src, n, 0, 3), abi.field(a, n, 0, 3, did), lid)
+    g = abi.resolve_bc(lid, [abi.BC["Periodic"]] * 6)
+    F = [2e-4, -1e-4, 5e-5] if forced else None
+    omega = 1.0 / 0.6
+    word = torch.int16 if dt != "f32" else torch.int32
+    mid, two = torch.empty_like(a), torch.empty_like(a)
+    abi.step(abi.field(a, n, 0, 3, did), abi.field(mid, n, 0, 3, did), g, omega, F)
+    abi.step(abi.field(mid, n, 0, 3, did), abi.field(two, n, 0, 3, did), g, omega, F)
+    pair = torch.full_like(a, float("nan"))
+    abi.step2(abi.field(a, n, 0, 3, did), abi.field(pair, n, 0, 3, did), g, omega, F)
+    ar = a.permute(1, 2, 0, 3).contiguous()            # (z, y, q, x)
+    pr = torch.full_like(ar, float("nan"))
+    abi.step2(abi.field(ar, n, 0, 3, did, layout="row"), abi.field(pr, n, 0, 3, did, layout="row"),
+              g, omega, F)
+    torch.cuda.synchronize()
+    assert torch.equal(pair.view(word), two.view(word))
+    assert torch.equal(pr.permute(2, 0, 1, 3).contiguous().view(word), two.view(word))
+
+
+def test_step2_reduced_refuses_2d_walls_and_16bit(dev):
+    """Where the pass is not offered (2-D, walls, FP16 / BF16 words, which
+    measured slower than two single steps) it returns MFA_ERR_UNSUPPORTED."""
+    import torch
+    from tests.gpu_util import bc_tables
+    g27 = abi.resolve_bc(3, [abi.BC["Periodic"]] * 6)
+    for dt in ("f16dev", "bf16dev"):
+        h = torch.zeros((27, 4, 4, 4), dtype=getattr(torch, TORCH_DT[dt]), device=dev)
+        with pytest.raises(abi.MfaError) as e:
+            abi.step2(abi.field(h, [4, 4, 4], 0, 3, abi.DTYPE[dt]),
+                      abi.field(h.clone(), [4, 4, 4], 0, 3, abi.DTYPE[dt]), g27, 1.0)
+        assert e.value.code == -2
+    a = torch.zeros((9, 1, 4, 4), dtype=torch.float16, device=dev)
+    g = abi.resolve_bc(0, [abi.BC["Periodic"]] * 6)
+    with pytest.raises(abi.MfaError) as e:
+        abi.step2(abi.field(a, [4, 4], 0, 2, abi.DTYPE["f16dev"]),
+                  abi.field(a.clone(), [4, 4], 0, 2, abi.DTYPE["f16dev"]), g, 1.0)
+    assert e.value.code == -2
+    g3, _, _ = bc_tables(O.Lattice("D3Q27"), "lid")
+    b = torch.zeros((27, 4, 4, 4), dtype=torch.float32, device=dev)
+    with pytest.raises(abi.MfaError) as e:
+        abi.step2(abi.field(b, [4, 4, 4], 0, 3, abi.DTYPE["f32"]),
+                  abi.field(b.clone(), [4, 4, 4], 0, 3, abi.DTYPE["f32"]), g3, 1.0)
+    assert e.value.code == -2
