@@ -223,7 +223,7 @@ def main():
         "momentum_growth_over_N3F": jx_growth,
         "copy_ceiling_GBs": copy,
         "frac_of_copy_ceiling": round(achieved / copy["nt"], 4) if copy else None,
-        "launch": "mfa_step2: 16x16 columns, 64 z-planes per block, LDS ring of 3 planes, "
+        "launch": "mfa_step2: 64x4 columns, 64 z-planes per block, per-c_z LDS rings, "
                   "XCD-contiguous tile order"
                   if pair else abi.get_launch(lattice),
         "wall_s_timed": round(wall, 4),

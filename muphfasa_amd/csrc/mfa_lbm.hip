@@ -307,29 +307,28 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double* __re
 // Two fused time steps per pass on a fully periodic 3-D box (temporal
 // blocking).  A block owns a TX x TY column of output cells and marches
 // through its z-range.  Phase A computes the intermediate state (the first
-// step) on the column grown by one cell in x and y, plane by plane, into a
-// ring of three LDS planes; phase B pulls the second step from those planes
-// and writes it to HBM.  The intermediate state never leaves the chip, so a
+// step) on the column grown by one cell in x and y, plane by plane, into
+// LDS rings (CzRings); phase B pulls the second step from those planes and
+// writes it to HBM.  The intermediate state never leaves the chip, so a
 // pair of steps reads the source lattice once (plus the one-cell rim, which
 // neighbouring blocks also read and L2 mostly serves) and writes once.  Per
 // cell the arithmetic is that of k_step (pull, collide), so two-step results
 // equal two single steps bit for bit.  The next plane's pulls are issued
 // before the current plane's second step, so they are in flight meanwhile.
-template <class L>
-struct Step2Tile {                          // LDS: 3 planes x Q x (TX+2)(TY+2) doubles
-    static constexpr int TX = 16, TY = TX;
+template <int TX_, int TY_>
+struct Step2Tile {                          // (TX+2) x (TY+2) grown column
+    static constexpr int TX = TX_, TY = TY_;
     static constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
     static constexpr int THREADS = (EC + 63) / 64 * 64;
 };
 
-template <class L, bool FORCED>
-__global__ __launch_bounds__(Step2Tile<L>::THREADS) void k_step2(
+template <class L, bool FORCED, class T>
+__global__ __launch_bounds__(T::THREADS) void k_step2(
     const double* __restrict__ fin, double* __restrict__ fout, const Geo g, const View vi,
     const View vo, const double omega, const double F0, const double F1, const double F2,
     const int TZ)
 {
-    using T = Step2Tile<L>;
-    __shared__ double lds[3][L::Q][T::EC];
+    __shared__ CzRings<L, double, T::EC> lds;
     const int tid = threadIdx.x;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     // The hardware deals consecutive blocks round-robin to the 8 XCDs.  Give
@@ -354,16 +353,14 @@ __global__ __launch_bounds__(Step2Tile<L>::THREADS) void k_step2(
     const int ox = tid % T::TX, oy = tid / T::TX;
     const bool inB = tid < T::TX * T::TY && X0 + ox < nx && Y0 + oy < ny;
     double P[L::Q];
-    auto slot = [&](int z) { return (z - Z0 + 3) % 3; };
     auto loadA = [&](int z) {
         if (inA) pull<L, 0, false, 0>(P, fin, gx, gy, ((z % nz) + nz) % nz, g, vi);
     };
     auto storeA = [&](int z) {
         if (!inA) return;
-        const int sl = slot(z);
         collide<L, FORCED>(P, omega, F0, F1, F2, [&](auto qc, double out) {
             constexpr int q = decltype(qc)::value;
-            lds[sl][q][tid] = out;
+            lds.template at<q>(z, Z0, tid) = out;
         });
     };
     loadA(Z0 - 1);
@@ -380,7 +377,7 @@ __global__ __launch_bounds__(Step2Tile<L>::THREADS) void k_step2(
             for_q<L>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
-                f[q] = lds[slot(z - cz)][q][(ox + 1 - cx) + T::EX * (oy + 1 - cy)];
+                f[q] = lds.template at<q>(z - cz, Z0, (ox + 1 - cx) + T::EX * (oy + 1 - cy));
             });
             collide_store<L, FORCED>(f, fout,
                                      (X0 + ox) + (long long)(Y0 + oy) * vo.sy + (long long)z * vo.sz,
@@ -742,30 +739,33 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
     const int TZ = geo.n[2] < kStep2Planes ? geo.n[2] : kStep2Planes;
     if (fin->dtype != MFA_F64)
         return step2_dev(fin, fout, geo, lat, omega, force, TZ, (hipStream_t)stream);
-    // In FP64, D3Q27's 27 populations leave LDS room for 12x12 columns only,
-    // and the pass then runs slower than two single steps (384^3: 5.91
-    // against 4.25 ms per step), so it is not offered.
-    if (lat != MFA_D3Q15 && lat != MFA_D3Q19)
-        return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: FP64 D3Q15 and D3Q19 only (use mfa_step)");
+
     const View vi = view_of(*fin), vo = view_of(*fout);
     const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0, F2 = force ? force[2] : 0.0;
     auto run = [&](auto Lc) {
         using L = decltype(Lc);
-        using T = Step2Tile<L>;
+        // Columns (measured, forced): D3Q27 384^3 32 x 8, 2.86 ms per step
+        // against 3.09 with 16 x 16 and 4.36 for two single steps; D3Q19
+        // 256^3 64 x 4, 0.583 ms against 0.586 with 32 x 8, 0.642 with
+        // 16 x 16 and 0.852 for two single steps.  Wider rows touch fewer
+        // cache lines per wavefront; the per-c_z rings leave LDS room for them.
+        using T = std::conditional_t<L::Q == 27, Step2Tile<32, 8>, Step2Tile<64, 4>>;
         const dim3 grid((geo.n[0] + T::TX - 1) / T::TX, (geo.n[1] + T::TY - 1) / T::TY,
                         (geo.n[2] + TZ - 1) / TZ);
         const double* a = (const double*)fin->data;
         double* b = (double*)fout->data;
         if (force)
-            hipLaunchKernelGGL((k_step2<L, true>), grid, dim3(T::THREADS), 0, (hipStream_t)stream, a, b,
+            hipLaunchKernelGGL((k_step2<L, true, T>), grid, dim3(T::THREADS), 0, (hipStream_t)stream, a, b,
                                geo, vi, vo, omega, F0, F1, F2, TZ);
         else
-            hipLaunchKernelGGL((k_step2<L, false>), grid, dim3(T::THREADS), 0, (hipStream_t)stream, a,
+            hipLaunchKernelGGL((k_step2<L, false, T>), grid, dim3(T::THREADS), 0, (hipStream_t)stream, a,
                                b, geo, vi, vo, omega, F0, F1, F2, TZ);
         MFA_HIP_CHECK(hipGetLastError());
         return (int)MFA_OK;
     };
-    return lat == MFA_D3Q15 ? run(Lattice<MFA_D3Q15>{}) : run(Lattice<MFA_D3Q19>{});
+    if (lat == MFA_D3Q15) return run(Lattice<MFA_D3Q15>{});
+    if (lat == MFA_D3Q19) return run(Lattice<MFA_D3Q19>{});
+    return run(Lattice<MFA_D3Q27>{});
 }
 
 int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, double omega,

@@ -143,28 +143,31 @@ __global__ __launch_bounds__(256) void k_step_dev_x2(const T* __restrict__ fin, 
 // (mfa_lbm.hip) for stored deviations.  A block owns a TX x TY column of
 // output cells and marches through its z-range.  Phase A pulls and collides
 // the column grown by one cell in x and y one plane ahead; phase B pulls the
-// second step from a ring of LDS planes and writes it to HBM.  The
+// second step from LDS planes (Step2DevStore) and writes it to HBM.  The
 // intermediate state is rounded to the storage type before it enters LDS,
 // exactly as a single step stores it, so the pair equals two k_step_dev
 // launches bit for bit, and it takes sizeof(T) bytes of LDS per population.
 // Fully periodic 3-D boxes.
-// RING: LDS planes of the ring.  With three, the plane phase A overwrites
-// was read by the previous plane's phase B, so a second barrier per plane
-// guards it; with four it was last read two planes back, behind the one
-// barrier every plane has.
-template <int TX_, int TY_, int RING_>
+// MODE: how LDS holds the intermediate planes (kernels_common.hpp): 0 the
+// per-c_z rings (CzRings), 3 or 4 a ring of whole planes (PlaneRing).
+template <int TX_, int TY_, int MODE_>
 struct Step2DevTile {
-    static constexpr int TX = TX_, TY = TY_, RING = RING_;
+    static constexpr int TX = TX_, TY = TY_, MODE = MODE_;
     static constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
     static constexpr int THREADS = (EC + 63) / 64 * 64;
-    static_assert(RING == 3 || RING == 4, "ring of 3 or 4 planes");
+    static_assert(MODE == 0 || MODE == 3 || MODE == 4, "plane storage mode");
 };
 
-// The tile of a lattice and storage type: RING planes where they fit in the
-// 160 KiB of LDS of a CU, three otherwise.
-template <class L, class T, int TX, int TY, int RING>
-using Step2DevFit = std::conditional_t<(RING * L::Q * (TX + 2) * (TY + 2) * sizeof(T) <= 160 * 1024),
-                                       Step2DevTile<TX, TY, RING>, Step2DevTile<TX, TY, 3>>;
+template <class L, class W, class TL>
+using Step2DevStore = std::conditional_t<TL::MODE == 0, CzRings<L, W, TL::EC>,
+                                         PlaneRing<L, W, TL::EC, (TL::MODE == 0 ? 3 : TL::MODE)>>;
+
+// The tile of a lattice and storage type: MODE where it fits in the
+// 160 KiB of LDS of a CU, the per-c_z rings otherwise.
+template <class L, class T, int TX, int TY, int MODE>
+using Step2DevFit =
+    std::conditional_t<sizeof(Step2DevStore<L, T, Step2DevTile<TX, TY, MODE>>) <= 160 * 1024,
+                       Step2DevTile<TX, TY, MODE>, Step2DevTile<TX, TY, 0>>;
 
 template <class L, bool FORCED, class T, class TL>
 __global__ __launch_bounds__(TL::THREADS) void k_step2_dev(const T* __restrict__ fin, T* __restrict__ fout,
@@ -173,7 +176,8 @@ __global__ __launch_bounds__(TL::THREADS) void k_step2_dev(const T* __restrict__
                                                            const float F1, const float F2, const int TZ)
 {
     using U = typename Word<T>::U;
-    __shared__ U lds[TL::RING][L::Q][TL::EC];
+    using Store = Step2DevStore<L, U, TL>;
+    __shared__ Store lds;
     const int tid = threadIdx.x;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     // one contiguous run of tiles (x fastest) per XCD, as in k_step2
@@ -206,7 +210,6 @@ __global__ __launch_bounds__(TL::THREADS) void k_step2_dev(const T* __restrict__
     const bool inB = tid < TL::TX * TL::TY && X0 + ox < nx && Y0 + oy < ny;
     const unsigned dstB = inB ? (unsigned)((X0 + ox) + (Y0 + oy) * vo.sy) * (unsigned)sizeof(T) : 0u;
     float P[L::Q];
-    auto slot = [&](int z) { return (z - Z0 + TL::RING) % TL::RING; };
     auto loadA = [&](int z) {
         if (!inA) return;
         const T* plane[3];
@@ -220,12 +223,11 @@ __global__ __launch_bounds__(TL::THREADS) void k_step2_dev(const T* __restrict__
     };
     auto storeA = [&](int z) {
         if (!inA) return;
-        const int sl = slot(z);
         float o[L::Q];
         collide_dev<L, FORCED>(P, o, omega, F0, F1, F2);
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            lds[sl][q][tid] = f_to_word<T>(o[q]);
+            lds.template at<q>(z, Z0, tid) = f_to_word<T>(o[q]);
         });
     };
     loadA(Z0 - 1);
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(TL::THREADS) void k_step2_dev(const T* __restrict__
             for_q<L>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
-                d[q] = word_to_f<T>(lds[slot(z - cz)][q][(ox + 1 - cx) + TL::EX * (oy + 1 - cy)]);
+                d[q] = word_to_f<T>(lds.template at<q>(z - cz, Z0, (ox + 1 - cx) + TL::EX * (oy + 1 - cy)));
             });
             float o[L::Q];
             collide_dev<L, FORCED>(d, o, omega, F0, F1, F2);
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(TL::THREADS) void k_step2_dev(const T* __restrict__
                 else *p = sv;
             });
         }
-        if constexpr (TL::RING == 3) __syncthreads();
+        if constexpr (!Store::ONE_BARRIER) __syncthreads();
     }
 }
 
@@ -522,8 +524,9 @@ int step_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doubl
 // which then bounds it, and it measured slower than two single steps on
 // MI355X (D3Q27 384^3 forced: 1.27 against 1.09 ms per step at its best tile).
 // Tiles (measured, D3Q27 384^3 / D3Q19 256^3 FP32 forced): D3Q27 32 x 8 with
-// a four-plane ring, 1.50 against 2.21 ms per step (147 KiB of LDS); D3Q15 and
-// D3Q19 64 x 4 with four planes, 0.330 against 0.462 ms.
+// a four-plane ring, 1.56 against 2.25 ms per step (147 KiB of LDS; the
+// per-c_z rings, 73 KiB, gave 1.75); D3Q15 and D3Q19 64 x 4 with the per-c_z
+// rings, 0.305 against 0.464 ms (a four-plane ring gave 0.331).
 int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, double omega,
               const double* force, int TZ, hipStream_t st)
 {
@@ -540,7 +543,7 @@ int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doub
             return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: 3-D lattices only");
         } else {
             using T = float;
-            using TL = std::conditional_t<L::Q == 27, Step2DevFit<L, T, 32, 8, 4>, Step2DevFit<L, T, 64, 4, 4>>;
+            using TL = std::conditional_t<L::Q == 27, Step2DevFit<L, T, 32, 8, 4>, Step2DevFit<L, T, 64, 4, 0>>;
             const dim3 grid((g.n[0] + TL::TX - 1) / TL::TX, (g.n[1] + TL::TY - 1) / TL::TY,
                             (g.n[2] + TZ - 1) / TZ);
             const float om = (float)omega;
