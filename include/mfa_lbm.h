@@ -162,10 +162,12 @@ int mfa_fill_periodic(mfa_field* f, const mfa_geom_bc* g, void* stream);
 int mfa_apply_bc(mfa_field* f, const mfa_geom_bc* g, void* stream);
 
 /* rho/u moments (updateMacroscopic, src/Simulation/SimulationSetup.cpp:141-166:
- * streamMF + computeMacroscopic).  streamed != 0: moments of the pulled
- * populations (inline wrap/boundary, like mfa_step); streamed == 0: of f at
- * the cell itself (computeMacroscopic on unstreamed f).  rho: 1 comp,
- * vel: D comps, both F64 with ngrow 0.  force may be NULL. */
+ * streamMF + computeMacroscopic).  streamed == 1: moments of the pulled
+ * populations (inline wrap/boundary, like mfa_step); streamed == 2: of the
+ * populations pulled through f's filled ghost cells (FP64, ngrow 1; for the
+ * boxes of an mfa_level); streamed == 0: of f at the cell itself
+ * (computeMacroscopic on unstreamed f).  rho: 1 comp, vel: D comps, both
+ * F64 with ngrow 0.  force may be NULL. */
 int mfa_moments(const mfa_field* f, mfa_field* rho, mfa_field* vel,
                 const mfa_geom_bc* g, const double* force, int streamed, void* stream);
 

@@ -1,0 +1,65 @@
+/* mfa_level.h -- several boxes of one lattice on one device: the ghost
+ * exchange between them (the role of amrex::MultiFab::FillBoundary in
+ * src/Simulation/SimulationSetup.cpp:227 when the domain is chopped into
+ * boxes of at most amr.max_grid_size cells) and the physical boundary fill
+ * of each box at its place in the domain (PhysBCFunct<BCManager>,
+ * SimulationSetup.cpp:230, src/LBM/BoundaryConditions.H:124-189).
+ *
+ * A level is built once on the host from the box layout, like the
+ * BoxArray and the copy metadata AMReX caches for FillBoundary.  The
+ * populations of its boxes (FP64, ngrow 1, any strides mfa_field accepts)
+ * are bound to numbered slots, typically the two buffers of the A/B pull.
+ * A step of the level is then
+ *     mfa_level_fill_boundary(lv, a, s); mfa_level_apply_bc(lv, a, g, s);
+ *     for each box b: mfa_pull_collide(box b of a, box b of b, ...);
+ * and gives the same valid-cell values, bit for bit, as mfa_step on one box
+ * holding the whole domain.  Moments of a box through its ghosts:
+ * mfa_moments(..., streamed = 2) after the two fills.
+ *
+ * Status codes and error messages as in mfa_lbm.h. */
+#ifndef MFA_LEVEL_H
+#define MFA_LEVEL_H
+
+#include "mfa_lbm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mfa_level mfa_level;
+
+/* Build a level: nboxes boxes, box i with valid cells [lo, lo + n) of the
+ * domain (box_lo, box_n: nboxes x 3 ints; the third entry 0 / 1 in 2-D).
+ * The boxes must tile the domain exactly (no overlap, no gap).  Plans the
+ * ghost copies of every box, periodic images included, and keeps them in
+ * device memory of the current device (allocated here; freed by
+ * mfa_level_destroy).  No counterpart in the reference (AMReX builds this
+ * inside FabArrayBase). */
+int mfa_level_create(int lattice, const int domain_n[3], const int periodic[3], int nboxes,
+                     const int* box_lo, const int* box_n, mfa_level** out);
+int mfa_level_destroy(mfa_level* lv);
+
+/* Number of ghost cells the level's fill_boundary writes (all boxes). */
+long long mfa_level_ghost_cells(const mfa_level* lv);
+
+/* Bind the fields of all boxes (boxes[i]: box i, FP64, ngrow >= 1, extents
+ * equal to the box's) to slot 0..3.  Copies their views to device memory,
+ * synchronously; rebind after reallocating. */
+int mfa_level_bind(mfa_level* lv, int slot, const mfa_field* boxes);
+
+/* FillBoundary(periodicity): every ghost cell of every box that lies in the
+ * periodic-grown domain gets the value (all components) of the valid cell
+ * it images, from whichever box owns that cell.  One launch. */
+int mfa_level_fill_boundary(const mfa_level* lv, int slot, void* stream);
+
+/* PhysBCFunct on every box: each ghost cell outside the periodic-grown
+ * domain gets, for every population whose source g + c_q is a cell of the
+ * box's grown region that lies in the domain, the wall value of its
+ * resolved boundary class.  Call after mfa_level_fill_boundary (sources
+ * may be ghost cells it filled).  One launch per box. */
+int mfa_level_apply_bc(const mfa_level* lv, int slot, const mfa_geom_bc* g, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFA_LEVEL_H */

@@ -24,7 +24,8 @@ EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_s
            "mfa_fill_periodic", "mfa_apply_bc", "mfa_moments", "mfa_init_equilibrium",
            "mfa_reduce", "mfa_residual", "mfa_set_launch", "mfa_get_launch", "mfa_set_launch_dev",
            "mfa_get_launch_dev", "mfa_stream_copy",
-           "mfa_convert"]
+           "mfa_convert", "mfa_level_create", "mfa_level_destroy", "mfa_level_bind",
+           "mfa_level_fill_boundary", "mfa_level_apply_bc", "mfa_level_ghost_cells"]
 
 
 class MfaError(RuntimeError):
@@ -100,8 +101,16 @@ def lib():
         L.mfa_get_launch_dev.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Launch)]
         L.mfa_stream_copy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
         L.mfa_convert.argtypes = [F, F, ctypes.c_int, vp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.mfa_level_create.argtypes = [ctypes.c_int, ip, ip, ctypes.c_int, ip, ip, ctypes.POINTER(vp)]
+        L.mfa_level_destroy.argtypes = [vp]
+        L.mfa_level_bind.argtypes = [vp, ctypes.c_int, F]
+        L.mfa_level_fill_boundary.argtypes = [vp, ctypes.c_int, vp]
+        L.mfa_level_apply_bc.argtypes = [vp, ctypes.c_int, G, vp]
+        L.mfa_level_ghost_cells.argtypes = [vp]
         for name in EXPORTS[2:]:
             getattr(L, name).restype = ctypes.c_int
+        L.mfa_level_ghost_cells.restype = ctypes.c_longlong
         _lib = L
     return _lib
 
@@ -209,9 +218,52 @@ def apply_bc(f: Field, g: GeomBC, stream=None):
 
 def moments(f: Field, rho: Field, vel: Field, g: GeomBC, force=None, streamed=True,
             stream=None):
+    """streamed: True (pull with inline wrap and walls), "halo" (pull through
+    f's filled ghost cells) or False (unstreamed)."""
+    mode = 2 if streamed == "halo" else (1 if streamed else 0)
     check(lib().mfa_moments(ctypes.byref(f), ctypes.byref(rho), ctypes.byref(vel),
-                            ctypes.byref(g), _dbl(force), 1 if streamed else 0,
-                            stream_ptr(stream)))
+                            ctypes.byref(g), _dbl(force), mode, stream_ptr(stream)))
+
+
+class Level:
+    """Boxes of one lattice on one device (include/mfa_level.h): the ghost
+    exchange between them and the physical boundary fill of each box."""
+
+    def __init__(self, lattice: int, domain_n, periodic, box_lo, box_n):
+        nb = len(box_lo)
+        lo = (ctypes.c_int * (3 * nb))(*[int(v) for b in box_lo for v in (list(b) + [0, 0])[:3]])
+        nn = (ctypes.c_int * (3 * nb))(*[int(v) for b in box_n for v in (list(b) + [1, 1])[:3]])
+        dn = (ctypes.c_int * 3)(*(list(domain_n) + [1, 1])[:3])
+        per = (ctypes.c_int * 3)(*(list(periodic) + [1, 1])[:3])
+        h = ctypes.c_void_p()
+        check(lib().mfa_level_create(lattice, dn, per, nb, lo, nn, ctypes.byref(h)))
+        self.h = h
+        self.nboxes = nb
+
+    def close(self):
+        if self.h:
+            lib().mfa_level_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ghost_cells(self) -> int:
+        return int(lib().mfa_level_ghost_cells(self.h))
+
+    def bind(self, slot: int, fields):
+        arr = (Field * len(fields))(*fields)
+        check(lib().mfa_level_bind(self.h, slot, arr))
+
+    def fill_boundary(self, slot: int, stream=None):
+        check(lib().mfa_level_fill_boundary(self.h, slot, stream_ptr(stream)))
+
+    def apply_bc(self, slot: int, g: GeomBC, stream=None):
+        check(lib().mfa_level_apply_bc(self.h, slot, ctypes.byref(g), stream_ptr(stream)))
 
 
 def init_equilibrium(rho: Field, vel: Field, f: Field, lattice: int, force=None, stream=None):

@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["mfa_lbm.hip", "mfa_lowprec.hip", "mfa_ensemble.hip", "mfa_surrogate.hip"]
+SOURCES = ["mfa_lbm.hip", "mfa_lowprec.hip", "mfa_ensemble.hip", "mfa_surrogate.hip", "mfa_level.hip"]
 
 
 def build(verbose: bool = False) -> str:

@@ -482,6 +482,21 @@ static Geo geo_of(const mfa_field& f, const mfa_geom_bc* gb, int lat)
     return g;
 }
 
+// The same checks and geometry for the multi-box level (mfa_level.hip).
+int field_check(const mfa_field* f, int ncomp_min, int need_grow, const char* what)
+{
+    return check_field(f, ncomp_min, need_grow, what);
+}
+
+Geo geo_for(const int n[3], const mfa_geom_bc* gb, int lat)
+{
+    mfa_field f{};
+    f.n[0] = n[0];
+    f.n[1] = n[1];
+    f.n[2] = n[2];
+    return geo_of(f, gb, lat);
+}
+
 static void launch_shape(const int n[3], dim3& grid, dim3& block, int grow)
 {
     const int nx = n[0] + 2 * grow;
@@ -913,6 +928,8 @@ int mfa_moments(const mfa_field* f, mfa_field* rho, mfa_field* vel, const mfa_ge
     if (std::memcmp(f->n, rho->n, sizeof(f->n)) || std::memcmp(f->n, vel->n, sizeof(f->n)))
         return fail(MFA_ERR_INVALID, "mfa_moments: boxes differ");
     if ((rc = check_2d(f, lat))) return rc;
+    if (streamed == 2 && (f->dtype != MFA_F64 || f->ngrow < 1))
+        return fail(MFA_ERR_INVALID, "mfa_moments: streamed = 2 pulls through ghost cells: FP64 with ngrow 1");
     const Geo geo = geo_of(*f, g, lat);
     if (f->dtype != MFA_F64)
         return moments_dev(f, rho, vel, geo, lat, force, streamed, (hipStream_t)stream);
@@ -928,7 +945,10 @@ int mfa_moments(const mfa_field* f, mfa_field* rho, mfa_field* vel, const mfa_ge
         const double* a = (const double*)f->data;
         double* r = (double*)rho->data;
         double* u = (double*)vel->data;
-        if (streamed) {
+        if (streamed == 2) {
+            if (force) hipLaunchKernelGGL((k_moments<LT, true, 1>), grid, block, 0, st, a, r, u, geo, vi, vr, vv, F0, F1, F2);
+            else hipLaunchKernelGGL((k_moments<LT, false, 1>), grid, block, 0, st, a, r, u, geo, vi, vr, vv, F0, F1, F2);
+        } else if (streamed) {
             if (force) hipLaunchKernelGGL((k_moments<LT, true, 0>), grid, block, 0, st, a, r, u, geo, vi, vr, vv, F0, F1, F2);
             else hipLaunchKernelGGL((k_moments<LT, false, 0>), grid, block, 0, st, a, r, u, geo, vi, vr, vv, F0, F1, F2);
         } else {

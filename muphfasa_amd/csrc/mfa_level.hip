@@ -1,0 +1,363 @@
+// mfa_level.hip -- several boxes of one lattice on one device: the ghost
+// exchange between them (FillBoundary with periodic images) and the
+// physical boundary fill of each box at its place in the domain.
+//
+// The exchange is planned once on the host as rectangular copy regions
+// ("tags"), like the copy metadata AMReX keeps for FillBoundary: for every
+// destination box b, every source box a and every periodic image s of the
+// domain, the cells of a's valid region shifted by s that fall in b's
+// one-cell-grown region.  Valid regions tile the domain, so these regions
+// are disjoint and lie in b's ghost layer.  One launch walks the ghost
+// cells of all tags (a binary search over the tags' prefix sums finds a
+// cell's tag) and copies every component.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels_common.hpp"
+#include "../../include/mfa_level.h"
+
+namespace mfa {
+
+int field_check(const mfa_field* f, int ncomp_min, int need_grow, const char* what);
+Geo geo_for(const int n[3], const mfa_geom_bc* gb, int lat);
+
+struct BoxView {                  // one box's populations, relative to valid cell (0,0,0)
+    double* data;
+    long long sy, sz, sq;
+};
+
+struct CopyTag {                  // a ghost region of box dst imaged from valid cells of box src
+    int dst, src;
+    int n[3];                     // extents of the region
+    int dlo[3];                   // its first cell in dst-local coordinates (a ghost, -1 or n)
+    int slo[3];                   // the same cell in src-local coordinates (a valid cell)
+};
+
+constexpr int kLevelSlots = 4;
+
+template <class L>
+__global__ __launch_bounds__(256) void k_level_fill(const CopyTag* __restrict__ tags,
+                                                    const long long* __restrict__ start, const int ntags,
+                                                    const long long total,
+                                                    const BoxView* __restrict__ views)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += stride) {
+        int lo = 0, hi = ntags - 1;           // last tag whose start is <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const CopyTag t = tags[lo];
+        long long r = i - start[lo];
+        const int rx = (int)(r % t.n[0]);
+        r /= t.n[0];
+        const int ry = (int)(r % t.n[1]);
+        const int rz = (int)(r / t.n[1]);
+        const BoxView d = views[t.dst], s = views[t.src];
+        const long long od = (t.dlo[0] + rx) + (long long)(t.dlo[1] + ry) * d.sy +
+                             (long long)(t.dlo[2] + rz) * d.sz;
+        const long long os = (t.slo[0] + rx) + (long long)(t.slo[1] + ry) * s.sy +
+                             (long long)(t.slo[2] + rz) * s.sz;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            d.data[q * d.sq + od] = s.data[q * s.sq + os];
+        });
+    }
+}
+
+// Physical boundary fill of one box whose valid cells start at lo in a
+// domain of dn cells (g.n holds the box extents).  A ghost cell outside the
+// domain in a non-periodic dimension takes, for each population q whose
+// source g + c_q lies in the box's grown region and in the domain, the wall
+// value of its resolved class (src/LBM/BoundaryConditions.H:124-189).
+template <class L>
+__global__ __launch_bounds__(256) void k_level_bc(double* __restrict__ f, const Geo g, const View v,
+                                                  const int lo0, const int lo1, const int lo2,
+                                                  const int dn0, const int dn1, const int dn2)
+{
+    const int gz = (L::D == 3);
+    const int x = blockIdx.x * blockDim.x + threadIdx.x - 1;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y - 1;
+    const int z = (int)blockIdx.z - gz;
+    if (x >= g.n[0] + 1 || y >= g.n[1] + 1) return;
+    const int p[3] = {x, y, z}, lo[3] = {lo0, lo1, lo2}, dn[3] = {dn0, dn1, dn2};
+    int code[3] = {0, 0, 0};
+    bool wall = false;
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+        const int P = lo[d] + p[d];
+        if (P < 0) code[d] = -1;
+        else if (P >= dn[d]) code[d] = 1;
+        if (g.per[d]) code[d] = 0;
+        if (code[d]) wall = true;
+    }
+    if (!wall) return;
+    const int idx = (code[0] + 1) + 3 * (code[1] + 1) + (L::D == 3 ? 9 * (code[2] + 1) : 0);
+    if (g.rtype[idx] == MFA_BC_PERIODIC) return;
+    const long long ghost = x + y * v.sy + z * v.sz;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int s[3] = {x + L::c(q, 0), y + L::c(q, 1), z + L::c(q, 2)};
+#pragma unroll
+        for (int d = 0; d < L::D; ++d) {
+            if (s[d] < -1 || s[d] > g.n[d]) return;              // outside the grown box
+            if (lo[d] + s[d] < 0 || lo[d] + s[d] >= dn[d]) return;  // outside the domain
+        }
+        const long long src = s[0] + s[1] * v.sy + s[2] * v.sz;
+        f[q * v.sq + ghost] = wall_value<L, q>(f, src, v, g, idx);
+    });
+}
+
+template <class Fn>
+static int with_lattice_lv(int lat, Fn&& fn)
+{
+    switch (lat) {
+    case MFA_D2Q9: return fn(Lattice<MFA_D2Q9>{});
+    case MFA_D3Q15: return fn(Lattice<MFA_D3Q15>{});
+    case MFA_D3Q19: return fn(Lattice<MFA_D3Q19>{});
+    case MFA_D3Q27: return fn(Lattice<MFA_D3Q27>{});
+    default: return fail(MFA_ERR_INVALID, "unknown lattice id");
+    }
+}
+
+}  // namespace mfa
+
+using namespace mfa;
+
+struct mfa_level {
+    int lattice = 0, D = 3, Q = 0, device = 0;
+    int dn[3] = {1, 1, 1}, per[3] = {1, 1, 1};
+    std::vector<int> lo, n;                 // nboxes x 3
+    int nboxes = 0, ntags = 0;
+    long long total = 0;
+    CopyTag* d_tags = nullptr;
+    long long* d_start = nullptr;
+    BoxView* d_views = nullptr;             // kLevelSlots x nboxes
+    bool bound[kLevelSlots] = {false, false, false, false};
+    std::vector<BoxView> views[kLevelSlots];
+};
+
+static int lattice_q_lv(int lat)
+{
+    switch (lat) {
+    case MFA_D2Q9: return 9;
+    case MFA_D3Q15: return 15;
+    case MFA_D3Q19: return 19;
+    case MFA_D3Q27: return 27;
+    default: return -1;
+    }
+}
+
+static void free_level(mfa_level* lv)
+{
+    if (lv->d_tags) (void)hipFree(lv->d_tags);
+    if (lv->d_start) (void)hipFree(lv->d_start);
+    if (lv->d_views) (void)hipFree(lv->d_views);
+    delete lv;
+}
+
+int mfa_level_create(int lattice, const int domain_n[3], const int periodic[3], int nboxes,
+                     const int* box_lo, const int* box_n, mfa_level** out)
+{
+    if (!out) return fail(MFA_ERR_INVALID, "mfa_level_create: null out");
+    *out = nullptr;
+    const int Q = lattice_q_lv(lattice);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_level_create: unknown lattice");
+    if (!domain_n || !periodic || !box_lo || !box_n || nboxes < 1)
+        return fail(MFA_ERR_INVALID, "mfa_level_create: null argument or no boxes");
+    const int D = lattice == MFA_D2Q9 ? 2 : 3;
+    int dn[3], per[3];
+    for (int d = 0; d < 3; ++d) {
+        dn[d] = d < D ? domain_n[d] : 1;
+        per[d] = d < D ? (periodic[d] != 0) : 1;
+        if (dn[d] < 1) return fail(MFA_ERR_INVALID, "mfa_level_create: empty domain");
+    }
+    if (D == 2 && domain_n[2] > 1) return fail(MFA_ERR_INVALID, "mfa_level_create: 2-D lattice needs domain_n[2] == 1");
+    long long cells = 0;
+    for (int b = 0; b < nboxes; ++b) {
+        for (int d = 0; d < 3; ++d) {
+            const int l = d < D ? box_lo[3 * b + d] : 0, m = d < D ? box_n[3 * b + d] : 1;
+            if (d >= D && (box_lo[3 * b + d] != 0 || box_n[3 * b + d] > 1))
+                return fail(MFA_ERR_INVALID, "mfa_level_create: 2-D boxes need lo[2] == 0, n[2] == 1");
+            if (m < 1 || l < 0 || l + m > dn[d])
+                return fail(MFA_ERR_INVALID, "mfa_level_create: box " + std::to_string(b) + " outside the domain");
+        }
+        cells += (long long)box_n[3 * b] * box_n[3 * b + 1] * (D == 3 ? box_n[3 * b + 2] : 1);
+    }
+    for (int a = 0; a < nboxes; ++a)
+        for (int b = a + 1; b < nboxes; ++b) {
+            bool overlap = true;
+            for (int d = 0; d < D; ++d) {
+                const int a0 = box_lo[3 * a + d], a1 = a0 + box_n[3 * a + d];
+                const int b0 = box_lo[3 * b + d], b1 = b0 + box_n[3 * b + d];
+                if (std::max(a0, b0) >= std::min(a1, b1)) overlap = false;
+            }
+            if (overlap)
+                return fail(MFA_ERR_INVALID, "mfa_level_create: boxes " + std::to_string(a) + " and " +
+                                                 std::to_string(b) + " overlap");
+        }
+    if (cells != (long long)dn[0] * dn[1] * dn[2])
+        return fail(MFA_ERR_INVALID, "mfa_level_create: the boxes do not cover the domain");
+
+    mfa_level* lv = new mfa_level;
+    lv->lattice = lattice;
+    lv->D = D;
+    lv->Q = Q;
+    lv->nboxes = nboxes;
+    std::memcpy(lv->dn, dn, sizeof(dn));
+    std::memcpy(lv->per, per, sizeof(per));
+    lv->lo.resize(3 * nboxes);
+    lv->n.resize(3 * nboxes);
+    for (int b = 0; b < nboxes; ++b)
+        for (int d = 0; d < 3; ++d) {
+            lv->lo[3 * b + d] = d < D ? box_lo[3 * b + d] : 0;
+            lv->n[3 * b + d] = d < D ? box_n[3 * b + d] : 1;
+        }
+    // plan the ghost copies
+    std::vector<CopyTag> tags;
+    std::vector<long long> start;
+    const int sr = 1;
+    for (int b = 0; b < nboxes; ++b) {
+        int G0[3], G1[3];                   // grown region of b, [G0, G1)
+        for (int d = 0; d < 3; ++d) {
+            G0[d] = lv->lo[3 * b + d] - (d < D ? 1 : 0);
+            G1[d] = lv->lo[3 * b + d] + lv->n[3 * b + d] + (d < D ? 1 : 0);
+        }
+        for (int a = 0; a < nboxes; ++a)
+            for (int sz = -sr; sz <= sr; ++sz)
+                for (int sy = -sr; sy <= sr; ++sy)
+                    for (int sx = -sr; sx <= sr; ++sx) {
+                        const int s[3] = {sx, sy, sz};
+                        bool ok = true, zero = true;
+                        for (int d = 0; d < 3; ++d) {
+                            if (s[d] && (d >= D || !per[d])) ok = false;
+                            if (s[d]) zero = false;
+                        }
+                        if (!ok || (zero && a == b)) continue;
+                        CopyTag t;
+                        t.dst = b;
+                        t.src = a;
+                        long long count = 1;
+                        for (int d = 0; d < 3; ++d) {
+                            const int S0 = lv->lo[3 * a + d] + s[d] * dn[d];
+                            const int S1 = S0 + lv->n[3 * a + d];
+                            const int I0 = std::max(S0, G0[d]), I1 = std::min(S1, G1[d]);
+                            if (I0 >= I1) { count = 0; break; }
+                            t.n[d] = I1 - I0;
+                            t.dlo[d] = I0 - lv->lo[3 * b + d];
+                            t.slo[d] = I0 - S0;
+                            count *= t.n[d];
+                        }
+                        if (!count) continue;
+                        start.push_back(lv->total);
+                        tags.push_back(t);
+                        lv->total += count;
+                    }
+    }
+    lv->ntags = (int)tags.size();
+    hipError_t e = hipGetDevice(&lv->device);
+    if (e == hipSuccess && lv->ntags) e = hipMalloc((void**)&lv->d_tags, tags.size() * sizeof(CopyTag));
+    if (e == hipSuccess && lv->ntags) e = hipMalloc((void**)&lv->d_start, start.size() * sizeof(long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&lv->d_views, (size_t)kLevelSlots * nboxes * sizeof(BoxView));
+    if (e == hipSuccess && lv->ntags)
+        e = hipMemcpy(lv->d_tags, tags.data(), tags.size() * sizeof(CopyTag), hipMemcpyHostToDevice);
+    if (e == hipSuccess && lv->ntags)
+        e = hipMemcpy(lv->d_start, start.data(), start.size() * sizeof(long long), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_level(lv);
+        return fail(e == hipErrorOutOfMemory ? MFA_ERR_NOMEM : MFA_ERR_HIP,
+                    std::string("mfa_level_create: ") + hipGetErrorString(e));
+    }
+    *out = lv;
+    return MFA_OK;
+}
+
+int mfa_level_destroy(mfa_level* lv)
+{
+    if (lv) free_level(lv);
+    return MFA_OK;
+}
+
+long long mfa_level_ghost_cells(const mfa_level* lv) { return lv ? lv->total : 0; }
+
+int mfa_level_bind(mfa_level* lv, int slot, const mfa_field* boxes)
+{
+    if (!lv || !boxes) return fail(MFA_ERR_INVALID, "mfa_level_bind: null argument");
+    if (slot < 0 || slot >= kLevelSlots) return fail(MFA_ERR_INVALID, "mfa_level_bind: slot out of range");
+    std::vector<BoxView> v(lv->nboxes);
+    for (int b = 0; b < lv->nboxes; ++b) {
+        const mfa_field* f = &boxes[b];
+        int rc = field_check(f, lv->Q, 1, "mfa_level_bind box");
+        if (rc) return rc;
+        for (int d = 0; d < 3; ++d)
+            if (f->n[d] != lv->n[3 * b + d])
+                return fail(MFA_ERR_INVALID, "mfa_level_bind: box " + std::to_string(b) + " has other extents");
+        v[b] = BoxView{(double*)f->data, f->sy, f->sz, f->sq};
+    }
+    const hipError_t e = hipMemcpy(lv->d_views + (size_t)slot * lv->nboxes, v.data(),
+                                   v.size() * sizeof(BoxView), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(MFA_ERR_HIP, std::string("mfa_level_bind: ") + hipGetErrorString(e));
+    lv->views[slot] = v;
+    lv->bound[slot] = true;
+    return MFA_OK;
+}
+
+int mfa_level_fill_boundary(const mfa_level* lv, int slot, void* stream)
+{
+    if (!lv) return fail(MFA_ERR_INVALID, "mfa_level_fill_boundary: null level");
+    if (slot < 0 || slot >= kLevelSlots || !lv->bound[slot])
+        return fail(MFA_ERR_INVALID, "mfa_level_fill_boundary: slot not bound");
+    if (!lv->total) return MFA_OK;
+    return with_lattice_lv(lv->lattice, [&](auto Lc) {
+        using L = decltype(Lc);
+        const long long blocks = std::min<long long>((lv->total + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_level_fill<L>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                           lv->d_tags, lv->d_start, lv->ntags, lv->total,
+                           lv->d_views + (size_t)slot * lv->nboxes);
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_level_apply_bc(const mfa_level* lv, int slot, const mfa_geom_bc* g, void* stream)
+{
+    if (!lv || !g) return fail(MFA_ERR_INVALID, "mfa_level_apply_bc: null argument");
+    if (slot < 0 || slot >= kLevelSlots || !lv->bound[slot])
+        return fail(MFA_ERR_INVALID, "mfa_level_apply_bc: slot not bound");
+    if (g->lattice != lv->lattice) return fail(MFA_ERR_INVALID, "mfa_level_apply_bc: lattice differs from the level's");
+    for (int d = 0; d < lv->D; ++d)
+        if ((g->periodic[d] != 0) != (lv->per[d] != 0))
+            return fail(MFA_ERR_INVALID, "mfa_level_apply_bc: periodicity differs from the level's");
+    bool walls = false;
+    for (int d = 0; d < lv->D; ++d) walls = walls || !lv->per[d];
+    if (!walls) return MFA_OK;
+    return with_lattice_lv(lv->lattice, [&](auto Lc) {
+        using L = decltype(Lc);
+        for (int b = 0; b < lv->nboxes; ++b) {
+            const int* n = &lv->n[3 * b];
+            const int* lo = &lv->lo[3 * b];
+            // boxes away from every non-periodic face have no physical ghosts
+            bool touches = false;
+            for (int d = 0; d < L::D; ++d)
+                if (!lv->per[d] && (lo[d] == 0 || lo[d] + n[d] == lv->dn[d])) touches = true;
+            if (!touches) continue;
+            const Geo geo = geo_for(n, g, lv->lattice);
+            const BoxView& bv = lv->views[slot][b];
+            const View v{bv.sy, bv.sz, bv.sq};
+            const int nx = n[0] + 2;
+            const int bx = nx <= 64 ? 64 : (nx <= 128 ? 128 : 256);
+            const dim3 block(bx, 256 / bx, 1);
+            const dim3 grid((nx + bx - 1) / bx, (n[1] + 2 + block.y - 1) / block.y, n[2] + (L::D == 3 ? 2 : 0));
+            hipLaunchKernelGGL(k_level_bc<L>, grid, block, 0, (hipStream_t)stream, bv.data, geo, v, lo[0], lo[1],
+                               lo[2], lv->dn[0], lv->dn[1], lv->dn[2]);
+            MFA_HIP_CHECK(hipGetLastError());
+        }
+        return (int)MFA_OK;
+    });
+}

@@ -1,0 +1,170 @@
+"""A periodic D3Q19 domain chopped into boxes of at most max_grid_size cells
+per side, stepped through the multi-box level (include/mfa_level.h), against
+the same domain as one box.
+
+The chopping matches the role of AMReX's ``amr.max_grid_size`` for the
+reference solver (default 64, so a 256^3 domain runs as 64 boxes of 64^3):
+each dimension is cut into ceil(N / max_grid_size) pieces whose sizes differ
+by at most one cell.  A step of the level is one ghost exchange
+(``mfa_level_fill_boundary``), the physical boundary fill (nothing on a
+periodic box) and one ``mfa_pull_collide`` per box; the sequence of two
+steps is captured once in a HIP graph and replayed.
+
+``python -m muphfasa_amd.level --n 256 --max-grid 64 --steps 200`` prints the
+MLUPS of the level and of the single box, and whether their final states are
+bit-identical.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+
+from . import abi
+
+
+def chop(n: int, max_grid: int):
+    """Cut [0, n) into ceil(n / max_grid) nearly equal pieces: (lo, size)."""
+    k = max(1, math.ceil(n / max_grid))
+    base, extra = divmod(n, k)
+    out, lo = [], 0
+    for i in range(k):
+        m = base + (1 if i < extra else 0)
+        out.append((lo, m))
+        lo += m
+    return out
+
+
+def boxes(domain, max_grid):
+    lo, n = [], []
+    for z0, nz in chop(domain[2], max_grid):
+        for y0, ny in chop(domain[1], max_grid):
+            for x0, nx in chop(domain[0], max_grid):
+                lo.append([x0, y0, z0])
+                n.append([nx, ny, nz])
+    return lo, n
+
+
+class BoxLevel:
+    """Two population buffers of every box (ngrow 1), bound to level slots
+    0 and 1."""
+
+    def __init__(self, lattice: str, domain, max_grid: int, g, dev="cuda:0"):
+        import torch
+        self.L = abi.LATTICE[lattice]
+        self.Q = {"D3Q15": 15, "D3Q19": 19, "D3Q27": 27}[lattice]
+        self.domain = list(domain)
+        self.lo, self.n = boxes(self.domain, max_grid)
+        self.g = g
+        per = [int(g.periodic[d]) for d in range(3)]
+        self.level = abi.Level(self.L, self.domain, per, self.lo, self.n)
+        self.t = [[torch.full((self.Q, m[2] + 2, m[1] + 2, m[0] + 2), float("nan"), dtype=torch.float64,
+                              device=dev) for m in self.n] for _ in range(2)]
+        self.f = [[abi.field(t, self.n[b], 1, 3) for b, t in enumerate(ts)] for ts in self.t]
+        self.level.bind(0, self.f[0])
+        self.level.bind(1, self.f[1])
+
+    def scatter(self, full, slot=0):
+        """Copy a (Q, nz, ny, nx) tensor of the whole domain into the boxes."""
+        for b, (l, m) in enumerate(zip(self.lo, self.n)):
+            self.t[slot][b][:, 1:-1, 1:-1, 1:-1] = full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1],
+                                                        l[0]:l[0] + m[0]]
+
+    def gather(self, full, slot=0):
+        for b, (l, m) in enumerate(zip(self.lo, self.n)):
+            full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1], l[0]:l[0] + m[0]] = \
+                self.t[slot][b][:, 1:-1, 1:-1, 1:-1]
+
+    def step(self, src: int, omega: float, force=None):
+        """One step from slot src into the other slot."""
+        self.level.fill_boundary(src)
+        self.level.apply_bc(src, self.g)
+        for b in range(len(self.n)):
+            abi.pull_collide(self.f[src][b], self.f[1 - src][b], self.L, omega, force)
+
+    def close(self):
+        self.level.close()
+
+
+def run(n=256, max_grid=64, steps=200, tau=0.8, force=(1e-6, 0.0, 0.0), dev="cuda:0"):
+    import torch
+    lat = "D3Q19"
+    Q = 19
+    dom = [n, n, n]
+    g = abi.resolve_bc(abi.LATTICE[lat], [abi.BC["Periodic"]] * 6)
+    omega = 1.0 / tau
+    F = list(force)
+    # initial state: feq of rho = 1 and seeded small velocities
+    torch.manual_seed(2024)
+    rho = torch.ones((1, n, n, n), dtype=torch.float64, device=dev)
+    vel = (torch.rand((3, n, n, n), dtype=torch.float64, device=dev) - 0.5) * 2e-3
+    full = torch.empty((Q, n, n, n), dtype=torch.float64, device=dev)
+    abi.init_equilibrium(abi.field(rho, dom, 0, 3), abi.field(vel, dom, 0, 3), abi.field(full, dom, 0, 3),
+                         abi.LATTICE[lat], F)
+    del rho, vel
+    steps -= steps % 2
+    out = {"workload": f"D3Q19 periodic {n}^3 FP64 forced, boxes of at most {max_grid}^3",
+           "steps": steps}
+    # one box
+    a, b = full.clone(), torch.empty_like(full)
+    fa, fb = abi.field(a, dom, 0, 3), abi.field(b, dom, 0, 3)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        abi.step(fa, fb, g, omega, F)
+        abi.step(fb, fa, g, omega, F)
+    torch.cuda.synchronize()
+    a.copy_(full)
+    gr = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gr):
+        abi.step(fa, fb, g, omega, F)
+        abi.step(fb, fa, g, omega, F)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps // 2):
+        gr.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    out["one_box"] = {"boxes": 1, "ms_per_step": round(ms / steps, 4),
+                      "mlups": round(n ** 3 * steps / (ms * 1e-3) / 1e6, 1)}
+    single = a
+    del b, gr
+    # the level
+    lv = BoxLevel(lat, dom, max_grid, g, dev)
+    lv.scatter(full, 0)
+    with torch.cuda.stream(s):
+        lv.step(0, omega, F)
+        lv.step(1, omega, F)
+    torch.cuda.synchronize()
+    lv.scatter(full, 0)
+    gl = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gl):
+        lv.step(0, omega, F)
+        lv.step(1, omega, F)
+    e0.record()
+    for _ in range(steps // 2):
+        gl.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    out["level"] = {"boxes": len(lv.n), "ghost_cells_per_exchange": lv.level.ghost_cells,
+                    "ms_per_step": round(ms / steps, 4),
+                    "mlups": round(n ** 3 * steps / (ms * 1e-3) / 1e6, 1)}
+    got = torch.empty_like(full)
+    lv.gather(got, 0)
+    out["bit_identical"] = bool(torch.equal(got.view(torch.int64), single.view(torch.int64)))
+    lv.close()
+    return out
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--n", type=int, default=256)
+    p.add_argument("--max-grid", type=int, default=64)
+    p.add_argument("--steps", type=int, default=200)
+    a = p.parse_args()
+    print(json.dumps(run(a.n, a.max_grid, a.steps)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -17,8 +17,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_exports_match_header():
-    text = open(os.path.join(ROOT, "include", "mfa_lbm.h")).read()
-    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(mfa_\w+)\s*\(", text, re.M))
+    """abi.EXPORTS lists exactly the functions of mfa_lbm.h and mfa_level.h."""
+    declared = set()
+    for h in ("mfa_lbm.h", "mfa_level.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        declared |= set(re.findall(r"^\s*(?:int|long long|const char\*)\s+(mfa_\w+)\s*\(", text, re.M))
     assert declared, "no declarations parsed"
     L = abi.lib()
     for name in declared:
@@ -149,3 +152,21 @@ def test_every_header_declaration_is_exported():
         assert not missing, (h, missing)
         seen += len(names)
     assert seen >= 50
+
+
+def test_level_layout_checks_need_no_gpu():
+    """mfa_level_create validates the box layout on the host before it
+    touches the device: overlapping boxes, gaps, boxes outside the domain and
+    2-D boxes with a z extent are refused with MFA_ERR_INVALID."""
+    from muphfasa_amd import abi
+    D2 = abi.LATTICE["D2Q9"]
+    bad = [
+        ([8, 8], [[0, 0], [4, 0]], [[5, 8], [4, 8]]),      # overlap
+        ([8, 8], [[0, 0]], [[8, 7]]),                      # gap
+        ([8, 8], [[0, 0], [4, 0]], [[4, 8], [5, 8]]),      # past the domain
+        ([8, 8], [[0, 0, 1]], [[8, 8, 1]]),                # 2-D box off z = 0
+    ]
+    for dom, lo, n in bad:
+        with pytest.raises(abi.MfaError) as e:
+            abi.Level(D2, dom, [1, 1], lo, n)
+        assert e.value.code == -1, (dom, lo, n)

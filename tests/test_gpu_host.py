@@ -171,11 +171,15 @@ def test_poiseuille_regression(case, window, tol, trans, spread):
     assert err < tol and mean_err < tol and tr < trans and rs < spread
 
 
-def test_graph_replayed_steps_equal_single_steps():
-    """advance(n) replays captured graphs of 64 steps from either buffer; the
-    state, time and moments equal those of n single advance() calls, for
-    counts that start from both buffers and leave a remainder."""
+@pytest.mark.parametrize("lattice", ["D3Q15", "D3Q19", "D3Q27"])
+def test_graph_replayed_steps_equal_single_steps(lattice):
+    """advance(n) replays captured graphs of 64 steps (taken in pairs through
+    mfa_step2 on these periodic boxes) from either buffer; the state, time and
+    moments equal those of n single advance() calls, for counts that start
+    from both buffers and leave a remainder."""
     text = _read("periodic_d3q19_256").replace("256 256 256", "24 20 16")
+    text = text.replace("lbm.lattice_type   = D3Q19", f"lbm.lattice_type   = {lattice}")
+    assert f"= {lattice}" in text
     a = host.Simulation(text)
     b = host.Simulation(text)
     a.initialize()
