@@ -11,9 +11,10 @@
  * are bound to numbered slots, typically the two buffers of the A/B pull.
  * A step of the level is then
  *     mfa_level_fill_boundary(lv, a, s); mfa_level_apply_bc(lv, a, g, s);
- *     for each box b: mfa_pull_collide(box b of a, box b of b, ...);
- * and gives the same valid-cell values, bit for bit, as mfa_step on one box
- * holding the whole domain.  Moments of a box through its ghosts:
+ *     mfa_level_step(lv, a, b, omega, force, s);
+ * (the last is mfa_pull_collide on every box, in one launch) and gives the
+ * same valid-cell values, bit for bit, as mfa_step on one box holding the
+ * whole domain.  Moments of a box through its ghosts:
  * mfa_moments(..., streamed = 2) after the two fills.
  *
  * Status codes and error messages as in mfa_lbm.h. */
@@ -51,6 +52,14 @@ int mfa_level_bind(mfa_level* lv, int slot, const mfa_field* boxes);
  * periodic-grown domain gets the value (all components) of the valid cell
  * it images, from whichever box owns that cell.  One launch. */
 int mfa_level_fill_boundary(const mfa_level* lv, int slot, void* stream);
+
+/* pullCollideFusedMF over the level (src/LBM/Collision.cpp:95-143): every
+ * box of slot_in steps through its ghost cells into the same box of
+ * slot_out, in one launch; equal, bit for bit, to mfa_pull_collide on each
+ * box.  Call after mfa_level_fill_boundary and mfa_level_apply_bc on
+ * slot_in.  force: D doubles or NULL. */
+int mfa_level_step(const mfa_level* lv, int slot_in, int slot_out, double omega, const double* force,
+                   void* stream);
 
 /* PhysBCFunct on every box: each ghost cell outside the periodic-grown
  * domain gets, for every population whose source g + c_q is a cell of the

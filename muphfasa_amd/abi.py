@@ -25,7 +25,8 @@ EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_s
            "mfa_reduce", "mfa_residual", "mfa_set_launch", "mfa_get_launch", "mfa_set_launch_dev",
            "mfa_get_launch_dev", "mfa_stream_copy",
            "mfa_convert", "mfa_level_create", "mfa_level_destroy", "mfa_level_bind",
-           "mfa_level_fill_boundary", "mfa_level_apply_bc", "mfa_level_ghost_cells"]
+           "mfa_level_fill_boundary", "mfa_level_apply_bc", "mfa_level_ghost_cells",
+           "mfa_level_step"]
 
 
 class MfaError(RuntimeError):
@@ -108,6 +109,7 @@ def lib():
         L.mfa_level_fill_boundary.argtypes = [vp, ctypes.c_int, vp]
         L.mfa_level_apply_bc.argtypes = [vp, ctypes.c_int, G, vp]
         L.mfa_level_ghost_cells.argtypes = [vp]
+        L.mfa_level_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, vp]
         for name in EXPORTS[2:]:
             getattr(L, name).restype = ctypes.c_int
         L.mfa_level_ghost_cells.restype = ctypes.c_longlong
@@ -264,6 +266,9 @@ class Level:
 
     def apply_bc(self, slot: int, g: GeomBC, stream=None):
         check(lib().mfa_level_apply_bc(self.h, slot, ctypes.byref(g), stream_ptr(stream)))
+
+    def step(self, slot_in: int, slot_out: int, omega: float, force=None, stream=None):
+        check(lib().mfa_level_step(self.h, slot_in, slot_out, omega, _dbl(force), stream_ptr(stream)))
 
 
 def init_equilibrium(rho: Field, vel: Field, f: Field, lattice: int, force=None, stream=None):

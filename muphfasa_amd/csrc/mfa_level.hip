@@ -7,9 +7,8 @@
 // destination box b, every source box a and every periodic image s of the
 // domain, the cells of a's valid region shifted by s that fall in b's
 // one-cell-grown region.  Valid regions tile the domain, so these regions
-// are disjoint and lie in b's ghost layer.  One launch walks the ghost
-// cells of all tags (a binary search over the tags' prefix sums finds a
-// cell's tag) and copies every component.
+// are disjoint and lie in b's ghost layer.  One launch copies every
+// component of all tags, in chunks of 256 cells.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,36 +38,74 @@ struct CopyTag {                  // a ghost region of box dst imaged from valid
 
 constexpr int kLevelSlots = 4;
 
+// Each tag is cut into chunks of 256 cells (x fastest); one block per chunk
+// finds its tag by binary search over the chunks' prefix sums (the same
+// for all its threads) and copies every component of its cells.
+constexpr int kFillChunk = 256;
+
 template <class L>
-__global__ __launch_bounds__(256) void k_level_fill(const CopyTag* __restrict__ tags,
-                                                    const long long* __restrict__ start, const int ntags,
-                                                    const long long total,
-                                                    const BoxView* __restrict__ views)
+__global__ __launch_bounds__(kFillChunk) void k_level_fill(const CopyTag* __restrict__ tags,
+                                                           const long long* __restrict__ cstart,
+                                                           const int ntags,
+                                                           const BoxView* __restrict__ views)
 {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += stride) {
-        int lo = 0, hi = ntags - 1;           // last tag whose start is <= i
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (start[mid] <= i) lo = mid;
-            else hi = mid - 1;
-        }
-        const CopyTag t = tags[lo];
-        long long r = i - start[lo];
-        const int rx = (int)(r % t.n[0]);
-        r /= t.n[0];
-        const int ry = (int)(r % t.n[1]);
-        const int rz = (int)(r / t.n[1]);
-        const BoxView d = views[t.dst], s = views[t.src];
-        const long long od = (t.dlo[0] + rx) + (long long)(t.dlo[1] + ry) * d.sy +
-                             (long long)(t.dlo[2] + rz) * d.sz;
-        const long long os = (t.slo[0] + rx) + (long long)(t.slo[1] + ry) * s.sy +
-                             (long long)(t.slo[2] + rz) * s.sz;
-        for_q<L>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            d.data[q * d.sq + od] = s.data[q * s.sq + os];
-        });
+    const long long blk = blockIdx.x + (long long)gridDim.x * blockIdx.y;
+    if (blk >= cstart[ntags]) return;
+    int lo = 0, hi = ntags - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cstart[mid] <= blk) lo = mid;
+        else hi = mid - 1;
     }
+    const CopyTag t = tags[lo];
+    const int i = (int)(blk - cstart[lo]) * kFillChunk + (int)threadIdx.x;
+    if (i >= t.n[0] * t.n[1] * t.n[2]) return;
+    const int rx = i % t.n[0], ry = (i / t.n[0]) % t.n[1], rz = i / (t.n[0] * t.n[1]);
+    const BoxView d = views[t.dst], s = views[t.src];
+    const long long od = (t.dlo[0] + rx) + (long long)(t.dlo[1] + ry) * d.sy +
+                         (long long)(t.dlo[2] + rz) * d.sz;
+    const long long os = (t.slo[0] + rx) + (long long)(t.slo[1] + ry) * s.sy +
+                         (long long)(t.slo[2] + rz) * s.sz;
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        d.data[q * d.sq + od] = s.data[q * s.sq + os];
+    });
+}
+
+// The step of every box through its ghost cells (mfa_pull_collide on each
+// box) in one launch.  Blocks of 64 x 4 cells of one z-plane; box b owns
+// blocks [bstart[b], bstart[b+1]); a block finds its box by binary search
+// (the same for all its threads).
+template <class L, bool FORCED>
+__global__ __launch_bounds__(256) void k_level_step(const BoxView* __restrict__ vin,
+                                                    const BoxView* __restrict__ vout,
+                                                    const int* __restrict__ boxn,
+                                                    const long long* __restrict__ bstart, const int nboxes,
+                                                    const Geo g, const double omega, const double F0,
+                                                    const double F1, const double F2)
+{
+    const long long blk = blockIdx.x + (long long)gridDim.x * blockIdx.y;
+    if (blk >= bstart[nboxes]) return;
+    int lo = 0, hi = nboxes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const int nx = boxn[3 * lo], ny = boxn[3 * lo + 1];
+    const int tx = (nx + 63) / 64, ty = (ny + 3) / 4;
+    long long r = blk - bstart[lo];
+    const int bx = (int)(r % tx);
+    r /= tx;
+    const int by = (int)(r % ty);
+    const int z = (int)(r / ty);
+    const int x = bx * 64 + (int)threadIdx.x, y = by * 4 + (int)threadIdx.y;
+    if (x >= nx || y >= ny) return;
+    const BoxView a = vin[lo], b = vout[lo];
+    const View vi{a.sy, a.sz, a.sq}, vo{b.sy, b.sz, b.sq};
+    double f[L::Q];
+    pull<L, 1, false, 0>(f, a.data, x, y, z, g, vi);
+    collide_store<L, FORCED>(f, b.data, x + y * vo.sy + (long long)z * vo.sz, vo, omega, F0, F1, F2);
 }
 
 // Physical boundary fill of one box whose valid cells start at lo in a
@@ -137,7 +174,11 @@ struct mfa_level {
     int nboxes = 0, ntags = 0;
     long long total = 0;
     CopyTag* d_tags = nullptr;
-    long long* d_start = nullptr;
+    long long* d_cstart = nullptr;          // fill chunks per tag, prefix sums (ntags + 1)
+    long long fill_blocks = 0;
+    int* d_boxn = nullptr;                  // nboxes x 3 extents
+    long long* d_bstart = nullptr;          // step blocks per box, prefix sums (nboxes + 1)
+    long long step_blocks = 0;
     BoxView* d_views = nullptr;             // kLevelSlots x nboxes
     bool bound[kLevelSlots] = {false, false, false, false};
     std::vector<BoxView> views[kLevelSlots];
@@ -157,7 +198,9 @@ static int lattice_q_lv(int lat)
 static void free_level(mfa_level* lv)
 {
     if (lv->d_tags) (void)hipFree(lv->d_tags);
-    if (lv->d_start) (void)hipFree(lv->d_start);
+    if (lv->d_cstart) (void)hipFree(lv->d_cstart);
+    if (lv->d_boxn) (void)hipFree(lv->d_boxn);
+    if (lv->d_bstart) (void)hipFree(lv->d_bstart);
     if (lv->d_views) (void)hipFree(lv->d_views);
     delete lv;
 }
@@ -221,7 +264,6 @@ int mfa_level_create(int lattice, const int domain_n[3], const int periodic[3], 
         }
     // plan the ghost copies
     std::vector<CopyTag> tags;
-    std::vector<long long> start;
     const int sr = 1;
     for (int b = 0; b < nboxes; ++b) {
         int G0[3], G1[3];                   // grown region of b, [G0, G1)
@@ -255,20 +297,36 @@ int mfa_level_create(int lattice, const int domain_n[3], const int periodic[3], 
                             count *= t.n[d];
                         }
                         if (!count) continue;
-                        start.push_back(lv->total);
                         tags.push_back(t);
                         lv->total += count;
                     }
     }
     lv->ntags = (int)tags.size();
+    std::vector<long long> cstart(tags.size() + 1, 0);
+    for (size_t t = 0; t < tags.size(); ++t)
+        cstart[t + 1] = cstart[t] + ((long long)tags[t].n[0] * tags[t].n[1] * tags[t].n[2] + kFillChunk - 1) /
+                                        kFillChunk;
+    lv->fill_blocks = cstart.back();
+    // blocks of the batched step: 64 x 4 cells of one z-plane
+    std::vector<long long> bstart(nboxes + 1, 0);
+    for (int b = 0; b < nboxes; ++b)
+        bstart[b + 1] = bstart[b] + (long long)((lv->n[3 * b] + 63) / 64) * ((lv->n[3 * b + 1] + 3) / 4) *
+                                        lv->n[3 * b + 2];
+    lv->step_blocks = bstart[nboxes];
     hipError_t e = hipGetDevice(&lv->device);
     if (e == hipSuccess && lv->ntags) e = hipMalloc((void**)&lv->d_tags, tags.size() * sizeof(CopyTag));
-    if (e == hipSuccess && lv->ntags) e = hipMalloc((void**)&lv->d_start, start.size() * sizeof(long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&lv->d_cstart, cstart.size() * sizeof(long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&lv->d_boxn, lv->n.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&lv->d_bstart, bstart.size() * sizeof(long long));
     if (e == hipSuccess) e = hipMalloc((void**)&lv->d_views, (size_t)kLevelSlots * nboxes * sizeof(BoxView));
     if (e == hipSuccess && lv->ntags)
         e = hipMemcpy(lv->d_tags, tags.data(), tags.size() * sizeof(CopyTag), hipMemcpyHostToDevice);
-    if (e == hipSuccess && lv->ntags)
-        e = hipMemcpy(lv->d_start, start.data(), start.size() * sizeof(long long), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(lv->d_cstart, cstart.data(), cstart.size() * sizeof(long long), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(lv->d_boxn, lv->n.data(), lv->n.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(lv->d_bstart, bstart.data(), bstart.size() * sizeof(long long), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_level(lv);
         return fail(e == hipErrorOutOfMemory ? MFA_ERR_NOMEM : MFA_ERR_HIP,
@@ -313,13 +371,47 @@ int mfa_level_fill_boundary(const mfa_level* lv, int slot, void* stream)
     if (!lv) return fail(MFA_ERR_INVALID, "mfa_level_fill_boundary: null level");
     if (slot < 0 || slot >= kLevelSlots || !lv->bound[slot])
         return fail(MFA_ERR_INVALID, "mfa_level_fill_boundary: slot not bound");
-    if (!lv->total) return MFA_OK;
+    if (!lv->ntags) return MFA_OK;
     return with_lattice_lv(lv->lattice, [&](auto Lc) {
         using L = decltype(Lc);
-        const long long blocks = std::min<long long>((lv->total + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_level_fill<L>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
-                           lv->d_tags, lv->d_start, lv->ntags, lv->total,
-                           lv->d_views + (size_t)slot * lv->nboxes);
+        const long long nb = lv->fill_blocks;
+        const unsigned gx = (unsigned)std::min<long long>(nb, 65535);
+        const unsigned gy = (unsigned)((nb + gx - 1) / gx);
+        hipLaunchKernelGGL(k_level_fill<L>, dim3(gx, gy), dim3(kFillChunk), 0, (hipStream_t)stream, lv->d_tags,
+                           lv->d_cstart, lv->ntags, lv->d_views + (size_t)slot * lv->nboxes);
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_level_step(const mfa_level* lv, int slot_in, int slot_out, double omega, const double* force,
+                   void* stream)
+{
+    if (!lv) return fail(MFA_ERR_INVALID, "mfa_level_step: null level");
+    for (int s : {slot_in, slot_out})
+        if (s < 0 || s >= kLevelSlots || !lv->bound[s])
+            return fail(MFA_ERR_INVALID, "mfa_level_step: slot not bound");
+    if (slot_in == slot_out) return fail(MFA_ERR_INVALID, "mfa_level_step: slot_in == slot_out");
+    for (int b = 0; b < lv->nboxes; ++b)
+        if (lv->views[slot_in][b].data == lv->views[slot_out][b].data)
+            return fail(MFA_ERR_INVALID, "mfa_level_step: a box aliases itself");
+    return with_lattice_lv(lv->lattice, [&](auto Lc) {
+        using L = decltype(Lc);
+        Geo g;
+        std::memset(&g, 0, sizeof(g));
+        const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0,
+                     F2 = (force && L::D == 3) ? force[2] : 0.0;
+        const long long nb = lv->step_blocks;
+        const unsigned gx = (unsigned)std::min<long long>(nb, 65535);
+        const unsigned gy = (unsigned)((nb + gx - 1) / gx);
+        const BoxView* vi = lv->d_views + (size_t)slot_in * lv->nboxes;
+        const BoxView* vo = lv->d_views + (size_t)slot_out * lv->nboxes;
+        if (force)
+            hipLaunchKernelGGL((k_level_step<L, true>), dim3(gx, gy), dim3(64, 4), 0, (hipStream_t)stream, vi, vo,
+                               lv->d_boxn, lv->d_bstart, lv->nboxes, g, omega, F0, F1, F2);
+        else
+            hipLaunchKernelGGL((k_level_step<L, false>), dim3(gx, gy), dim3(64, 4), 0, (hipStream_t)stream, vi,
+                               vo, lv->d_boxn, lv->d_bstart, lv->nboxes, g, omega, F0, F1, F2);
         MFA_HIP_CHECK(hipGetLastError());
         return (int)MFA_OK;
     });

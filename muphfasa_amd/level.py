@@ -7,8 +7,9 @@ reference solver (default 64, so a 256^3 domain runs as 64 boxes of 64^3):
 each dimension is cut into ceil(N / max_grid_size) pieces whose sizes differ
 by at most one cell.  A step of the level is one ghost exchange
 (``mfa_level_fill_boundary``), the physical boundary fill (nothing on a
-periodic box) and one ``mfa_pull_collide`` per box; the sequence of two
-steps is captured once in a HIP graph and replayed.
+periodic box) and the step of every box through its ghosts in one launch
+(``mfa_level_step``); the sequence of two steps is captured once in a HIP
+graph and replayed.
 
 ``python -m muphfasa_amd.level --n 256 --max-grid 64 --steps 200`` prints the
 MLUPS of the level and of the single box, and whether their final states are
@@ -79,8 +80,7 @@ class BoxLevel:
         """One step from slot src into the other slot."""
         self.level.fill_boundary(src)
         self.level.apply_bc(src, self.g)
-        for b in range(len(self.n)):
-            abi.pull_collide(self.f[src][b], self.f[1 - src][b], self.L, omega, force)
+        self.level.step(src, 1 - src, omega, force)
 
     def close(self):
         self.level.close()

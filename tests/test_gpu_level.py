@@ -1,7 +1,8 @@
 """Several boxes of one lattice on one device (include/mfa_level.h): the ghost
 exchange between boxes (FillBoundary with periodic images), the physical
 boundary fill of each box at its place in the domain, and the step of every
-box through its ghosts (mfa_pull_collide), against the oracle on the whole
+box through its ghosts (mfa_pull_collide per box, and mfa_level_step for all
+boxes in one launch, on alternate steps), against the oracle on the whole
 domain as one box, bit for bit.  The box layouts are uneven (as AMReX's
 max_grid_size chopping leaves them) and include boxes of one cell."""
 import numpy as np
@@ -88,11 +89,14 @@ def test_level_steps_equal_one_box_oracle(dev, lat, bc, forced):
         lv.bind(0, fa)
         lv.bind(1, fb)
         cur, nxt = (0, fa), (1, fb)
-        for _ in range(steps):
+        for s in range(steps):
             lv.fill_boundary(cur[0])
             lv.apply_bc(cur[0], g)
-            for b in range(len(lo)):
-                abi.pull_collide(cur[1][b], nxt[1][b], abi.LATTICE[lat], omega, F)
+            if s % 2:        # one launch for all boxes
+                lv.step(cur[0], nxt[0], omega, F)
+            else:            # mfa_pull_collide box by box
+                for b in range(len(lo)):
+                    abi.pull_collide(cur[1][b], nxt[1][b], abi.LATTICE[lat], omega, F)
             cur, nxt = nxt, cur
         # moments through the ghosts of the final state
         lv.fill_boundary(cur[0])
