@@ -142,25 +142,25 @@ __device__ __forceinline__ Src make_src(int x, int y, int z, const G& g, const V
 }
 
 // Ghost value f(g, q) for g = x - c_q outside a non-periodic face, evaluated
-// from fin's valid cells as the reference's BCManager would have written it
-// (src/LBM/BoundaryConditions.H:124-189 and the src/LBM/BCTypes/ headers).
-template <class L, int q, class G>
-__device__ __forceinline__ double wall_value(const double* __restrict__ fin, long long self,
-                                             const View& v, const G& g, int idx)
+// from the populations of the cell x itself as the reference's BCManager
+// would have written it (src/LBM/BoundaryConditions.H:124-189 and the
+// src/LBM/BCTypes/ headers).  self_pop(m) returns population m of cell x:
+// from HBM in the one-step kernels, from LDS in the two-step kernels.
+template <class L, int q, class G, class Ld>
+__device__ __forceinline__ double wall_value_at(Ld&& self_pop, const G& g, int idx)
 {
     const int t = g.rtype[idx];
     if (t == MFA_BC_MOVING_WALL || t == MFA_BC_NO_SLIP || t == MFA_BC_INFLOW) {
         constexpr int m = L::opp(q);
-        return fin[m * v.sq + self] +
-               bb_contrib<L, m>(g.rrho[idx], g.ru[idx][0], g.ru[idx][1], g.ru[idx][2]);
+        return self_pop(m) + bb_contrib<L, m>(g.rrho[idx], g.ru[idx][0], g.ru[idx][1], g.ru[idx][2]);
     }
-    if (t == MFA_BC_OUTFLOW) return fin[q * v.sq + self];
+    if (t == MFA_BC_OUTFLOW) return self_pop(q);
     if (t == MFA_BC_PRESSURE_OUTLET) {
         // src/LBM/BCTypes/PressureOutletKernel.H:29-64
         double rho = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
 #pragma unroll
         for (int m = 0; m < L::Q; ++m) {
-            const double fm = fin[m * v.sq + self];
+            const double fm = self_pop(m);
             rho += fm;
             j0 += L::c(m, 0) * fm;
             j1 += L::c(m, 1) * fm;
@@ -174,15 +174,23 @@ __device__ __forceinline__ double wall_value(const double* __restrict__ fin, lon
         const double feq_even =
             L::w(q) * g.rrho[idx] * (1.0 + 0.5 * cu * cu * 3.0 * 3.0 - 0.5 * uu * 3.0);
         constexpr int m = L::opp(q);
-        return -fin[m * v.sq + self] + 2.0 * feq_even;
+        return -self_pop(m) + 2.0 * feq_even;
     }
     return __builtin_nan("");
 }
 
 template <class L, int q, class G>
-__device__ __forceinline__ void fix_wall(double& fq, const double* __restrict__ fin, long long self,
-                                         const View& v, const G& g, const int lo[3],
-                                         const int hi[3])
+__device__ __forceinline__ double wall_value(const double* __restrict__ fin, long long self,
+                                             const View& v, const G& g, int idx)
+{
+    return wall_value_at<L, q>([&](int m) { return fin[m * v.sq + self]; }, g, idx);
+}
+
+// Replace population q, pulled from behind a non-periodic face of the cell
+// with edge flags lo/hi, by its wall value.
+template <class L, int q, class G, class Ld>
+__device__ __forceinline__ void fix_wall_at(double& fq, Ld&& self_pop, const G& g, const int lo[3],
+                                            const int hi[3])
 {
     int code[3] = {0, 0, 0};
     bool wall = false;
@@ -194,8 +202,16 @@ __device__ __forceinline__ void fix_wall(double& fq, const double* __restrict__ 
     }
     if (wall) {
         const int idx = (code[0] + 1) + 3 * (code[1] + 1) + 9 * (code[2] + 1) * (L::D == 3);
-        fq = wall_value<L, q>(fin, self, v, g, idx);
+        fq = wall_value_at<L, q>(self_pop, g, idx);
     }
+}
+
+template <class L, int q, class G>
+__device__ __forceinline__ void fix_wall(double& fq, const double* __restrict__ fin, long long self,
+                                         const View& v, const G& g, const int lo[3],
+                                         const int hi[3])
+{
+    fix_wall_at<L, q>(fq, [&](int m) { return fin[m * v.sq + self]; }, g, lo, hi);
 }
 
 template <class L, int q>

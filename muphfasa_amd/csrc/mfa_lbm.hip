@@ -387,6 +387,73 @@ __global__ __launch_bounds__(T::THREADS) void k_step2(
     }
 }
 
+// Two steps per pass on a 2-D box, periodic or with walls.  A block owns a
+// TX x TY tile of output cells.  Phase A computes the first step on the tile
+// grown by one cell (periodic images; grown cells behind a wall are not
+// needed) into LDS; phase B pulls the second step from LDS.  At a wall the
+// second step's ghost values come from the cell's own intermediate
+// populations in LDS, through the same wall_value arithmetic as the
+// one-step kernel, so the pair equals two k_step launches bit for bit.
+template <class L, bool FORCED, bool WALLS, class T>
+__global__ __launch_bounds__(T::THREADS) void k_step2_2d(
+    const double* __restrict__ fin, double* __restrict__ fout, const Geo g, const View vi,
+    const View vo, const double omega, const double F0, const double F1, const double F2)
+{
+    __shared__ double lds[L::Q][T::EC];
+    const int tid = threadIdx.x;
+    const int nx = g.n[0], ny = g.n[1];
+    const unsigned nb = gridDim.x * gridDim.y;
+    const unsigned b = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned per = nb / kXcds, rem = nb % kXcds, k = b % kXcds;
+    const unsigned t = k * per + (k < rem ? k : rem) + b / kXcds;
+    const int X0 = (int)(t % gridDim.x) * T::TX, Y0 = (int)(t / gridDim.x) * T::TY;
+    if (tid < T::EC) {
+        int gx = X0 - 1 + tid % T::EX, gy = Y0 - 1 + tid / T::EX;
+        bool in = true;
+        if (g.per[0]) gx = (gx % nx + nx) % nx;
+        else in = in && gx >= 0 && gx < nx;
+        if (g.per[1]) gy = (gy % ny + ny) % ny;
+        else in = in && gy >= 0 && gy < ny;
+        if (in) {
+            double P[L::Q];
+            pull<L, 0, WALLS, 0>(P, fin, gx, gy, 0, g, vi);
+            collide<L, FORCED>(P, omega, F0, F1, F2, [&](auto qc, double out) {
+                constexpr int q = decltype(qc)::value;
+                lds[q][tid] = out;
+            });
+        }
+    }
+    __syncthreads();
+    if (tid >= T::TX * T::TY) return;
+    const int ox = tid % T::TX, oy = tid / T::TX;
+    const int x = X0 + ox, y = Y0 + oy;
+    if (x >= nx || y >= ny) return;
+    const int c0 = (ox + 1) + T::EX * (oy + 1);
+    double f[L::Q];
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = lds[q][c0 - L::c(q, 0) - T::EX * L::c(q, 1)];
+    });
+    if constexpr (WALLS) {
+        const int p[2] = {x, y};
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        bool edge = false;
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            lo[d] = !g.per[d] && p[d] == 0;
+            hi[d] = !g.per[d] && p[d] == g.n[d] - 1;
+            edge = edge || lo[d] || hi[d];
+        }
+        if (edge) {
+            for_q<L>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                fix_wall_at<L, q>(f[q], [&](int m) { return lds[m][c0]; }, g, lo, hi);
+            });
+        }
+    }
+    collide_store<L, FORCED>(f, fout, x + (long long)y * vo.sy, vo, omega, F0, F1, F2);
+}
+
 // ----------------------------------------------------- copy-ceiling probe --
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
@@ -747,10 +814,32 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
         return fail(MFA_ERR_INVALID, "mfa_step2: fin/fout boxes differ");
     if (fin->data == fout->data) return fail(MFA_ERR_INVALID, "mfa_step2: fin aliases fout");
     if (fin->dtype != fout->dtype) return fail(MFA_ERR_INVALID, "mfa_step2: fin/fout storage types differ");
-    if (lattice_dim(lat) != 3) return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: 3-D lattices only (use mfa_step)");
+    if ((rc = check_2d(fin, lat))) return rc;
     const Geo geo = geo_of(*fin, g, lat);
+    if (lattice_dim(lat) == 2) {
+        // D2Q9, FP64, periodic or with walls; 64 x 4 tiles
+        if (fin->dtype != MFA_F64)
+            return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: 2-D boxes in FP64 only (use mfa_step)");
+        using L = Lattice<MFA_D2Q9>;
+        using T = Step2Tile<64, 4>;
+        const View vi = view_of(*fin), vo = view_of(*fout);
+        const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0;
+        const dim3 grid((geo.n[0] + T::TX - 1) / T::TX, (geo.n[1] + T::TY - 1) / T::TY);
+        const double* a = (const double*)fin->data;
+        double* b = (double*)fout->data;
+        hipStream_t st = (hipStream_t)stream;
+        if (force) {
+            if (geo.walls) hipLaunchKernelGGL((k_step2_2d<L, true, true, T>), grid, dim3(T::THREADS), 0, st, a, b, geo, vi, vo, omega, F0, F1, 0.0);
+            else hipLaunchKernelGGL((k_step2_2d<L, true, false, T>), grid, dim3(T::THREADS), 0, st, a, b, geo, vi, vo, omega, F0, F1, 0.0);
+        } else {
+            if (geo.walls) hipLaunchKernelGGL((k_step2_2d<L, false, true, T>), grid, dim3(T::THREADS), 0, st, a, b, geo, vi, vo, omega, F0, F1, 0.0);
+            else hipLaunchKernelGGL((k_step2_2d<L, false, false, T>), grid, dim3(T::THREADS), 0, st, a, b, geo, vi, vo, omega, F0, F1, 0.0);
+        }
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    }
     if (geo.walls)
-        return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: fully periodic boxes only (use mfa_step)");
+        return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: 3-D boxes with walls are not offered (use mfa_step)");
     const int TZ = geo.n[2] < kStep2Planes ? geo.n[2] : kStep2Planes;
     if (fin->dtype != MFA_F64)
         return step2_dev(fin, fout, geo, lat, omega, force, TZ, (hipStream_t)stream);

@@ -285,10 +285,10 @@ void SimulationImpl::advanceSteps(int n)
             DeviceField* a = m_fin;
             DeviceField* b = m_fout;
             const double* F = m_has_forcing ? m_force.data() : nullptr;
-            // fully periodic 3-D boxes take the steps in pairs (mfa_step2,
-            // bit-identical to two mfa_step calls)
-            const bool pairs = m_lattice != MFA_D2Q9 && m_geom.periodic[0] && m_geom.periodic[1] &&
-                               m_geom.periodic[2];
+            // 2-D boxes and fully periodic 3-D boxes take the steps in pairs
+            // (mfa_step2, bit-identical to two mfa_step calls)
+            const bool pairs = m_lattice == MFA_D2Q9 ||
+                               (m_geom.periodic[0] && m_geom.periodic[1] && m_geom.periodic[2]);
             int rc = MFA_OK;
             for (int s = 0; s < kGraphSteps && rc == MFA_OK; s += pairs ? 2 : 1) {
                 rc = pairs ? mfa_step2(a->view(), b->view(), &m_bc, m_omega, F, m_stream)

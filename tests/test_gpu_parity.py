@@ -289,7 +289,42 @@ def test_step2_equals_two_oracle_steps(dev, lat, si, forced, layout):
     assert np.array_equal(got, O.valid(L, expect))
 
 
-def test_step2_refuses_walls_and_2d(dev):
+STEP2_2D_SIZES = [[13, 7], [64, 3], [130, 9], [1, 5], [70, 33]]
+
+
+@pytest.mark.parametrize("bc", ["periodic", "noslip", "lid", "xper_walls", "inflow_pressure", "inflow_outflow"])
+@pytest.mark.parametrize("si", range(len(STEP2_2D_SIZES)))
+@pytest.mark.parametrize("forced", [False, True])
+def test_step2_2d_equals_two_oracle_steps(dev, bc, si, forced):
+    """mfa_step2 on D2Q9 boxes, periodic or with walls of every kind (the
+    second step's ghost values come from the intermediate state in LDS):
+    the oracle's bits after two steps, for boxes smaller than the 64 x 4
+    tile, not a multiple of it, and one cell wide, in both population
+    orders."""
+    L = O.Lattice("D2Q9")
+    n = STEP2_2D_SIZES[si]
+    print(f"case step2_2d bc={bc} n={n} forced={forced}", flush=True)
+    g, ref, per = bc_tables(L, bc)
+    F = [2e-5, -1e-5] if forced else None
+    omega = 1.0 / 0.7
+    f0 = random_state(L, n, seed=zlib.crc32(f"step2_2d/{bc}/{si}".encode()))
+    _, mid = _oracle_step(L, f0, n, per, ref, omega, F)
+    _, expect = _oracle_step(L, mid, n, per, ref, omega, F)
+    import torch
+    for layout in ("component", "row"):
+        if layout == "row":
+            tin = to_dev(_to_rows(O.valid(L, f0)), dev)
+        else:
+            tin = to_dev(np.ascontiguousarray(O.valid(L, f0)), dev)
+        tout = torch.full_like(tin, float("nan"))
+        abi.step2(abi.field(tin, n, 0, 2, layout=layout), abi.field(tout, n, 0, 2, layout=layout),
+                  g, omega, F)
+        torch.cuda.synchronize()
+        got = _from_rows(to_host(tout)) if layout == "row" else to_host(tout)
+        assert np.array_equal(got, O.valid(L, expect)), layout
+
+
+def test_step2_refuses_3d_walls_and_2d_reduced(dev):
     import torch
     L = O.Lattice("D3Q19")
     g, _, _ = bc_tables(L, "lid")
@@ -300,7 +335,8 @@ def test_step2_refuses_walls_and_2d(dev):
     assert e.value.code == -2
     L2 = O.Lattice("D2Q9")
     g2, _, _ = bc_tables(L2, "periodic")
-    a2 = torch.zeros((9, 1, 4, 4), dtype=torch.float64, device=dev)
+    a2 = torch.zeros((9, 1, 4, 4), dtype=torch.float32, device=dev)
     with pytest.raises(abi.MfaError) as e:
-        abi.step2(abi.field(a2, [4, 4], 0, 2), abi.field(a2.clone(), [4, 4], 0, 2), g2, 1.0)
+        abi.step2(abi.field(a2, [4, 4], 0, 2, abi.DTYPE["f32"]), abi.field(a2.clone(), [4, 4], 0, 2, abi.DTYPE["f32"]),
+                  g2, 1.0)
     assert e.value.code == -2
