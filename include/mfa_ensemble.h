@@ -55,8 +55,15 @@ int mfa_ensemble_init_warm(mfa_ensemble* e, void* fa, void* fb, const unsigned c
                            const double* rho, const double* vel, void* stream);
 int mfa_ensemble_step(mfa_ensemble* e, const void* fin, void* fout, const unsigned char* masks,
                       void* stream);
-/* nsteps steps alternating fa->fb->fa...; use_graph captures them in one
- * hipGraph and replays it. */
+/* Two steps fin -> fout in one launch, bit-identical to two
+ * mfa_ensemble_step calls (the intermediate state stays in LDS; 32 x 8
+ * tiles).  No counterpart in the reference. */
+int mfa_ensemble_step2(mfa_ensemble* e, const void* fin, void* fout, const unsigned char* masks,
+                       void* stream);
+/* nsteps steps alternating fa->fb->fa...: two per launch for whole
+ * multiples of four steps, then single steps, so the state ends in fa when
+ * nsteps is even; use_graph captures the launches in one hipGraph and
+ * replays it. */
 int mfa_ensemble_run(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks, int nsteps,
                      int use_graph, void* stream);
 int mfa_ensemble_moments(mfa_ensemble* e, const void* f, const unsigned char* masks, double* rho,

@@ -24,25 +24,25 @@ __device__ __forceinline__ T ldnt(const T* p)
     else return *p;
 }
 
-// Wall ghost value in deviation form (see wall_value in kernels_common.hpp):
-// bounce-back d(x,opp)+contrib (w_opp = w_q), outflow d(x,q), pressure
-// outlet -d(x,opp) + 2 (feq_even - w_q).
-template <class L, int q, class T, class G>
-__device__ __forceinline__ float wall_value_dev(const T* __restrict__ fin, long long self, const View& v,
-                                                const G& g, int idx)
+// Wall ghost value in deviation form (see wall_value_at in
+// kernels_common.hpp): bounce-back d(x,opp)+contrib (w_opp = w_q), outflow
+// d(x,q), pressure outlet -d(x,opp) + 2 (feq_even - w_q).  self_pop(m)
+// returns the deviation of population m of the cell itself, as float.
+template <class L, int q, class G, class Ld>
+__device__ __forceinline__ float wall_value_dev_at(Ld&& self_pop, const G& g, int idx)
 {
     const int t = g.rtype[idx];
     constexpr int m = L::opp(q);
     if (t == MFA_BC_MOVING_WALL || t == MFA_BC_NO_SLIP || t == MFA_BC_INFLOW)
-        return to_f(fin[m * v.sq + self]) +
+        return self_pop(m) +
                bb_contrib<L, m, float>((float)g.rrho[idx], (float)g.ru[idx][0], (float)g.ru[idx][1],
                                        (float)g.ru[idx][2]);
-    if (t == MFA_BC_OUTFLOW) return to_f(fin[q * v.sq + self]);
+    if (t == MFA_BC_OUTFLOW) return self_pop(q);
     if (t == MFA_BC_PRESSURE_OUTLET) {
         float drho = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
 #pragma unroll
         for (int k = 0; k < L::Q; ++k) {
-            const float d = to_f(fin[k * v.sq + self]);
+            const float d = self_pop(k);
             drho += d;
             j0 += L::c(k, 0) * d;
             j1 += L::c(k, 1) * d;
@@ -53,16 +53,23 @@ __device__ __forceinline__ float wall_value_dev(const T* __restrict__ fin, long 
         const float uu = (j0 * j0 + j1 * j1 + j2 * j2) * inv_rho * inv_rho;
         const float even_dev =
             (float)L::w(q) * ((float)g.rrho[idx] * (1.0f + 4.5f * cu * cu - 1.5f * uu) - 1.0f);
-        return -to_f(fin[m * v.sq + self]) + 2.0f * even_dev;
+        return -self_pop(m) + 2.0f * even_dev;
     }
     return __builtin_nanf("");
 }
 
+template <class L, int q, class T, class G>
+__device__ __forceinline__ float wall_value_dev(const T* __restrict__ fin, long long self, const View& v,
+                                                const G& g, int idx)
+{
+    return wall_value_dev_at<L, q>([&](int k) { return to_f(fin[k * v.sq + self]); }, g, idx);
+}
+
 // Replace the pulled values whose source lies behind a non-periodic face by
-// the wall ghost value (cells on the domain edge only).
-template <class L, class T, class G>
-__device__ __forceinline__ void wall_fix_dev(float (&d)[L::Q], const T* __restrict__ fin, int x, int y,
-                                             int z, const G& g, const View& v)
+// the wall value (cells on the domain edge only); self_pop as above.
+template <class L, class G, class Ld>
+__device__ __forceinline__ void wall_fix_dev_at(float (&d)[L::Q], Ld&& self_pop, int x, int y, int z,
+                                                const G& g)
 {
     const int p[3] = {x, y, z};
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
@@ -74,7 +81,6 @@ __device__ __forceinline__ void wall_fix_dev(float (&d)[L::Q], const T* __restri
         edge = edge || lo[dd] || hi[dd];
     }
     if (edge) {
-        const long long self = x + y * v.sy + z * v.sz;
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             int code[3] = {0, 0, 0};
@@ -87,10 +93,18 @@ __device__ __forceinline__ void wall_fix_dev(float (&d)[L::Q], const T* __restri
             }
             if (wall) {
                 const int idx = (code[0] + 1) + 3 * (code[1] + 1) + (L::D == 3 ? 9 * (code[2] + 1) : 0);
-                d[q] = wall_value_dev<L, q>(fin, self, v, g, idx);
+                d[q] = wall_value_dev_at<L, q>(self_pop, g, idx);
             }
         });
     }
+}
+
+template <class L, class T, class G>
+__device__ __forceinline__ void wall_fix_dev(float (&d)[L::Q], const T* __restrict__ fin, int x, int y,
+                                             int z, const G& g, const View& v)
+{
+    const long long self = x + y * v.sy + z * v.sz;
+    wall_fix_dev_at<L>(d, [&](int k) { return to_f(fin[k * v.sq + self]); }, x, y, z, g);
 }
 
 // NT: non-temporal loads for 4-byte words (2-byte words always load cached).

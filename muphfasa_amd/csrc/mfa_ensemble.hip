@@ -136,6 +136,116 @@ __global__ __launch_bounds__(256) void k_batch_step(const T* __restrict__ fin, T
     }
 }
 
+// Two steps per launch (temporal blocking).  Pair tiles are PX x PY cells;
+// phase A computes the first step on the tile grown by one cell (grown
+// cells outside the lattice are not needed: the second step replaces their
+// pulls by wall values) into LDS, in the storage type (FP64, or the FP32
+// deviation exactly as a single step stores it); phase B pulls the second
+// step from LDS.  Wall values and obstacle bounce-back of the second step
+// read the cell's own intermediate populations from LDS, through the same
+// arithmetic as k_batch_step, so a pair equals two k_batch_step launches
+// bit for bit.  near: a solid cell lies within two cells of the tile.
+constexpr int kPairX = 32, kPairY = 8;
+constexpr int kPairEX = kPairX + 2, kPairEY = kPairY + 2, kPairEC = kPairEX * kPairEY;
+constexpr int kPairThreads = (kPairEC + 63) / 64 * 64;
+
+template <class L, class T, bool MASKED>
+__global__ __launch_bounds__(kPairThreads) void k_batch_step2(const T* __restrict__ fin, T* __restrict__ fout,
+                                                             const unsigned char* __restrict__ masks,
+                                                             const Tile* __restrict__ tiles, int ntiles,
+                                                             const LatInfo* __restrict__ lats,
+                                                             const BcTable* __restrict__ bc)
+{
+    constexpr bool F64 = std::is_same<T, double>::value;
+    __shared__ T lds[L::Q][kPairEC];
+    const int tid = threadIdx.x;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        const LatInfo li = lats[tl.lat];
+        const View v = pop_view<L>(li);
+        const T* a = fin + li.pop_off;
+        T* b = fout + li.pop_off;
+        const unsigned char* mask = MASKED ? masks + li.cell_off : nullptr;
+        const bool near = MASKED && tl.near;
+        const GeoRef g = geo_for<L>(li, bc);
+        if (tid < kPairEC) {                                   // phase A
+            const int x = tl.x0 - 1 + tid % kPairEX, y = tl.y0 - 1 + tid / kPairEX;
+            if (x >= 0 && x < li.nx && y >= 0 && y < li.ny) {
+                const long long self = x + (long long)y * v.sy;
+                if (near && mask[x + (long long)y * li.nx]) {
+                    for_q<L>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        if constexpr (F64) lds[q][tid] = L::w(q);
+                        else lds[q][tid] = from_f<T>(0.0f);
+                    });
+                } else if constexpr (F64) {
+                    double f[L::Q];
+                    pull<L, 0, true, false>(f, a, x, y, 0, g, v);
+                    if (near)
+                        obstacle_bb<L>(f, mask, x, y, li.nx, li.ny, [&](int m) { return a[m * v.sq + self]; });
+                    collide<L, false>(f, li.omega, 0.0, 0.0, 0.0, [&](auto qc, double out) {
+                        constexpr int q = decltype(qc)::value;
+                        lds[q][tid] = out;
+                    });
+                } else {
+                    float d[L::Q], o[L::Q];
+                    pull_dev<L, true>(d, a, x, y, 0, g, v);
+                    if (near)
+                        obstacle_bb<L>(d, mask, x, y, li.nx, li.ny, [&](int m) { return to_f(a[m * v.sq + self]); });
+                    collide_dev<L, false>(d, o, (float)li.omega, 0.f, 0.f, 0.f);
+                    for_q<L>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        lds[q][tid] = from_f<T>(o[q]);
+                    });
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < kPairX * kPairY) {                           // phase B
+            const int ox = tid % kPairX, oy = tid / kPairX;
+            const int x = tl.x0 + ox, y = tl.y0 + oy;
+            if (x < li.nx && y < li.ny) {
+                const long long self = x + (long long)y * v.sy;
+                const int c0 = (ox + 1) + kPairEX * (oy + 1);
+                if (near && mask[x + (long long)y * li.nx]) {
+                    for_q<L>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        if constexpr (F64) b[q * v.sq + self] = L::w(q);
+                        else b[q * v.sq + self] = from_f<T>(0.0f);
+                    });
+                } else if constexpr (F64) {
+                    double f[L::Q];
+                    for_q<L>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        f[q] = lds[q][c0 - L::c(q, 0) - kPairEX * L::c(q, 1)];
+                    });
+                    const int lo[3] = {x == 0, y == 0, 0}, hi[3] = {x == li.nx - 1, y == li.ny - 1, 0};
+                    if (lo[0] || lo[1] || hi[0] || hi[1]) {
+                        for_q<L>([&](auto qc) {
+                            constexpr int q = decltype(qc)::value;
+                            fix_wall_at<L, q>(f[q], [&](int m) { return lds[m][c0]; }, g, lo, hi);
+                        });
+                    }
+                    if (near)
+                        obstacle_bb<L>(f, mask, x, y, li.nx, li.ny, [&](int m) { return lds[m][c0]; });
+                    collide_store<L, false>(f, b, self, v, li.omega, 0.0, 0.0, 0.0);
+                } else {
+                    float d[L::Q];
+                    for_q<L>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        d[q] = to_f(lds[q][c0 - L::c(q, 0) - kPairEX * L::c(q, 1)]);
+                    });
+                    wall_fix_dev_at<L>(d, [&](int m) { return to_f(lds[m][c0]); }, x, y, 0, g);
+                    if (near)
+                        obstacle_bb<L>(d, mask, x, y, li.nx, li.ny, [&](int m) { return to_f(lds[m][c0]); });
+                    collide_store_dev<L, false>(d, b, self, v, (float)li.omega, 0.f, 0.f, 0.f);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // rho, u of every cell (FP64), pulled populations like updateMacroscopic.
 template <class L, class T, bool MASKED>
 __global__ __launch_bounds__(256) void k_batch_moments(const T* __restrict__ fin,
@@ -219,19 +329,20 @@ __global__ __launch_bounds__(256) void k_batch_init(T* __restrict__ fout, const 
     }
 }
 
-// near flag of each tile: does the tile or its one-cell rim hold a solid cell?
+// near flag of each tile: does the tile or its rim of `rim` cells hold a
+// solid cell?
 __global__ __launch_bounds__(256) void k_tile_near(const unsigned char* __restrict__ masks,
                                                    const Tile* __restrict__ tiles, int ntiles,
                                                    const LatInfo* __restrict__ lats, int tx, int ty,
-                                                   int* __restrict__ near)
+                                                   int rim, int* __restrict__ near)
 {
-    const int hx = tx + 2, hy = ty + 2;
+    const int hx = tx + 2 * rim, hy = ty + 2 * rim;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
         const LatInfo li = lats[tl.lat];
         int any = 0;
         for (int i = threadIdx.x; i < hx * hy; i += blockDim.x) {
-            const int x = tl.x0 - 1 + i % hx, y = tl.y0 - 1 + i / hx;
+            const int x = tl.x0 - rim + i % hx, y = tl.y0 - rim + i / hx;
             if (x >= 0 && x < li.nx && y >= 0 && y < li.ny &&
                 masks[li.cell_off + x + (long long)y * li.nx])
                 any = 1;
@@ -333,6 +444,11 @@ struct mfa_ensemble {
     // near flag of every tile of every lattice (active or not), in the order
     // rebuild_tiles enumerates them; empty until masks have been seen
     std::vector<int> near;
+    // the same for the pair tiles (kPairX x kPairY, two-cell rim)
+    std::vector<Tile> ptiles;
+    std::vector<int> pnear;
+    Tile* d_ptiles = nullptr;
+    int nptiles = 0, ptiles_cap = 0;
 };
 
 static thread_local std::string g_ens_err;
@@ -369,17 +485,33 @@ static int rebuild_tiles(mfa_ensemble* e)
     }
     if (e->ntiles)
         ENS_HIP(hipMemcpy(e->d_tiles, e->tiles.data(), sizeof(Tile) * e->ntiles, hipMemcpyHostToDevice));
+    e->ptiles.clear();
+    k = 0;
+    for (int l = 0; l < e->nlat; ++l)
+        for (int y0 = 0; y0 < e->lats[l].ny; y0 += kPairY)
+            for (int x0 = 0; x0 < e->lats[l].nx; x0 += kPairX, ++k)
+                if (e->active[l])
+                    e->ptiles.push_back(Tile{l, x0, y0, k < e->pnear.size() ? e->pnear[k] : near_default});
+    e->nptiles = (int)e->ptiles.size();
+    if (e->nptiles > e->ptiles_cap) {
+        if (e->d_ptiles) (void)hipFree(e->d_ptiles);
+        e->ptiles_cap = e->nptiles;
+        ENS_HIP(hipMalloc((void**)&e->d_ptiles, sizeof(Tile) * e->ptiles_cap));
+    }
+    if (e->nptiles)
+        ENS_HIP(hipMemcpy(e->d_ptiles, e->ptiles.data(), sizeof(Tile) * e->nptiles, hipMemcpyHostToDevice));
     return MFA_OK;
 }
 
-// Near flags of every tile from the masks (synchronises the stream), then the
-// active tile list again with the flags in it.
-static int compute_near(mfa_ensemble* e, const unsigned char* masks, hipStream_t st)
+// Near flags of every tile of one tiling (tx x ty tiles, rim cells) from the
+// masks; synchronises the stream.
+static int near_flags(mfa_ensemble* e, const unsigned char* masks, int tx, int ty, int rim,
+                      std::vector<int>& out, hipStream_t st)
 {
     std::vector<Tile> all;
     for (int l = 0; l < e->nlat; ++l)
-        for (int y0 = 0; y0 < e->lats[l].ny; y0 += e->ty)
-            for (int x0 = 0; x0 < e->lats[l].nx; x0 += e->tx) all.push_back(Tile{l, x0, y0, 1});
+        for (int y0 = 0; y0 < e->lats[l].ny; y0 += ty)
+            for (int x0 = 0; x0 < e->lats[l].nx; x0 += tx) all.push_back(Tile{l, x0, y0, 1});
     Tile* d_all = nullptr;
     int* d_near = nullptr;
     const int n = (int)all.size();
@@ -392,7 +524,7 @@ static int compute_near(mfa_ensemble* e, const unsigned char* masks, hipStream_t
     hipError_t err = hipMemcpyAsync(d_all, all.data(), sizeof(Tile) * n, hipMemcpyHostToDevice, st);
     if (err == hipSuccess) {
         hipLaunchKernelGGL(k_tile_near, dim3(n < 4096 ? n : 4096), dim3(256), 0, st, masks, d_all, n,
-                           e->d_lats, e->tx, e->ty, d_near);
+                           e->d_lats, tx, ty, rim, d_near);
         err = hipGetLastError();
     }
     if (err == hipSuccess) err = hipMemcpyAsync(near.data(), d_near, sizeof(int) * n, hipMemcpyDeviceToHost, st);
@@ -400,7 +532,17 @@ static int compute_near(mfa_ensemble* e, const unsigned char* masks, hipStream_t
     (void)hipFree(d_all);
     (void)hipFree(d_near);
     if (err != hipSuccess) return ens_fail(MFA_ERR_HIP, std::string("ensemble: near flags: ") + hipGetErrorString(err));
-    e->near.swap(near);
+    out.swap(near);
+    return MFA_OK;
+}
+
+// Near flags of the step tiles (one-cell rim) and of the pair tiles
+// (two-cell rim), then the active tile lists again with the flags in them.
+static int compute_near(mfa_ensemble* e, const unsigned char* masks, hipStream_t st)
+{
+    int rc = near_flags(e, masks, e->tx, e->ty, 1, e->near, st);
+    if (rc == MFA_OK) rc = near_flags(e, masks, kPairX, kPairY, 2, e->pnear, st);
+    if (rc != MFA_OK) return rc;
     return rebuild_tiles(e);
 }
 
@@ -478,6 +620,7 @@ void mfa_ensemble_destroy(mfa_ensemble* e)
     if (e->d_tiles) (void)hipFree(e->d_tiles);
     if (e->d_bc) (void)hipFree(e->d_bc);
     if (e->d_part) (void)hipFree(e->d_part);
+    if (e->d_ptiles) (void)hipFree(e->d_ptiles);
     delete e;
 }
 
@@ -569,20 +712,52 @@ int mfa_ensemble_step(mfa_ensemble* e, const void* fin, void* fout, const unsign
     });
 }
 
+int mfa_ensemble_step2(mfa_ensemble* e, const void* fin, void* fout, const unsigned char* masks,
+                       void* stream)
+{
+    if (!e || !fin || !fout || fin == fout) return ens_fail(MFA_ERR_INVALID, "bad populations");
+    if (e->masked && !masks) return ens_fail(MFA_ERR_INVALID, "ensemble was created with masks");
+    if (!e->nptiles) return MFA_OK;
+    return with_type(e->dtype, [&](auto tv) {
+        using T = decltype(tv);
+        using L = Lattice<MFA_D2Q9>;
+        const int grid = e->nptiles < 65535 ? e->nptiles : 65535;
+        if (e->masked)
+            hipLaunchKernelGGL((k_batch_step2<L, T, true>), dim3(grid), dim3(kPairThreads), 0, (hipStream_t)stream, (const T*)fin, (T*)fout, masks, e->d_ptiles, e->nptiles, e->d_lats, e->d_bc);
+        else
+            hipLaunchKernelGGL((k_batch_step2<L, T, false>), dim3(grid), dim3(kPairThreads), 0, (hipStream_t)stream, (const T*)fin, (T*)fout, masks, e->d_ptiles, e->nptiles, e->d_lats, e->d_bc);
+        ENS_HIP(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+// The launches of nsteps steps: pairs (fa->fb, fb->fa) for whole multiples
+// of four steps, so that the state ends in fa after an even number of steps
+// and in fb after an odd one, then single steps for the rest.
+static int run_launches(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks, int nsteps,
+                        void* stream)
+{
+    const int quads = nsteps / 4;
+    for (int i = 0; i < quads; ++i) {
+        int rc = mfa_ensemble_step2(e, fa, fb, masks, stream);
+        if (rc == MFA_OK) rc = mfa_ensemble_step2(e, fb, fa, masks, stream);
+        if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps - 4 * quads; ++s) {
+        const int rc = mfa_ensemble_step(e, (s & 1) ? fb : fa, (s & 1) ? fa : fb, masks, stream);
+        if (rc) return rc;
+    }
+    return MFA_OK;
+}
+
 int mfa_ensemble_run(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks, int nsteps,
                      int use_graph, void* stream)
 {
-    // nsteps launches alternating fa->fb, fb->fa (the state ends in fa when
-    // nsteps is even).  With use_graph the launches are captured once into a
-    // hipGraph on a private stream and replayed.
+    // nsteps steps, mostly two per launch (run_launches); the state ends in
+    // fa when nsteps is even.  With use_graph the launches are captured once
+    // into a hipGraph on a private stream and replayed.
     if (!e || nsteps < 0) return ens_fail(MFA_ERR_INVALID, "bad arguments");
-    if (!use_graph || nsteps == 0) {
-        for (int s = 0; s < nsteps; ++s) {
-            const int rc = mfa_ensemble_step(e, (s & 1) ? fb : fa, (s & 1) ? fa : fb, masks, stream);
-            if (rc) return rc;
-        }
-        return MFA_OK;
-    }
+    if (!use_graph || nsteps == 0) return run_launches(e, fa, fb, masks, nsteps, stream);
     hipStream_t cap;
     ENS_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     hipEvent_t ev;
@@ -592,9 +767,7 @@ int mfa_ensemble_run(mfa_ensemble* e, void* fa, void* fb, const unsigned char* m
     hipGraph_t graph;
     hipGraphExec_t exec;
     ENS_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    int rc = MFA_OK;
-    for (int s = 0; s < nsteps && rc == MFA_OK; ++s)
-        rc = mfa_ensemble_step(e, (s & 1) ? fb : fa, (s & 1) ? fa : fb, masks, cap);
+    const int rc = run_launches(e, fa, fb, masks, nsteps, cap);
     const hipError_t ce = hipStreamEndCapture(cap, &graph);
     if (rc != MFA_OK) return rc;
     if (ce != hipSuccess) return ens_fail(MFA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));

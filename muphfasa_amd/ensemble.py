@@ -43,11 +43,13 @@ def lib():
         L.mfa_ensemble_init.argtypes = [vp, vp, vp, vp, vp, vp]
         L.mfa_ensemble_init_warm.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.mfa_ensemble_step.argtypes = [vp, vp, vp, vp, vp]
+        L.mfa_ensemble_step2.argtypes = [vp, vp, vp, vp, vp]
         L.mfa_ensemble_run.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
         L.mfa_ensemble_moments.argtypes = [vp, vp, vp, vp, vp, vp]
         L.mfa_ensemble_residual.argtypes = [vp, vp, vp, vp, dp, vp]
         for n in ("mfa_ensemble_info", "mfa_ensemble_set_active", "mfa_ensemble_set_grid",
-                  "mfa_ensemble_init", "mfa_ensemble_init_warm", "mfa_ensemble_step", "mfa_ensemble_run",
+                  "mfa_ensemble_init", "mfa_ensemble_init_warm", "mfa_ensemble_step", "mfa_ensemble_step2",
+                  "mfa_ensemble_run",
                   "mfa_ensemble_moments", "mfa_ensemble_residual"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
@@ -150,6 +152,12 @@ class Ensemble:
         nt = ctypes.c_int()
         _check(lib().mfa_ensemble_info(self.h, None, None, None, None, ctypes.byref(nt)))
         self.ntiles = nt.value
+
+    def step(self, pair=False):
+        """One launch fa -> fb, then swap: one step, or two with pair=True."""
+        fn = lib().mfa_ensemble_step2 if pair else lib().mfa_ensemble_step
+        _check(fn(self.h, _p(self.fa), _p(self.fb), _p(self.mask), self.stream()))
+        self.fa, self.fb = self.fb, self.fa
 
     def run(self, nsteps, graph=True):
         """nsteps steps (state stays in fa when nsteps is even)."""
