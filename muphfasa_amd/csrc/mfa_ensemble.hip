@@ -189,7 +189,8 @@ __global__ __launch_bounds__(kPairThreads) void k_batch_step2(const T* __restric
                     });
                 } else {
                     float d[L::Q], o[L::Q];
-                    pull_dev<L, true>(d, a, x, y, 0, g, v);
+                    // cached loads: the rims of neighbouring tiles re-read these rows
+                    pull_dev<L, true, T, GeoRef, false>(d, a, x, y, 0, g, v);
                     if (near)
                         obstacle_bb<L>(d, mask, x, y, li.nx, li.ny, [&](int m) { return to_f(a[m * v.sq + self]); });
                     collide_dev<L, false>(d, o, (float)li.omega, 0.f, 0.f, 0.f);
