@@ -6,7 +6,7 @@ inference run entirely on the engine's hand-written BF16 MFMA kernels
 (include/mfa_surrogate.h) with FP32 accumulation and an FP32 Adam master
 copy; torch only owns the device memory.  The solver then starts from
 f = feq(rho_pred, u_pred) and runs to the same steady-state criterion as a
-cold start, so the plain LBM stays both the fallback and the judge.
+cold start, so the plain LBM stays both the fallback and the arbiter of convergence.
 
 Network (NHWC, 3x3 convolutions, ReLU except the head, additive skips):
   x0[8] -> e1[32] -> e1b[32] -s2-> d1[64] -> e2[64] -s2-> d2[128] -> e3[128]

@@ -284,7 +284,7 @@ def main():
     p.add_argument("--batch", type=int, default=16)
     p.add_argument("--lr", type=float, default=2e-3)
     p.add_argument("--train-seconds", type=float, default=None)
-    p.add_argument("--weights", default="gpurun_out/warmstart_weights.npz")
+    p.add_argument("--weights", default="warmstart_weights.npz")
     p.add_argument("--n-eval", type=int, default=256)
     p.add_argument("--eval-max-steps", type=int, default=200000)
     p.add_argument("--tol", type=float, default=1e-8)
