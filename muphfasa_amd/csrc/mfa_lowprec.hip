@@ -438,16 +438,22 @@ static void shape(const int n[3], dim3& grid, dim3& block)
 // Launch configuration of the reduced-storage step, per lattice and storage
 // type (index dtype - MFA_F32).  cells_per_thread 2 selects k_step_dev_x2
 // (block_x then counts threads, i.e. cell pairs); nt_loads applies to it.
-// Defaults: the autotuner's picks on MI355X for D3Q27 384^3 in the
-// row-interleaved order (FP32: pairs in 64x4 blocks, 5.60 TB/s; FP16 / BF16:
-// one cell per thread in 128x1 blocks, 5.70 TB/s), used for every lattice.
+// Defaults: the autotuner's picks on MI355X in the row-interleaved order.
+// D3Q27 384^3 (also used for D2Q9): FP32 pairs in 64x4 blocks, 5.60 TB/s;
+// FP16 / BF16 one cell per thread in 128x1 blocks, 5.70 TB/s.  D3Q19 256^3
+// (also used for D3Q15): FP16 / BF16 one cell per thread in 256x1 blocks,
+// 0.216 ms against 0.239 ms with 128x1.
 static std::mutex g_dev_mu;
-static const mfa_launch k_dev_default[3] = {{64, 4, 2, 0}, {128, 1, 1, 0}, {128, 1, 1, 0}};
+static const mfa_launch k_dev_default[4][3] = {
+    {{64, 4, 2, 0}, {128, 1, 1, 0}, {128, 1, 1, 0}},
+    {{64, 4, 2, 0}, {256, 1, 1, 0}, {256, 1, 1, 0}},
+    {{64, 4, 2, 0}, {256, 1, 1, 0}, {256, 1, 1, 0}},
+    {{64, 4, 2, 0}, {128, 1, 1, 0}, {128, 1, 1, 0}}};
 static mfa_launch g_dev_cfg[4][3] = {
-    {k_dev_default[0], k_dev_default[1], k_dev_default[2]},
-    {k_dev_default[0], k_dev_default[1], k_dev_default[2]},
-    {k_dev_default[0], k_dev_default[1], k_dev_default[2]},
-    {k_dev_default[0], k_dev_default[1], k_dev_default[2]}};
+    {k_dev_default[0][0], k_dev_default[0][1], k_dev_default[0][2]},
+    {k_dev_default[1][0], k_dev_default[1][1], k_dev_default[1][2]},
+    {k_dev_default[2][0], k_dev_default[2][1], k_dev_default[2][2]},
+    {k_dev_default[3][0], k_dev_default[3][1], k_dev_default[3][2]}};
 
 static mfa_launch dev_cfg(int lat, int dtype)
 {
@@ -658,7 +664,7 @@ int mfa_set_launch_dev(int lattice, int dtype, const mfa_launch* cfg)
 {
     if (lattice < MFA_D2Q9 || lattice > MFA_D3Q27 || dtype < MFA_F32 || dtype > MFA_BF16DEV)
         return fail(MFA_ERR_INVALID, "mfa_set_launch_dev: unknown lattice or storage type");
-    const mfa_launch c = cfg ? *cfg : k_dev_default[dtype - MFA_F32];
+    const mfa_launch c = cfg ? *cfg : k_dev_default[lattice][dtype - MFA_F32];
     const bool bx_ok = c.block_x == 64 || c.block_x == 128 || c.block_x == 256;
     if (!bx_ok || c.block_y < 1 || c.block_x * c.block_y > 256 ||
         (c.cells_per_thread != 1 && c.cells_per_thread != 2) || (c.nt_loads != 0 && c.nt_loads != 1))
