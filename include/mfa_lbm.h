@@ -122,14 +122,14 @@ int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g,
  * to 1.3 times and writes fout once instead of reading and writing twice.
  * For callers that advance two steps with nothing in between, e.g. two
  * SimulationImpl::advance() calls (src/Simulation/SimulationSetup.cpp:119-139)
- * between outputs.  D2Q9 boxes in FP64, periodic or with walls of every kind
- * (the second step's ghost values come from the intermediate state); fully
- * periodic 3-D boxes (D3Q15, D3Q19, D3Q27) in FP64 or MFA_F32 storage (the
- * intermediate state rounded to FP32 as a single step stores it).  Anything
- * else returns MFA_ERR_UNSUPPORTED and the caller uses mfa_step: 3-D boxes
- * with walls, 2-D reduced storage, and FP16 / BF16 storage, where the pass
- * measured slower than two single steps (the words halve the bytes but not
- * the arithmetic, which then bounds the pass).  No counterpart in the
+ * between outputs.  D2Q9 boxes, periodic or with walls of every kind (the
+ * second step's ghost values come from the intermediate state), and fully
+ * periodic 3-D boxes (D3Q15, D3Q19, D3Q27), each in FP64 or MFA_F32 storage
+ * (the intermediate state rounded to FP32 as a single step stores it).
+ * Anything else returns MFA_ERR_UNSUPPORTED and the caller uses mfa_step:
+ * 3-D boxes with walls, and FP16 / BF16 storage, where the pass measured
+ * slower than two single steps (the words halve the bytes but not the
+ * arithmetic, which then bounds the pass).  No counterpart in the
  * reference. */
 int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g,
               double omega, const double* force, void* stream);

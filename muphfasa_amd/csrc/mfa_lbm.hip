@@ -816,10 +816,10 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
     if (fin->dtype != fout->dtype) return fail(MFA_ERR_INVALID, "mfa_step2: fin/fout storage types differ");
     if ((rc = check_2d(fin, lat))) return rc;
     const Geo geo = geo_of(*fin, g, lat);
+    if (lattice_dim(lat) == 2 && fin->dtype != MFA_F64)
+        return step2_dev(fin, fout, geo, lat, omega, force, 1, (hipStream_t)stream);
     if (lattice_dim(lat) == 2) {
         // D2Q9, FP64, periodic or with walls; 64 x 4 tiles
-        if (fin->dtype != MFA_F64)
-            return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: 2-D boxes in FP64 only (use mfa_step)");
         using L = Lattice<MFA_D2Q9>;
         using T = Step2Tile<64, 4>;
         const View vi = view_of(*fin), vo = view_of(*fout);

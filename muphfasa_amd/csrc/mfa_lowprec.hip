@@ -524,6 +524,59 @@ int step_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doubl
     });
 }
 
+// Two steps per pass on a 2-D box of reduced storage, periodic or with
+// walls: k_step2_2d (mfa_lbm.hip) for stored deviations.  The intermediate
+// state is rounded to the storage type before it enters LDS, and the second
+// step's wall values read the cell's own intermediate deviations from LDS
+// through the arithmetic of wall_fix_dev, so the pair equals two k_step_dev
+// launches bit for bit.
+template <class L, bool FORCED, bool WALLS, class T, class TL>
+__global__ __launch_bounds__(TL::THREADS) void k_step2_2d_dev(const T* __restrict__ fin, T* __restrict__ fout,
+                                                              const Geo g, const View vi, const View vo,
+                                                              const float omega, const float F0,
+                                                              const float F1)
+{
+    using U = typename Word<T>::U;
+    __shared__ U lds[L::Q][TL::EC];
+    const int tid = threadIdx.x;
+    const int nx = g.n[0], ny = g.n[1];
+    const unsigned nb = gridDim.x * gridDim.y;
+    const unsigned b = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned per = nb / kXcds, rem = nb % kXcds, k = b % kXcds;
+    const unsigned t = k * per + (k < rem ? k : rem) + b / kXcds;
+    const int X0 = (int)(t % gridDim.x) * TL::TX, Y0 = (int)(t / gridDim.x) * TL::TY;
+    if (tid < TL::EC) {
+        int gx = X0 - 1 + tid % TL::EX, gy = Y0 - 1 + tid / TL::EX;
+        bool in = true;
+        if (g.per[0]) gx = (gx % nx + nx) % nx;
+        else in = in && gx >= 0 && gx < nx;
+        if (g.per[1]) gy = (gy % ny + ny) % ny;
+        else in = in && gy >= 0 && gy < ny;
+        if (in) {
+            float d[L::Q], o[L::Q];
+            pull_dev<L, WALLS, T, Geo, false>(d, fin, gx, gy, 0, g, vi);
+            collide_dev<L, FORCED>(d, o, omega, F0, F1, 0.f);
+            for_q<L>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                lds[q][tid] = f_to_word<T>(o[q]);
+            });
+        }
+    }
+    __syncthreads();
+    if (tid >= TL::TX * TL::TY) return;
+    const int ox = tid % TL::TX, oy = tid / TL::TX;
+    const int x = X0 + ox, y = Y0 + oy;
+    if (x >= nx || y >= ny) return;
+    const int c0 = (ox + 1) + TL::EX * (oy + 1);
+    float d[L::Q];
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        d[q] = word_to_f<T>(lds[q][c0 - L::c(q, 0) - TL::EX * L::c(q, 1)]);
+    });
+    if constexpr (WALLS) wall_fix_dev_at<L>(d, [&](int m) { return word_to_f<T>(lds[m][c0]); }, x, y, 0, g);
+    collide_store_dev<L, FORCED>(d, fout, x + (long long)y * vo.sy, vo, omega, F0, F1, 0.f);
+}
+
 // Two steps in one pass (k_step2_dev); the caller has checked a fully
 // periodic 3-D box.  TZ: z-planes per block.  FP32 storage only: with FP16
 // and BF16 words the pass moves half the bytes but runs the same arithmetic,
@@ -539,6 +592,25 @@ int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doub
     if (fin->dtype != MFA_F32)
         return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: reduced storage in FP32 only (use mfa_step)");
     const View vi{fin->sy, fin->sz, fin->sq}, vo{fout->sy, fout->sz, fout->sq};
+    if (lat == MFA_D2Q9) {
+        using L = Lattice<MFA_D2Q9>;
+        using T = float;
+        using TL = Step2DevTile<64, 4, 3>;
+        const dim3 grid((g.n[0] + TL::TX - 1) / TL::TX, (g.n[1] + TL::TY - 1) / TL::TY);
+        const float om = (float)omega;
+        const float F0 = force ? (float)force[0] : 0.f, F1 = force ? (float)force[1] : 0.f;
+        const T* a = (const T*)fin->data;
+        T* b = (T*)fout->data;
+        if (force) {
+            if (g.walls) hipLaunchKernelGGL((k_step2_2d_dev<L, true, true, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g, vi, vo, om, F0, F1);
+            else hipLaunchKernelGGL((k_step2_2d_dev<L, true, false, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g, vi, vo, om, F0, F1);
+        } else {
+            if (g.walls) hipLaunchKernelGGL((k_step2_2d_dev<L, false, true, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g, vi, vo, om, F0, F1);
+            else hipLaunchKernelGGL((k_step2_2d_dev<L, false, false, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g, vi, vo, om, F0, F1);
+        }
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    }
     // in-plane byte offsets are 32-bit in the kernel
     for (const mfa_field* f : {fin, (const mfa_field*)fout})
         if (((long long)(f->n[0] - 1) + (long long)(f->n[1] - 1) * f->sy + 1) * 4 > 0xffffffffLL)

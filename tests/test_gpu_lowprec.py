@@ -269,3 +269,36 @@ def test_step2_reduced_refuses_2d_walls_and_16bit(dev):
         abi.step2(abi.field(b, [4, 4, 4], 0, 3, abi.DTYPE["f32"]),
                   abi.field(b.clone(), [4, 4, 4], 0, 3, abi.DTYPE["f32"]), g3, 1.0)
     assert e.value.code == -2
+
+
+@pytest.mark.parametrize("bc", ["periodic", "lid", "xper_walls", "inflow_pressure", "inflow_outflow"])
+@pytest.mark.parametrize("n", [[13, 7], [130, 9], [1, 5], [70, 33]])
+@pytest.mark.parametrize("forced", [False, True])
+def test_step2_2d_f32_equals_two_single_steps(dev, bc, n, forced):
+    """FP32-storage mfa_step2 on D2Q9 boxes, periodic or with walls (second-
+    step wall values from the intermediate deviations in LDS), against two
+    mfa_step calls, bit for bit, in both population orders."""
+    import torch
+    from tests.gpu_util import bc_tables
+    L = O.Lattice("D2Q9")
+    did = abi.DTYPE["f32"]
+    print(f"case step2_2d_f32 bc={bc} n={n} forced={forced}", flush=True)
+    rng = np.random.default_rng(31)
+    f64 = L.w[:, None, None, None] * (1 + 0.05 * rng.standard_normal((L.Q, 1, n[1], n[0])))
+    src = torch.from_numpy(f64).to(dev)
+    a = torch.empty((L.Q, 1, n[1], n[0]), dtype=torch.float32, device=dev)
+    abi.convert(abi.field(src, n, 0, 2), abi.field(a, n, 0, 2, did), 0)
+    g, _, _ = bc_tables(L, bc)
+    F = [2e-4, -1e-4] if forced else None
+    omega = 1.0 / 0.6
+    mid, two = torch.empty_like(a), torch.empty_like(a)
+    abi.step(abi.field(a, n, 0, 2, did), abi.field(mid, n, 0, 2, did), g, omega, F)
+    abi.step(abi.field(mid, n, 0, 2, did), abi.field(two, n, 0, 2, did), g, omega, F)
+    pair = torch.full_like(a, float("nan"))
+    abi.step2(abi.field(a, n, 0, 2, did), abi.field(pair, n, 0, 2, did), g, omega, F)
+    ar = a.permute(1, 2, 0, 3).contiguous()
+    pr = torch.full_like(ar, float("nan"))
+    abi.step2(abi.field(ar, n, 0, 2, did, layout="row"), abi.field(pr, n, 0, 2, did, layout="row"), g, omega, F)
+    torch.cuda.synchronize()
+    assert torch.equal(pair.view(torch.int32), two.view(torch.int32))
+    assert torch.equal(pr.permute(2, 0, 1, 3).contiguous().view(torch.int32), two.view(torch.int32))

@@ -324,7 +324,7 @@ def test_step2_2d_equals_two_oracle_steps(dev, bc, si, forced):
         assert np.array_equal(got, O.valid(L, expect)), layout
 
 
-def test_step2_refuses_3d_walls_and_2d_reduced(dev):
+def test_step2_refuses_3d_walls_and_2d_16bit(dev):
     import torch
     L = O.Lattice("D3Q19")
     g, _, _ = bc_tables(L, "lid")
@@ -335,8 +335,8 @@ def test_step2_refuses_3d_walls_and_2d_reduced(dev):
     assert e.value.code == -2
     L2 = O.Lattice("D2Q9")
     g2, _, _ = bc_tables(L2, "periodic")
-    a2 = torch.zeros((9, 1, 4, 4), dtype=torch.float32, device=dev)
+    a2 = torch.zeros((9, 1, 4, 4), dtype=torch.float16, device=dev)
     with pytest.raises(abi.MfaError) as e:
-        abi.step2(abi.field(a2, [4, 4], 0, 2, abi.DTYPE["f32"]), abi.field(a2.clone(), [4, 4], 0, 2, abi.DTYPE["f32"]),
-                  g2, 1.0)
+        abi.step2(abi.field(a2, [4, 4], 0, 2, abi.DTYPE["f16dev"]),
+                  abi.field(a2.clone(), [4, 4], 0, 2, abi.DTYPE["f16dev"]), g2, 1.0)
     assert e.value.code == -2
