@@ -56,7 +56,7 @@ int mfa_ensemble_init_warm(mfa_ensemble* e, void* fa, void* fb, const unsigned c
 int mfa_ensemble_step(mfa_ensemble* e, const void* fin, void* fout, const unsigned char* masks,
                       void* stream);
 /* Two steps fin -> fout in one launch, bit-identical to two
- * mfa_ensemble_step calls (the intermediate state stays in LDS; 32 x 8
+ * mfa_ensemble_step calls (the intermediate state stays in LDS; 32 x 16
  * tiles).  No counterpart in the reference. */
 int mfa_ensemble_step2(mfa_ensemble* e, const void* fin, void* fout, const unsigned char* masks,
                        void* stream);

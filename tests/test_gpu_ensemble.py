@@ -246,7 +246,7 @@ def test_step2_equals_two_steps(dev, dt, masked):
     """mfa_ensemble_step2 (two steps per launch, intermediate state in LDS,
     second-step wall values and obstacle bounce-back from it) against two
     mfa_ensemble_step launches, bit for bit: lattices narrower than the
-    32 x 8 pair tile, not a multiple of it, one cell wide, with obstacles at
+    32 x 16 pair tile, not a multiple of it, one cell wide, with obstacles at
     random cells (also on tile edges)."""
     import torch
     sizes = [(40, 23), (64, 64), (1, 5), (97, 33), (33, 1), (70, 17)]
