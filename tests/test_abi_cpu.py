@@ -170,3 +170,25 @@ def test_level_layout_checks_need_no_gpu():
         with pytest.raises(abi.MfaError) as e:
             abi.Level(D2, dom, [1, 1], lo, n)
         assert e.value.code == -1, (dom, lo, n)
+
+
+def test_level_chopping_tiles_the_domain():
+    """muphfasa_amd.level.chop / boxes: the max_grid_size layout covers the
+    domain exactly, every box is at most max_grid cells per side, and the
+    pieces of one dimension differ by at most one cell."""
+    from muphfasa_amd import level
+    for n, mg in ((256, 64), (100, 64), (7, 64), (130, 32), (1, 1)):
+        pieces = level.chop(n, mg)
+        assert sum(m for _, m in pieces) == n
+        assert all(m <= mg for _, m in pieces)
+        assert max(m for _, m in pieces) - min(m for _, m in pieces) <= 1
+        assert [lo for lo, _ in pieces] == [sum(m for _, m in pieces[:i]) for i in range(len(pieces))]
+    lo, n = level.boxes([40, 30, 20], 16)
+    cover = set()
+    for l, m in zip(lo, n):
+        for z in range(l[2], l[2] + m[2]):
+            for y in range(l[1], l[1] + m[1]):
+                for x in range(l[0], l[0] + m[0]):
+                    assert (x, y, z) not in cover
+                    cover.add((x, y, z))
+    assert len(cover) == 40 * 30 * 20
