@@ -38,14 +38,17 @@ typedef struct mfa_level mfa_level;
  * inside FabArrayBase). */
 int mfa_level_create(int lattice, const int domain_n[3], const int periodic[3], int nboxes,
                      const int* box_lo, const int* box_n, mfa_level** out);
+/* Frees the level's device memory: call it once no work that uses the level
+ * is left on any stream. */
 int mfa_level_destroy(mfa_level* lv);
 
 /* Number of ghost cells the level's fill_boundary writes (all boxes). */
 long long mfa_level_ghost_cells(const mfa_level* lv);
 
 /* Bind the fields of all boxes (boxes[i]: box i, FP64, ngrow >= 1, extents
- * equal to the box's) to slot 0..3.  Copies their views to device memory,
- * synchronously; rebind after reallocating. */
+ * equal to the box's) to slot 0..3.  Copies their views to device memory
+ * with a synchronous copy, so rebind a slot (after reallocating) only while
+ * no enqueued level call that uses that slot is still to run. */
 int mfa_level_bind(mfa_level* lv, int slot, const mfa_field* boxes);
 
 /* FillBoundary(periodicity): every ghost cell of every box that lies in the
