@@ -36,6 +36,12 @@ int mfa_sim_info(mfa_sim* s, int* lattice, int n[3], double* omega, double force
 /* host copies of the valid cells, component-major [comp][z][y][x] */
 int mfa_sim_copy_f(mfa_sim* s, double* f);
 int mfa_sim_copy_macro(mfa_sim* s, double* rho, double* vel);
+/* Health of the current populations (mfa_reduce MFA_RED_HEALTH): out[0] =
+ * cells with a non-finite population, out[1] = cells whose density is finite
+ * and <= 0.  mfa_run_main checks it after every run of steps and stops with
+ * an error, before writing output, when either is non-zero.  No counterpart
+ * in the reference (its solver has no divergence check). */
+int mfa_sim_health(mfa_sim* s, double out[2]);
 
 /* Host-only (no GPU): parse inputs and run the UnitConverter chain.
  * out[0..11] = dx, dt, u_lat, nu_lat, tau, omega, Re, nu_phys, Ma,

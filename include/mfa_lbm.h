@@ -58,7 +58,9 @@ enum { MFA_BC_MOVING_WALL = 0, MFA_BC_NO_SLIP = 1, MFA_BC_INFLOW = 2,
 /* reductions */
 enum { MFA_RED_MASS = 0,      /* sum over valid cells and q of f      -> 1 value  */
        MFA_RED_MOMENTUM = 1,  /* sum over valid cells of sum_q c_q f -> D values */
-       MFA_RED_COMP_SUM = 2   /* per-component sum over valid cells  -> ncomp    */
+       MFA_RED_COMP_SUM = 2,  /* per-component sum over valid cells  -> ncomp    */
+       MFA_RED_HEALTH = 3     /* populations: [cells with a non-finite f_q,
+                                 cells with finite rho = sum_q f_q <= 0]  -> 2 */
 };
 
 /* Non-owning view of one box of a field (replaces amrex::Array4). */

@@ -18,7 +18,7 @@ LATTICE_DQ = {0: (2, 9), 1: (3, 15), 2: (3, 19), 3: (3, 27)}
 BC = {"MovingWall": 0, "NoSlip": 1, "Inflow": 2, "Outflow": 3, "PressureOutlet": 4,
       "Periodic": 5}
 DTYPE = {"f64": 0, "f32": 1, "f16dev": 2, "bf16dev": 3}
-RED = {"mass": 0, "momentum": 1, "comp_sum": 2}
+RED = {"mass": 0, "momentum": 1, "comp_sum": 2, "health": 3}
 
 EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_step2", "mfa_pull_collide",
            "mfa_fill_periodic", "mfa_apply_bc", "mfa_moments", "mfa_init_equilibrium",

@@ -228,7 +228,10 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_f(const double* __restri
                                                           const View v, int ncomp,
                                                           double* __restrict__ part)
 {
-    constexpr int NV = (WHAT == MFA_RED_MASS) ? 1 : (WHAT == MFA_RED_MOMENTUM ? 3 : kRedMaxOut);
+    constexpr int NV = (WHAT == MFA_RED_MASS)       ? 1
+                       : (WHAT == MFA_RED_MOMENTUM) ? 3
+                       : (WHAT == MFA_RED_HEALTH)   ? 2
+                                                    : kRedMaxOut;
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
@@ -242,6 +245,18 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_f(const double* __restri
         const long long self = x + y * v.sy + z * v.sz;
         if constexpr (WHAT == MFA_RED_COMP_SUM) {
             for (int k = 0; k < ncomp && k < kRedMaxOut; ++k) acc[k] += f[k * v.sq + self];
+        } else if constexpr (WHAT == MFA_RED_HEALTH) {
+            // counts of cells (exact in FP64): a non-finite population, or a
+            // finite density that is not positive
+            double rho = 0.0;
+            bool finite = true;
+            for_q<L>([&](auto qc) {
+                const double fq = f[decltype(qc)::value * v.sq + self];
+                finite = finite && __builtin_isfinite(fq);
+                rho += fq;
+            });
+            acc[0] += finite ? 0.0 : 1.0;
+            acc[1] += (finite && rho <= 0.0) ? 1.0 : 0.0;
         } else {
             for_q<L>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
@@ -1099,7 +1114,7 @@ int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void*
     int nv = 0;
     if (f->dtype != MFA_F64) {
         if ((rc = reduce_dev(f, geo, lattice, what, ws, kRedBlocks, st))) return rc;
-        nv = (what == MFA_RED_MASS) ? 1 : lattice_dim(lattice);
+        nv = (what == MFA_RED_MASS) ? 1 : (what == MFA_RED_HEALTH ? 2 : lattice_dim(lattice));
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, ws, kRedBlocks, nv, out_dev);
         MFA_HIP_CHECK(hipGetLastError());
         return MFA_OK;
@@ -1112,6 +1127,9 @@ int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void*
         } else if (what == MFA_RED_MOMENTUM) {
             hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_MOMENTUM>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
             nv = LT::D;
+        } else if (what == MFA_RED_HEALTH) {
+            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_HEALTH>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
+            nv = 2;
         } else if (what == MFA_RED_COMP_SUM) {
             hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_COMP_SUM>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
             nv = f->ncomp;

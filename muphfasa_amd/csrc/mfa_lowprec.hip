@@ -360,8 +360,9 @@ __global__ __launch_bounds__(256) void k_convert(const void* __restrict__ src, v
     });
 }
 
-// FP64 mass / momentum of reduced storage (per-block partials, see
-// k_reduce_final in mfa_lbm.hip for the deterministic final pass).
+// FP64 mass / momentum / health counts of reduced storage (per-block
+// partials, see k_reduce_final in mfa_lbm.hip for the deterministic final
+// pass).  The stored value is the deviation d_q = f_q - w_q.
 template <class L, int WHAT, class T>
 __global__ __launch_bounds__(256) void k_reduce_dev(const T* __restrict__ f, const Geo g, const View v,
                                                     double* __restrict__ part)
@@ -375,6 +376,19 @@ __global__ __launch_bounds__(256) void k_reduce_dev(const T* __restrict__ f, con
         const int y = (int)(r % g.n[1]);
         const int z = (int)(r / g.n[1]);
         const long long self = x + y * v.sy + z * v.sz;
+        if constexpr (WHAT == MFA_RED_HEALTH) {
+            double rho = 0.0;
+            bool finite = true;
+            for_q<L>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const double d = (double)to_f(f[q * v.sq + self]);
+                finite = finite && __builtin_isfinite(d);
+                rho += L::w(q) + d;
+            });
+            acc[0] += finite ? 0.0 : 1.0;
+            acc[1] += (finite && rho <= 0.0) ? 1.0 : 0.0;
+            continue;
+        }
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const double d = (double)to_f(f[q * v.sq + self]);
@@ -720,6 +734,8 @@ int reduce_dev(const mfa_field* f, const Geo& g, int lat, int what, double* part
                 hipLaunchKernelGGL((k_reduce_dev<L, MFA_RED_MASS, T>), dim3(nblocks), dim3(256), 0, st, (const T*)f->data, g, v, part);
             else if (what == MFA_RED_MOMENTUM)
                 hipLaunchKernelGGL((k_reduce_dev<L, MFA_RED_MOMENTUM, T>), dim3(nblocks), dim3(256), 0, st, (const T*)f->data, g, v, part);
+            else if (what == MFA_RED_HEALTH)
+                hipLaunchKernelGGL((k_reduce_dev<L, MFA_RED_HEALTH, T>), dim3(nblocks), dim3(256), 0, st, (const T*)f->data, g, v, part);
             else
                 return fail(MFA_ERR_UNSUPPORTED, "mfa_reduce: per-component sums need FP64 storage");
             MFA_HIP_CHECK(hipGetLastError());

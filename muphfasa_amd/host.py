@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmuphfasa_host.so")
 EXE_PATH = os.path.join(HERE, "lib", "muphfasa")
 EXPORTS = ["mfa_sim_error", "mfa_sim_create", "mfa_sim_destroy", "mfa_sim_initialize",
            "mfa_sim_advance", "mfa_sim_update_macroscopic", "mfa_sim_write_output", "mfa_sim_sync",
-           "mfa_sim_info", "mfa_sim_copy_f", "mfa_sim_copy_macro", "mfa_inputs_units",
+           "mfa_sim_info", "mfa_sim_copy_f", "mfa_sim_copy_macro", "mfa_sim_health", "mfa_inputs_units",
            "mfa_run_main"]
 _lib = None
 
@@ -40,6 +40,7 @@ def lib():
         L.mfa_sim_info.argtypes = [ctypes.c_void_p, ip, ip, dp, dp, dp, ip]
         L.mfa_sim_copy_f.argtypes = [ctypes.c_void_p, dp]
         L.mfa_sim_copy_macro.argtypes = [ctypes.c_void_p, dp, dp]
+        L.mfa_sim_health.argtypes = [ctypes.c_void_p, dp]
         L.mfa_inputs_units.argtypes = [cp, ctypes.c_int, ctypes.POINTER(cp), dp]
         L.mfa_run_main.argtypes = [ctypes.c_int, ctypes.POINTER(cp)]
         _lib = L
@@ -136,6 +137,12 @@ class Simulation:
         dp = ctypes.POINTER(ctypes.c_double)
         _check(lib().mfa_sim_copy_macro(self._h, r.ctypes.data_as(dp), v.ctypes.data_as(dp)))
         return r, v
+
+    def health(self):
+        """(cells with a non-finite population, cells with finite density <= 0)."""
+        out = (ctypes.c_double * 2)()
+        _check(lib().mfa_sim_health(self._h, out))
+        return int(out[0]), int(out[1])
 
     def close(self):
         if self._h:

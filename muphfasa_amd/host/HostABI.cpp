@@ -61,6 +61,15 @@ int mfa_sim_update_macroscopic(mfa_sim* s, int step)
 
 int mfa_sim_write_output(mfa_sim* s, int step) { return guarded([&] { s->sim->writeOutput(step); }); }
 
+int mfa_sim_health(mfa_sim* s, double out[2])
+{
+    return guarded([&] {
+        const std::array<double, 2> h = s->sim->health();
+        out[0] = h[0];
+        out[1] = h[1];
+    });
+}
+
 int mfa_sim_sync(mfa_sim* s)
 {
     return guarded([&] { hip_check(hipStreamSynchronize(s->sim->stream()), "sync"); });
@@ -145,6 +154,16 @@ int mfa_run_main(int argc, const char* const* argv)
                    !(stop_time_lat > 0 && stop_time_lat < sim->time() + k))
                 ++k;
             sim->advanceSteps(k);
+            // fail fast on a diverged state, before anything is written
+            const std::array<double, 2> h = sim->health();
+            if (h[0] > 0 || h[1] > 0) {
+                char msg[200];
+                std::snprintf(msg, sizeof msg,
+                              "step %d: the state has diverged: %.0f cells with non-finite "
+                              "populations, %.0f cells with non-positive density",
+                              step + k - 1, h[0], h[1]);
+                throw std::runtime_error(msg);
+            }
             sim->writeOutput(step + k - 1);
             step += k;
         }

@@ -124,6 +124,7 @@ public:
     bool hasForcing() const noexcept override { return m_has_forcing; }
     hipStream_t stream() const noexcept override { return m_stream; }
     double totalMass() override;
+    std::array<double, 2> health() override;
 
 private:
     void setInitialConditions();
@@ -327,6 +328,15 @@ double SimulationImpl::totalMass()
     mfa_check(mfa_reduce(m_rho.view(), m_lattice, MFA_RED_COMP_SUM, m_scratch, m_stream), "totalMass");
     double v = 0;
     hip_check(hipMemcpyAsync(&v, m_scratch, sizeof(double), hipMemcpyDeviceToHost, m_stream), "copy");
+    hip_check(hipStreamSynchronize(m_stream), "sync");
+    return v;
+}
+
+std::array<double, 2> SimulationImpl::health()
+{
+    mfa_check(mfa_reduce(m_fin->view(), m_lattice, MFA_RED_HEALTH, m_scratch, m_stream), "health");
+    std::array<double, 2> v{0., 0.};
+    hip_check(hipMemcpyAsync(v.data(), m_scratch, sizeof(v), hipMemcpyDeviceToHost, m_stream), "copy");
     hip_check(hipStreamSynchronize(m_stream), "sync");
     return v;
 }

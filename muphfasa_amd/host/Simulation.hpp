@@ -65,6 +65,9 @@ public:
     virtual hipStream_t stream() const noexcept = 0;
     // FP64 sum of rho over valid cells after updateMacroscopic (host value)
     virtual double totalMass() = 0;
+    // Health of the current populations (host values): {cells with a
+    // non-finite f_q, cells whose density sum_q f_q is finite and <= 0}
+    virtual std::array<double, 2> health() = 0;
 };
 
 std::unique_ptr<ISimulation> makeSimulation(const InputParser& params);

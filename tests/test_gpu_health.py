@@ -1,0 +1,131 @@
+"""The divergence guard: mfa_reduce(MFA_RED_HEALTH) counts the valid cells
+with a non-finite population and the cells whose density is finite and not
+positive, for FP64 and the reduced storages; the C++ host reports the counts
+(mfa_sim_health) and its run loop stops with an error, before writing output,
+once a run has diverged.  The expected counts are those of the planted cells
+(counted in numpy on the host copy), and, for a genuinely diverging cavity,
+those of the oracle's state after the same steps (the FP64 path is bit-exact
+to it)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from muphfasa_amd import abi, host
+from oracle import oracle as O
+from tests.gpu_util import random_state, to_dev
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TORCH_DT = {"f32": "float32", "f16dev": "float16", "bf16dev": "bfloat16"}
+
+
+def _counts(valid):
+    """Host count: (cells with a non-finite f_q, cells with finite rho <= 0)."""
+    finite = np.isfinite(valid).all(axis=0)
+    rho = valid.sum(axis=0)
+    return int((~finite).sum()), int((finite & (rho <= 0)).sum())
+
+
+def _plant(L, valid, seed, exact_zero):
+    """Plant non-finite values and non-positive densities in distinct cells."""
+    rng = np.random.default_rng(seed)
+    cells = valid.reshape(L.Q, -1)
+    pick = rng.choice(cells.shape[1], size=16, replace=False)
+    for i, c in enumerate(pick):
+        q = int(rng.integers(L.Q))
+        if i < 5:
+            cells[q, c] = np.nan
+        elif i < 7:
+            cells[q, c] = np.inf
+        elif i < 8:
+            cells[q, c] = -np.inf
+        elif i < 14:
+            cells[:, c] = -0.5 * L.w            # rho = -0.5
+        elif i < 15:
+            cells[:, c] = -0.5 * L.w            # negative and non-finite: counted once, as non-finite
+            cells[q, c] = np.nan
+        elif exact_zero:
+            cells[:, c] = 0.0                   # rho = 0 exactly
+    return valid
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q15", "D3Q19", "D3Q27"])
+@pytest.mark.parametrize("dt", ["f64", "f32", "f16dev", "bf16dev"])
+def test_health_counts_planted_cells(dev, lat, dt):
+    import torch
+    L = O.Lattice(lat)
+    n = [13, 7] if L.D == 2 else [13, 7, 5]
+    nn = O.n3(n)
+    lid = abi.LATTICE[lat]
+    print(f"case health lattice={lat} dtype={dt}", flush=True)
+    fab = random_state(L, n, seed=11)              # ghosts NaN: they must not count
+    out = torch.full((2,), -1.0, dtype=torch.float64, device=dev)
+    if dt == "f64":
+        t = to_dev(fab, dev)
+        f = abi.field(t, n, 1, L.D)
+        abi.reduce(f, lid, abi.RED["health"], out.data_ptr())
+        torch.cuda.synchronize()
+        assert out.tolist() == [0.0, 0.0]
+        v = _plant(L, O.valid(L, fab).copy(), 3, True)
+        O.valid(L, fab)[...] = v
+        t = to_dev(fab, dev)
+        f = abi.field(t, n, 1, L.D)
+    else:
+        v = _plant(L, np.ascontiguousarray(O.valid(L, fab)), 3, False)
+        src = to_dev(v, dev)
+        t = torch.empty((L.Q, nn[2], nn[1], nn[0]), dtype=getattr(torch, TORCH_DT[dt]), device=dev)
+        f = abi.field(t, n, 0, L.D, abi.DTYPE[dt])
+        abi.convert(abi.field(src, n, 0, L.D), f, lid)
+    expect = _counts(v)
+    assert expect[0] == 9 and expect[1] >= 6
+    abi.reduce(f, lid, abi.RED["health"], out.data_ptr())
+    torch.cuda.synchronize()
+    assert tuple(int(x) for x in out.tolist()) == expect
+
+
+def _diverging_cavity():
+    """32^2 lid cavity at Re = 1e5 with the lid at u_lat = 0.29 (tau = 0.50028):
+    BGK goes unstable; the oracle has cells of negative density from about
+    step 80 on."""
+    text = open(os.path.join(ROOT, "tests", "golden", "inputs", "lid_cavity_128")).read()
+    return (text.replace("128 128", "32 32").replace("lbm.u_lat          = 0.1", "lbm.u_lat = 0.29")
+            .replace("units.Re           = 100.0", "units.Re = 1e5"))
+
+
+def test_host_health_of_a_diverging_cavity_matches_oracle():
+    sim = host.Simulation(_diverging_cavity())
+    try:
+        sim.initialize()
+        assert sim.health() == (0, 0)
+        f0 = sim.f()
+        sim.advance(100)
+        got = sim.health()
+        L = O.Lattice("D2Q9")
+        n = [32, 32]
+        fu = np.zeros((6, 3))
+        fu[3, 0] = 0.29
+        res = O.resolve_bc(2, [1, 1, 1, 0], np.ones(6), fu)
+        fab = O.new_fab(L, n)
+        O.valid(L, fab)[...] = f0
+        O.apply_bc(L, fab, n, [0, 0, 0], res)
+        fe = O.run(L, fab, n, sim.omega, None, [0, 0], res, 100)
+        assert np.array_equal(sim.f(), O.valid(L, fe))
+        expect = _counts(O.valid(L, fe))
+        assert expect[1] > 0
+        assert got == expect
+    finally:
+        sim.close()
+
+
+def test_executable_stops_on_divergence_before_writing(tmp_path):
+    inp = tmp_path / "inputs"
+    inp.write_text(_diverging_cavity())
+    r = subprocess.run([host.EXE_PATH, str(inp), "sim.max_step=300", "amr.plot_int=100",
+                        f"amr.plot_file={tmp_path}/plt"], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "diverged" in r.stderr and "step 100" in r.stderr, r.stderr
+    assert os.path.isdir(tmp_path / "plt0000000")          # written after the first (healthy) step
+    assert not os.path.exists(tmp_path / "plt0000100")
