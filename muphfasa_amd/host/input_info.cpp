@@ -17,11 +17,13 @@
 #include <cstdarg>
 #include <cstdio>
 #include <exception>
+#include <fstream>
 #include <string>
 #include <vector>
 
 #include "InputParser.hpp"
 #include "ParmParse.hpp"
+#include "PlotIO.hpp"
 #include "Units.hpp"
 
 namespace {
@@ -172,7 +174,43 @@ int report(const std::string& file, const ParmParse& pp)
     for (int i = 0; i < D; ++i) nc += fmt("%s%d", i ? " x " : "", N[i]);
     row("cells", fmt("%s = %.0f", nc.c_str(), cells));
     row("device memory", fmt("~%s  (2 x %d populations + rho, u; no ghost layer)", bytes(mem).c_str(), Q));
-    row("initial state", in.initName());
+    const std::string restart = in.restartFile();
+    if (restart.empty()) {
+        row("initial state", in.initName());
+    } else {
+        // the checks restartFromCheckpoint makes before it reads the populations
+        BoxGeom g;
+        g.dim = D;
+        for (int i = 0; i < D && i < (int)N.size(); ++i) g.n[i] = N[i];
+        CheckpointHeader h;
+        std::string err = readCheckpointHeader(restart, h, D);
+        std::vector<std::string> problems;
+        if (err.empty() && Q) problems = checkpointMismatches(restart, h, g, Q);
+        if (!err.empty()) problems.insert(problems.begin(), err);
+        if (problems.empty()) {
+            // one FAB: a text header line, then Q planes of FP64 values
+            std::ifstream d(restart + "/f_D_00000", std::ios::binary);
+            std::string fabhdr;
+            if (!d) {
+                problems.push_back("data in " + restart + "/f is missing");
+            } else if (!std::getline(d, fabhdr) || fabhdr.rfind("FAB ", 0) != 0) {
+                problems.push_back("data in " + restart + "/f is not a FAB file");
+            } else {
+                const std::streamoff start = d.tellg();
+                d.seekg(0, std::ios::end);
+                if ((double)(d.tellg() - start) < cells * Q * 8.0)
+                    problems.push_back("data in " + restart + "/f does not cover the domain (incomplete checkpoint?)");
+            }
+        }
+        if (problems.empty()) {
+            row("initial state", fmt("restart from %s (written at step %d, t_lat = %.17g); the run resumes at "
+                                     "step %d", restart.c_str(), h.step, h.time, h.step + 1));
+        } else {
+            row("initial state", "restart from " + restart + ": unusable");
+            for (const auto& m : problems) notes.warn.push_back("amr.restart: " + m);
+            notes.fatal = true;
+        }
+    }
 
     std::printf("\nPhysical scales\n");
     row("L_ref", fmt("%.6g m", u.refLength()));

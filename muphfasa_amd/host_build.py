@@ -16,7 +16,7 @@ SOURCES = ["ParmParse.cpp", "InputParser.cpp", "Units.cpp", "PlotIO.cpp", "Simul
            "HostABI.cpp"]
 FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-I", os.path.join(ROOT, "include")]
 # the inputs inspector is host-only (parser + unit converter, no HIP)
-INFO_SOURCES = ["input_info.cpp", "ParmParse.cpp", "InputParser.cpp", "Units.cpp"]
+INFO_SOURCES = ["input_info.cpp", "ParmParse.cpp", "InputParser.cpp", "Units.cpp", "PlotIO.cpp"]
 
 
 def build(verbose: bool = False) -> str:

@@ -114,3 +114,51 @@ def test_inputinfo_advisories_and_refusals():
     assert _inputinfo(os.path.join(INP, "periodic_d3q19_256"), "lbm.lattice_type=D3Q7").returncode == 1
     r = _inputinfo(os.path.join(INP, "lid_cavity_128"), "units.nu_phys=0.1")
     assert r.returncode == 1 and "Both units.Re and units.nu_phys" in r.stdout
+
+
+def _fake_checkpoint(d, n, ncomp, step=20, time=21.0, version="muphfasa_checkpoint_v1", box=None,
+                     data_doubles=None, fab=True):
+    """A checkpoint directory in the muphfasa_checkpoint_v1 layout (text
+    Header; VisMF f_H / f_D_00000 with one FAB) for a 2-D box of n cells."""
+    import numpy as np
+    os.makedirs(d, exist_ok=True)
+    hi = f"({n[0] - 1},{n[1] - 1})"
+    box = box if box is not None else f"((0,0) {hi} (0,0))"
+    with open(os.path.join(d, "Header"), "w") as h:
+        h.write(f"{version}\n{step}\n{time!r}\n{box}\n{ncomp}\n(0,0)\n")
+    if data_doubles is None:
+        data_doubles = n[0] * n[1] * ncomp
+    if data_doubles >= 0:
+        with open(os.path.join(d, "f_D_00000"), "wb") as f:
+            if fab:
+                f.write(f"FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (1 2 3 4 5 6 7 8)))((0,0) {hi} (0,0)) "
+                        f"{ncomp}\n".encode())
+            f.write(np.full(data_doubles, 1.0 / 9.0).tobytes())
+    return d
+
+
+def test_inputinfo_validates_restart_checkpoints(tmp_path):
+    """amr.restart is checked as restartFromCheckpoint checks it
+    (src/IO/Checkpoint.cpp:61-103): a usable checkpoint names the step the
+    run resumes at; a missing or malformed Header, a different domain or
+    component count, and missing or short data make the inputs refused."""
+    inp = os.path.join(INP, "lid_cavity_128")
+    n = [16, 16]
+    args = ["grid.ncells=16 16"]
+    ok = _fake_checkpoint(str(tmp_path / "ok"), n, 9)
+    r = _inputinfo(inp, *args, f"amr.restart={ok}")
+    assert r.returncode == 0 and "resumes at step 21" in r.stdout, r.stdout
+    cases = {
+        "no Header file found": lambda d: os.makedirs(d) or d,
+        "unrecognized checkpoint version": lambda d: _fake_checkpoint(d, n, 9, version="chk_v0"),
+        "malformed Header": lambda d: _fake_checkpoint(d, n, 9, box="(0 0 15 15)"),
+        "domain mismatch": lambda d: _fake_checkpoint(d, [32, 32], 9),
+        "holds 19 distribution components": lambda d: _fake_checkpoint(d, n, 19),
+        "does not cover the domain": lambda d: _fake_checkpoint(d, n, 9, data_doubles=16 * 16 * 9 - 1),
+        "is missing": lambda d: _fake_checkpoint(d, n, 9, data_doubles=-1),
+        "is not a FAB file": lambda d: _fake_checkpoint(d, n, 9, fab=False),
+    }
+    for i, (msg, make) in enumerate(cases.items()):
+        d = make(str(tmp_path / f"bad{i}"))
+        r = _inputinfo(inp, *args, f"amr.restart={d}")
+        assert r.returncode == 1 and msg in r.stdout and "would refuse" in r.stdout, (msg, r.stdout)

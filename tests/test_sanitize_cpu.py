@@ -32,7 +32,7 @@ def builds(tmp_path_factory):
     info = d / "inputinfo_san"
     subprocess.run(["g++", *SAN, "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
                     "-o", str(info)] + [os.path.join(host, f) for f in
-                                        ("input_info.cpp", "ParmParse.cpp", "InputParser.cpp", "Units.cpp")],
+                                        ("input_info.cpp", "ParmParse.cpp", "InputParser.cpp", "Units.cpp", "PlotIO.cpp")],
                    check=True)
     return oracle, info
 
