@@ -20,7 +20,10 @@
  * order (sq = e0*e1*e2, one plane per population) and the row-interleaved
  * order (sq = e0, sy = ncomp*e0: the Q rows of one (y,z) side by side, the
  * engine's own allocation, about 5 % faster for the step on MI355X) run the
- * same kernels and give identical results.
+ * same kernels and give identical results.  The input and output of
+ * mfa_step, mfa_step2, mfa_pull_collide and mfa_convert must occupy disjoint
+ * byte ranges (all components, ghost layers included); overlapping views are
+ * refused with MFA_ERR_INVALID before any device work.
  */
 #ifndef MFA_LBM_H
 #define MFA_LBM_H

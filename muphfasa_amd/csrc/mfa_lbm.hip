@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -544,6 +545,31 @@ static int check_field(const mfa_field* f, int ncomp_min, int need_grow, const c
     return MFA_OK;
 }
 
+// Byte range [lo, hi] a field's kernels may touch: every component over the
+// valid box grown by its ghost layers (a z layer only when the box has more
+// than one plane, so 2-D views are not widened past their allocation).
+static void byte_range(const mfa_field& f, long long& lo, long long& hi)
+{
+    const long long es = f.dtype == MFA_F64 ? 8 : (f.dtype == MFA_F32 ? 4 : 2);
+    const long long g = f.ngrow, gz = f.n[2] > 1 ? g : 0;
+    const long long base = (long long)(intptr_t)f.data;
+    lo = base - es * (g + g * f.sy + gz * f.sz);
+    hi = base + es * ((f.n[0] - 1 + g) + (f.n[1] - 1 + g) * f.sy + (f.n[2] - 1 + gz) * f.sz +
+                      (long long)(f.ncomp - 1) * f.sq + 1) - 1;
+}
+
+// A kernel that reads one field while it writes another races if the two
+// share memory, so their byte ranges must be disjoint.
+static int check_disjoint(const mfa_field* a, const mfa_field* b, const char* what)
+{
+    long long alo, ahi, blo, bhi;
+    byte_range(*a, alo, ahi);
+    byte_range(*b, blo, bhi);
+    if (alo <= bhi && blo <= ahi)
+        return fail(MFA_ERR_INVALID, std::string(what) + ": input and output overlap in memory");
+    return MFA_OK;
+}
+
 static Geo geo_of(const mfa_field& f, const mfa_geom_bc* gb, int lat)
 {
     Geo g;
@@ -827,7 +853,7 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
         return rc;
     if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
         return fail(MFA_ERR_INVALID, "mfa_step2: fin/fout boxes differ");
-    if (fin->data == fout->data) return fail(MFA_ERR_INVALID, "mfa_step2: fin aliases fout");
+    if ((rc = check_disjoint(fin, fout, "mfa_step2"))) return rc;
     if (fin->dtype != fout->dtype) return fail(MFA_ERR_INVALID, "mfa_step2: fin/fout storage types differ");
     if ((rc = check_2d(fin, lat))) return rc;
     const Geo geo = geo_of(*fin, g, lat);
@@ -900,7 +926,7 @@ int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, double
         return rc;
     if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
         return fail(MFA_ERR_INVALID, "mfa_step: fin/fout boxes differ");
-    if (fin->data == fout->data) return fail(MFA_ERR_INVALID, "mfa_step: fin aliases fout");
+    if ((rc = check_disjoint(fin, fout, "mfa_step"))) return rc;
     if (fin->dtype != fout->dtype) return fail(MFA_ERR_INVALID, "mfa_step: fin/fout storage types differ");
     if ((rc = check_2d(fin, lat))) return rc;
     const Geo geo = geo_of(*fin, g, lat);
@@ -934,6 +960,7 @@ int mfa_convert(const mfa_field* src, mfa_field* dst, int lattice, void* stream)
     if (std::memcmp(src->n, dst->n, sizeof(src->n))) return fail(MFA_ERR_INVALID, "mfa_convert: boxes differ");
     if ((src->dtype == MFA_F64) == (dst->dtype == MFA_F64))
         return fail(MFA_ERR_INVALID, "mfa_convert: exactly one side must be FP64");
+    if ((rc = check_disjoint(src, dst, "mfa_convert"))) return rc;
     if ((rc = check_2d(src, lattice))) return rc;
     return convert_dev(src, dst, geo_of(*src, nullptr, lattice), lattice, (hipStream_t)stream);
 }
@@ -967,6 +994,7 @@ int mfa_pull_collide(const mfa_field* fin, mfa_field* fout, int lattice, double 
         return rc;
     if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
         return fail(MFA_ERR_INVALID, "mfa_pull_collide: fin/fout boxes differ");
+    if ((rc = check_disjoint(fin, fout, "mfa_pull_collide"))) return rc;
     if ((rc = check_2d(fin, lattice))) return rc;
     const Geo geo = geo_of(*fin, nullptr, lattice);
     return with_lattice(lattice, [&](auto L) {

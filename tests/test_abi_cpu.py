@@ -192,3 +192,52 @@ def test_level_chopping_tiles_the_domain():
                     assert (x, y, z) not in cover
                     cover.add((x, y, z))
     assert len(cover) == 40 * 30 * 20
+
+
+def test_overlapping_input_and_output_are_rejected_before_launch():
+    """mfa_step, mfa_step2, mfa_pull_collide and mfa_convert read one field
+    while writing another: views whose byte ranges (all components, ghost
+    layers included) intersect are refused with MFA_ERR_INVALID before any
+    device work; views that touch end to end pass this check (shown by the
+    next check refusing them for another reason)."""
+    g = abi.resolve_bc(abi.LATTICE["D3Q19"], [abi.BC["Periodic"]] * 6)
+    n = 8
+
+    def fab(addr, ngrow=0, dtype=0):
+        f = abi.Field()
+        f.data = addr
+        e = n + 2 * ngrow
+        f.n[0] = f.n[1] = f.n[2] = n
+        f.ngrow = ngrow
+        f.sy, f.sz, f.sq, f.ncomp, f.dtype = e, e * e, e * e * e, 19, dtype
+        return f
+
+    L = abi.lib()
+    size = 19 * n ** 3 * 8
+    base = 1 << 30
+    for shift in (0, 8, size - 8):                      # same start, one cell on, last element shared
+        a, b = fab(base), fab(base + shift)
+        for call in (lambda: L.mfa_step(ctypes.byref(a), ctypes.byref(b), ctypes.byref(g), 1.0, None, None),
+                     lambda: L.mfa_step2(ctypes.byref(a), ctypes.byref(b), ctypes.byref(g), 1.0, None, None)):
+            with pytest.raises(abi.MfaError) as e:
+                abi.check(call())
+            assert e.value.code == -1 and "overlap" in str(e.value), shift
+    # ghost layers count: an FP32 output [base, base + size32); an FP64
+    # ghosted input whose lowest ghost byte lies in the output's last element
+    # overlaps, one that starts right after it does not (and is then refused
+    # only because the storage types differ)
+    e1 = n + 2
+    reach = 8 * (1 + e1 + e1 * e1)                     # bytes from the lowest ghost to element (0,0,0)
+    size32 = size // 2
+    out = fab(base, dtype=abi.DTYPE["f32"])
+    for extra, refused in ((0, True), (4, False)):
+        b = fab(base + size32 - 4 + reach + extra, ngrow=1)
+        with pytest.raises(abi.MfaError) as e:
+            abi.check(L.mfa_step(ctypes.byref(b), ctypes.byref(out), ctypes.byref(g), 1.0, None, None))
+        assert ("overlap" in str(e.value)) == refused, (extra, str(e.value))
+        assert refused or "storage types differ" in str(e.value)
+    # FP32 output inside an FP64 input's range
+    a, b = fab(base), fab(base + size // 2, dtype=abi.DTYPE["f32"])
+    with pytest.raises(abi.MfaError) as e:
+        abi.check(L.mfa_convert(ctypes.byref(a), ctypes.byref(b), abi.LATTICE["D3Q19"], None))
+    assert "overlap" in str(e.value)
