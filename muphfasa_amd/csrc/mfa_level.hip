@@ -84,8 +84,14 @@ __global__ __launch_bounds__(256) void k_level_step(const BoxView* __restrict__ 
                                                     const Geo g, const double omega, const double F0,
                                                     const double F1, const double F2)
 {
-    const long long blk = blockIdx.x + (long long)gridDim.x * blockIdx.y;
-    if (blk >= bstart[nboxes]) return;
+    const long long nblk = bstart[nboxes];
+    const long long wg = blockIdx.x + (long long)gridDim.x * blockIdx.y;
+    if (wg >= nblk) return;
+    // Consecutive workgroups go to the XCDs in turn; give each XCD a
+    // contiguous range of tiles instead, so that tiles sharing input rows
+    // and planes meet in one L2 (a bijection on [0, nblk)).
+    const long long per = nblk / kXcds, rem = nblk % kXcds, k = wg % kXcds;
+    const long long blk = k * per + (k < rem ? k : rem) + wg / kXcds;
     int lo = 0, hi = nboxes - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
