@@ -48,9 +48,10 @@ def boxes(domain, max_grid):
 
 class BoxLevel:
     """Two population buffers of every box (ngrow 1), bound to level slots
-    0 and 1."""
+    0 and 1.  layout "component": one plane per population (the MultiFab's
+    order); "row": the Q rows of one (y, z) side by side."""
 
-    def __init__(self, lattice: str, domain, max_grid: int, g, dev="cuda:0"):
+    def __init__(self, lattice: str, domain, max_grid: int, g, dev="cuda:0", layout="component"):
         import torch
         self.L = abi.LATTICE[lattice]
         self.Q = {"D3Q15": 15, "D3Q19": 19, "D3Q27": 27}[lattice]
@@ -59,22 +60,30 @@ class BoxLevel:
         self.g = g
         per = [int(g.periodic[d]) for d in range(3)]
         self.level = abi.Level(self.L, self.domain, per, self.lo, self.n)
-        self.t = [[torch.full((self.Q, m[2] + 2, m[1] + 2, m[0] + 2), float("nan"), dtype=torch.float64,
-                              device=dev) for m in self.n] for _ in range(2)]
-        self.f = [[abi.field(t, self.n[b], 1, 3) for b, t in enumerate(ts)] for ts in self.t]
+        if layout == "component":
+            shape = lambda m: (self.Q, m[2] + 2, m[1] + 2, m[0] + 2)
+            self.t = [[torch.full(shape(m), float("nan"), dtype=torch.float64, device=dev) for m in self.n]
+                      for _ in range(2)]
+            self.cv = self.t
+        else:
+            shape = lambda m: (m[2] + 2, m[1] + 2, self.Q, m[0] + 2)
+            self.t = [[torch.full(shape(m), float("nan"), dtype=torch.float64, device=dev) for m in self.n]
+                      for _ in range(2)]
+            self.cv = [[t.permute(2, 0, 1, 3) for t in ts] for ts in self.t]   # (Q, z, y, x) views
+        self.f = [[abi.field(t, self.n[b], 1, 3, layout=layout) for b, t in enumerate(ts)] for ts in self.t]
         self.level.bind(0, self.f[0])
         self.level.bind(1, self.f[1])
 
     def scatter(self, full, slot=0):
         """Copy a (Q, nz, ny, nx) tensor of the whole domain into the boxes."""
         for b, (l, m) in enumerate(zip(self.lo, self.n)):
-            self.t[slot][b][:, 1:-1, 1:-1, 1:-1] = full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1],
+            self.cv[slot][b][:, 1:-1, 1:-1, 1:-1] = full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1],
                                                         l[0]:l[0] + m[0]]
 
     def gather(self, full, slot=0):
         for b, (l, m) in enumerate(zip(self.lo, self.n)):
             full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1], l[0]:l[0] + m[0]] = \
-                self.t[slot][b][:, 1:-1, 1:-1, 1:-1]
+                self.cv[slot][b][:, 1:-1, 1:-1, 1:-1]
 
     def step(self, src: int, omega: float, force=None):
         """One step from slot src into the other slot."""
@@ -86,7 +95,7 @@ class BoxLevel:
         self.level.close()
 
 
-def run(n=256, max_grid=64, steps=200, tau=0.8, force=(1e-6, 0.0, 0.0), dev="cuda:0"):
+def run(n=256, max_grid=64, steps=200, tau=0.8, force=(1e-6, 0.0, 0.0), dev="cuda:0", layout="component"):
     import torch
     lat = "D3Q19"
     Q = 19
@@ -104,16 +113,22 @@ def run(n=256, max_grid=64, steps=200, tau=0.8, force=(1e-6, 0.0, 0.0), dev="cud
     del rho, vel
     steps -= steps % 2
     out = {"workload": f"D3Q19 periodic {n}^3 FP64 forced, boxes of at most {max_grid}^3",
-           "steps": steps}
+           "steps": steps, "layout": layout}
     # one box
-    a, b = full.clone(), torch.empty_like(full)
-    fa, fb = abi.field(a, dom, 0, 3), abi.field(b, dom, 0, 3)
+    if layout == "component":
+        a, b = full.clone(), torch.empty_like(full)
+        av = a
+    else:
+        a = full.permute(1, 2, 0, 3).contiguous()          # (z, y, Q, x)
+        b = torch.empty_like(a)
+        av = a.permute(2, 0, 1, 3)
+    fa, fb = abi.field(a, dom, 0, 3, layout=layout), abi.field(b, dom, 0, 3, layout=layout)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         abi.step(fa, fb, g, omega, F)
         abi.step(fb, fa, g, omega, F)
     torch.cuda.synchronize()
-    a.copy_(full)
+    av.copy_(full)
     gr = torch.cuda.CUDAGraph()
     with torch.cuda.graph(gr):
         abi.step(fa, fb, g, omega, F)
@@ -127,10 +142,10 @@ def run(n=256, max_grid=64, steps=200, tau=0.8, force=(1e-6, 0.0, 0.0), dev="cud
     ms = e0.elapsed_time(e1)
     out["one_box"] = {"boxes": 1, "ms_per_step": round(ms / steps, 4),
                       "mlups": round(n ** 3 * steps / (ms * 1e-3) / 1e6, 1)}
-    single = a
-    del b, gr
+    single = av.contiguous()
+    del a, b, av, gr
     # the level
-    lv = BoxLevel(lat, dom, max_grid, g, dev)
+    lv = BoxLevel(lat, dom, max_grid, g, dev, layout)
     lv.scatter(full, 0)
     with torch.cuda.stream(s):
         lv.step(0, omega, F)
@@ -162,8 +177,9 @@ def main():
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--max-grid", type=int, default=64)
     p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--layout", default="component", choices=["component", "row"])
     a = p.parse_args()
-    print(json.dumps(run(a.n, a.max_grid, a.steps)), flush=True)
+    print(json.dumps(run(a.n, a.max_grid, a.steps, layout=a.layout)), flush=True)
 
 
 if __name__ == "__main__":

@@ -186,3 +186,15 @@ def test_level_rejects_bad_layouts_and_bindings(dev):
             lv.fill_boundary(1)                                                  # slot never bound
     finally:
         lv.close()
+
+
+@pytest.mark.parametrize("layout", ["component", "row"])
+def test_level_run_bit_identical_in_both_population_orders(dev, layout):
+    """muphfasa_amd.level.run: a 40^3 periodic forced domain as 27 uneven
+    boxes (max_grid 16: pieces of 14, 13, 13), stepped through graph-replayed
+    level steps, equals the one-box mfa_step run bit for bit, with the boxes
+    in the MultiFab's component-major order or in the row-interleaved order."""
+    from muphfasa_amd import level
+    print(f"case level.run layout={layout}", flush=True)
+    out = level.run(n=40, max_grid=16, steps=6, dev=dev, layout=layout)
+    assert out["level"]["boxes"] == 27 and out["bit_identical"]
