@@ -64,6 +64,7 @@ int mfa_sim_write_output(mfa_sim* s, int step) { return guarded([&] { s->sim->wr
 int mfa_sim_health(mfa_sim* s, double out[2])
 {
     return guarded([&] {
+        if (!out) throw std::runtime_error("mfa_sim_health: null output");
         const std::array<double, 2> h = s->sim->health();
         out[0] = h[0];
         out[1] = h[1];
