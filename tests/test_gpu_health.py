@@ -129,3 +129,37 @@ def test_executable_stops_on_divergence_before_writing(tmp_path):
     assert "diverged" in r.stderr and "step 100" in r.stderr, r.stderr
     assert os.path.isdir(tmp_path / "plt0000000")          # written after the first (healthy) step
     assert not os.path.exists(tmp_path / "plt0000100")
+
+
+@pytest.mark.parametrize("lat,n", [("D3Q19", [1, 1, 1]), ("D2Q9", [1, 1]), ("D3Q27", [3, 1, 2]),
+                                   ("D3Q15", [1, 5, 1])])
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_health_counts_degenerate_boxes(dev, lat, n, dt):
+    """Boxes one cell thick: a healthy state counts (0, 0); a NaN in the last
+    cell counts (1, 0); a negative density in the first cell counts (0, 1)."""
+    import torch
+    L = O.Lattice(lat)
+    nn = O.n3(n)
+    lid = abi.LATTICE[lat]
+    print(f"case health degenerate lattice={lat} n={n} dtype={dt}", flush=True)
+    out = torch.full((2,), -1.0, dtype=torch.float64, device=dev)
+    base = np.ascontiguousarray(O.valid(L, random_state(L, n, seed=5)))
+    variants = [(base.copy(), (0, 0))]
+    v = base.copy()
+    v[L.Q - 1, -1, -1, -1] = np.nan
+    variants.append((v, (1, 0)))
+    v = base.copy()
+    v[:, 0, 0, 0] = -0.25 * L.w
+    variants.append((v, (0, 1)))
+    for vals, expect in variants:
+        assert _counts(vals) == expect
+        src = to_dev(vals, dev)
+        if dt == "f64":
+            f = abi.field(src, n, 0, L.D)
+        else:
+            t = torch.empty((L.Q, nn[2], nn[1], nn[0]), dtype=torch.float32, device=dev)
+            f = abi.field(t, n, 0, L.D, abi.DTYPE[dt])
+            abi.convert(abi.field(src, n, 0, L.D), f, lid)
+        abi.reduce(f, lid, abi.RED["health"], out.data_ptr())
+        torch.cuda.synchronize()
+        assert tuple(int(x) for x in out.tolist()) == expect
