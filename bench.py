@@ -7,18 +7,32 @@ steps run in pairs through mfa_step2 (two steps per pass, the intermediate
 state kept in LDS; bit-identical to two mfa_step calls); --kernel single runs
 one mfa_step launch per step.
 
-Prints one JSON line with MLUPS, the roofline of the step kernel (algorithmic
-bytes per lattice update = 2*19*8 = 304, one read and one write of every
-population per step), the HBM traffic the counters measured for it, and the
-CPU baseline (the oracle, timed on this host's cores on a bounded number of
-steps of the same workload).
+Prints one JSON line with MLUPS, the roofline of the step kernel and the CPU
+baseline.  Roofline: the fewest bytes a pass can move are one read and one
+write of every population, 2*19*8 = 304 B per cell per launch, whether the
+launch advances one step or two; `achieved` is those bytes over the launch
+time measured here with HIP events, `frac` its fraction of the 8 TB/s HBM
+peak.  The metric's own definition (304 B per lattice update, so twice the
+bytes for a pass of two steps) is reported beside it as
+`frac_per_update_definition`, and the HBM bytes the counters measured for the
+kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel) as
+`traffic`.  Before the hot path, the copy-ceiling probe runs 100 copies of
+the step's bytes (about 0.3 s; --no-copy-probe skips it), which also brings
+the GPU to its sustained clock, and the output lattice is zero-filled when it
+is allocated: from a cold start the first twenty steps of the pass measured
+up to twice the steady time.  The CPU baseline is the oracle (the reference's algorithm in C,
+boxes of max_grid_size 64 with their own ghost layers, OpenMP over boxes),
+built with -march=native for this host and timed on every core of the
+process's affinity set, on a bounded number of steps of the same workload.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -36,7 +50,7 @@ def parse():
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
-                   help="target duration of the bounded CPU baseline sample")
+                   help="target duration of the timed CPU baseline sample")
     p.add_argument("--no-copy-probe", action="store_true",
                    help="skip the streaming-copy ceiling measurement (same bytes as one step)")
     p.add_argument("--seed", type=int, default=2024)
@@ -48,46 +62,73 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(n, steps_hint, seconds):
-    """Time the oracle (the reference's algorithm in C, OpenMP over z) on the
-    same workload for a bounded number of steps."""
+def _native_oracle():
+    """Build lbm_oracle.c with -march=native for this host into a temporary
+    directory; None if the compiler is not there or fails."""
+    src = os.path.join(ROOT, "oracle", "lbm_oracle.c")
+    out = os.path.join(tempfile.mkdtemp(prefix="mfa_oracle_"), "liblbm_oracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-fPIC", "-std=c11", "-ffp-contract=off",
+           "-fno-fast-math", "-fopenmp", "-shared", "-o", out, src, "-lm"]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True, timeout=180)
+    except Exception:
+        return None
+    return out
+
+
+def cpu_baseline(n, seconds, seed):
+    """The oracle on the same workload and seed: the reference's two-lattice
+    pull with the fused collide, the domain cut into boxes of max_grid_size 64
+    (AMReX's default) with one ghost layer each, filled from the neighbouring
+    boxes every step, OpenMP over boxes; 10 warm-up steps, then a bounded
+    number of timed steps."""
     import numpy as np
     from oracle import oracle as O
 
+    native = _native_oracle()
+    if native:
+        O.use_library(native)
+    cores = len(os.sched_getaffinity(0))
     L = O.Lattice("D3Q19")
     nn = [n, n, n]
     F = [1e-6, 0.0, 0.0]
-    rng = np.random.default_rng(2024)
+    rng = np.random.default_rng(seed)
     rho = np.ones((n, n, n))
     vel = rng.uniform(-1e-3, 1e-3, (3, n, n, n))
     f, _, _ = O.init_equilibrium(L, rho, vel, nn, F)
     del rho, vel
-    res = O.resolve_bc(3, [O.BC["Periodic"]] * 6)
-    O.fill_periodic(L, f, nn, [1, 1, 1])
-    fb = np.empty_like(f)
-    cores = min(os.cpu_count() or 1, 16)
-    t1 = O.time_run(L, f, fb, nn, 1.0 / 0.8, F, [1, 1, 1], res, 1, cores)
-    steps = max(2, min(steps_hint, int(seconds / max(t1, 1e-3))))
-    t = O.time_run(L, f, fb, nn, 1.0 / 0.8, F, [1, 1, 1], res, steps, cores)
+    omega = 1.0 / 0.8
+    # 10 warm-up steps and 2 timed ones size the sample; the sample then runs
+    # on the warm state
+    t2 = O.time_run_boxes(L, f, nn, 64, omega, F, 10, 2, cores)
+    steps = max(2, min(200, int(seconds / max(t2 / 2, 1e-4))))
+    t = O.time_run_boxes(L, f, nn, 64, omega, F, 0, steps, cores)
     mlups = n ** 3 * steps / t / 1e6
     return {"value": round(mlups, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
-            "sample": f"D3Q19 {n}^3 FP64 forced periodic, {steps} steps after 1 warm-up step, "
-                      f"{t:.2f} s, OpenMP over z on {cores} threads "
+            "sample": f"D3Q19 {n}^3 FP64 forced periodic, boxes of 64^3 with ghost exchange, "
+                      f"OpenMP over boxes on {cores} threads (the process's affinity set; "
+                      f"nproc {os.cpu_count()}, OMP_NUM_THREADS "
+                      f"{os.environ.get('OMP_NUM_THREADS', 'unset')}), "
+                      f"{'-O3 -march=native' if native else '-O3 (prebuilt; native build failed)'}; "
+                      f"{steps} steps timed after 12 warm-up steps, {t:.2f} s "
                       f"(= {mlups * 304 / 1e3:.1f} GB/s algorithmic)"}
 
 
 def traffic_from_profiles(n, kernel):
     """HBM bytes per launch of the timed kernel, from the committed counter
-    summary of the same workload (FETCH_SIZE + WRITE_SIZE, calibrated)."""
+    summary of the same workload (FETCH_SIZE + WRITE_SIZE, calibrated), with
+    the kernel and commit it was measured on."""
     name = f"step2_d3q19_{n}_f64_traffic.json" if kernel == "pair" else f"step_d3q19_{n}_f64_traffic.json"
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     try:
         with open(path) as fh:
-            return json.load(fh).get("hbm_bytes_per_launch")
+            d = json.load(fh)
     except Exception:
         return None
+    d["file"] = "profiles/" + name
+    return d
 
 
 def main():
@@ -109,13 +150,21 @@ def main():
     omega = 1.0 / tau
     g = abi.resolve_bc(lattice, [abi.BC["Periodic"]] * 6)
 
+    copy = None
+    if not args.no_copy_probe:
+        from muphfasa_amd import autotune
+        copy = autotune.copy_ceiling(N3 * 19 * 8, reps=100)
+        print(json.dumps({"copy_probe_bytes_per_launch": 2 * N3 * 19 * 8,
+                          "copy_probe_GBs": copy}), file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
     gen = torch.Generator(device=dev)
     gen.manual_seed(args.seed)
     rho = torch.ones((1, n, n, n), dtype=torch.float64, device=dev)
     vel = (torch.rand((3, n, n, n), dtype=torch.float64, device=dev, generator=gen) * 2 - 1) * 1e-3
     shape = abi.row_shape(19, [n] * 3, 0, 3) if args.layout == "row" else (19, n, n, n)
     fa_t = torch.empty(shape, dtype=torch.float64, device=dev)
-    fb_t = torch.empty_like(fa_t)
+    fb_t = torch.zeros_like(fa_t)
     fa = abi.field(fa_t, [n] * 3, 0, 3, layout=args.layout)
     fb = abi.field(fb_t, [n] * 3, 0, 3, layout=args.layout)
     stream = torch.cuda.current_stream()
@@ -169,24 +218,19 @@ def main():
     jx_growth = (out[1].item() - jx0) / ((args.warmup + args.steps) * N3 * F[0])
 
     mlups = N3 / (ms_per_step * 1e-3) / 1e6
-    bytes_per_lu = 2 * 19 * 8
-    achieved = N3 * bytes_per_lu / (ms_per_step * 1e-3) / 1e9
     steps_per_launch = 2 if pair else 1
-    traffic = traffic_from_profiles(n, args.kernel)
-
-    copy = None
-    if not args.no_copy_probe:
-        from muphfasa_amd import autotune
-        del out
-        copy = autotune.copy_ceiling(N3 * 19 * 8)
-        print(json.dumps({"copy_probe_bytes_per_launch": 2 * N3 * 19 * 8,
-                          "copy_probe_GBs": copy}), file=sys.stderr, flush=True)
+    launch_s = steps_per_launch * ms_per_step * 1e-3
+    bytes_pass = N3 * 2 * 19 * 8                     # one read + one write per launch
+    bytes_updates = steps_per_launch * N3 * 2 * 19 * 8  # 304 B per lattice update
+    achieved = bytes_pass / launch_s / 1e9
+    prof = traffic_from_profiles(n, args.kernel)
+    traffic = prof.get("hbm_bytes_per_launch") if prof else None
 
     cpu = None
     if not args.no_cpu_baseline:
-        del fa_t, fb_t
+        del fa_t, fb_t, cur, nxt, fa, fb
         torch.cuda.empty_cache()
-        cpu = cpu_baseline(n, 50, args.cpu_seconds)
+        cpu = cpu_baseline(n, args.cpu_seconds, args.seed)
 
     line = {
         "metric": METRIC,
@@ -212,19 +256,27 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "k_step2<D3Q19,forced>" if pair else "k_step<D3Q19,forced,fused>",
+                     "kernel": "k_step2p<D3Q19,forced,periodic,64x8>" if pair
+                               else "k_step<D3Q19,forced,fused>",
                      "steps_per_launch": steps_per_launch,
-                     "algorithmic_bytes_per_launch": steps_per_launch * N3 * bytes_per_lu,
-                     # the HBM rate the counters imply: traffic per launch over launch time
-                     "hbm_GBs_measured_traffic": (round(traffic / (steps_per_launch * ms_per_step
-                                                                   * 1e-3) / 1e9, 1)
-                                                  if traffic else None)},
+                     "bytes_per_launch": bytes_pass,
+                     "bytes_definition": "one read and one write of every population per launch "
+                                         "(2*19*8 B per cell), the least a pass can move",
+                     "launch_ms": round(launch_s * 1e3, 4),
+                     # the metric's own definition: 304 B per lattice update
+                     "frac_per_update_definition": round(bytes_updates / launch_s / 1e9 / HBM_PEAK_GBS, 4),
+                     # the HBM bytes the counters measured for this kernel
+                     "frac_physical": (round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4)
+                                       if traffic else None),
+                     "traffic_source": (f"{prof['file']} (kernel {prof.get('kernel', '?')[:60]}, "
+                                        f"measured at commit {prof.get('commit', '?')})"
+                                        if prof else None)},
         "mass_rel_drift": abs(mass1 - mass0) / mass0,
         "momentum_growth_over_N3F": jx_growth,
         "copy_ceiling_GBs": copy,
         "frac_of_copy_ceiling": round(achieved / copy["nt"], 4) if copy else None,
-        "launch": "mfa_step2: 64x4 columns, 64 z-planes per block, per-c_z LDS rings, "
-                  "XCD-contiguous tile order"
+        "launch": "mfa_step2: 64x8 columns, intermediate state in LDS at the pulling cell, "
+                  "z-range split for one round of blocks per CU, XCD-contiguous tile order"
                   if pair else abi.get_launch(lattice),
         "wall_s_timed": round(wall, 4),
         "cpu_baseline": cpu,

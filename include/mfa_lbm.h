@@ -127,14 +127,14 @@ int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g,
  * to 1.3 times and writes fout once instead of reading and writing twice.
  * For callers that advance two steps with nothing in between, e.g. two
  * SimulationImpl::advance() calls (src/Simulation/SimulationSetup.cpp:119-139)
- * between outputs.  D2Q9 boxes, periodic or with walls of every kind (the
- * second step's ghost values come from the intermediate state), and fully
- * periodic 3-D boxes (D3Q15, D3Q19, D3Q27), each in FP64 or MFA_F32 storage
- * (the intermediate state rounded to FP32 as a single step stores it).
+ * between outputs.  FP64: every lattice, periodic or with walls of every
+ * kind (the second step's ghost values come from the intermediate state).
+ * MFA_F32 storage: D2Q9 boxes with or without walls and fully periodic 3-D
+ * boxes (the intermediate state rounded to FP32 as a single step stores it).
  * Anything else returns MFA_ERR_UNSUPPORTED and the caller uses mfa_step:
- * 3-D boxes with walls, and FP16 / BF16 storage, where the pass measured
- * slower than two single steps (the words halve the bytes but not the
- * arithmetic, which then bounds the pass).  No counterpart in the
+ * 3-D boxes with walls in FP32, and FP16 / BF16 storage, where the pass
+ * measured slower than two single steps (the words halve the bytes but not
+ * the arithmetic, which then bounds the pass).  No counterpart in the
  * reference. */
 int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g,
               double omega, const double* force, void* stream);
@@ -187,8 +187,9 @@ int mfa_init_equilibrium(mfa_field* rho, mfa_field* vel, mfa_field* f, int latti
                          const double* force, void* stream);
 
 /* FP64 reductions over valid cells, deterministic order (fixed tree), written
- * to out_dev (device memory).  Uses a library-owned device workspace that is
- * allocated on first use per device (hipMalloc) and kept until exit. */
+ * to out_dev (device memory).  Uses a library-owned device workspace, one per
+ * device and stream, allocated on first use (hipMalloc) and kept until exit,
+ * so calls queued on different streams do not share partial sums. */
 int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void* stream);
 
 /* Steady-state residual: out_dev[0] = sum (a-b)^2, out_dev[1] = sum a^2 over

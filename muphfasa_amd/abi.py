@@ -199,8 +199,8 @@ def step(fin: Field, fout: Field, g: GeomBC, omega: float, force=None, stream=No
 
 def step2(fin: Field, fout: Field, g: GeomBC, omega: float, force=None, stream=None):
     """Two steps fin -> fout in one pass, bit-identical to two step() calls
-    (fully periodic 3-D boxes: D3Q15 / D3Q19 in FP64, every 3-D lattice in
-    FP32 storage; MfaError with code -2 elsewhere)."""
+    (FP64: every lattice, periodic or with walls; FP32 storage: 2-D boxes and
+    fully periodic 3-D boxes; MfaError with code -2 elsewhere)."""
     check(lib().mfa_step2(ctypes.byref(fin), ctypes.byref(fout), ctypes.byref(g), omega,
                           _dbl(force), stream_ptr(stream)))
 

@@ -214,6 +214,60 @@ __device__ __forceinline__ void fix_wall(double& fq, const double* __restrict__ 
     fix_wall_at<L, q>(fq, [&](int m) { return fin[m * v.sq + self]; }, g, lo, hi);
 }
 
+// Word of T at a 32-bit byte offset from a wave-uniform base: the load or
+// store addresses the base held in scalar registers plus one 32-bit vector
+// offset, with no 64-bit address arithmetic per population.
+template <class W>
+__device__ __forceinline__ const W* at_bytes(const W* base, unsigned off)
+{
+    return (const W*)((const char*)base + off);
+}
+template <class W>
+__device__ __forceinline__ W* at_bytes(W* base, unsigned off)
+{
+    return (W*)((char*)base + off);
+}
+
+// Element offsets of the pull-source rows of one cell whose y and z are the
+// same across its wavefront: rows[cy+1][cz+1] = y' * sy + z' * sz with the
+// source coordinates wrapped (periodic) or clamped (walls; the clamped value
+// is replaced by the wall value).
+template <int D, class G>
+__device__ __forceinline__ void source_rows(long long (&rows)[3][3], int y, int z, const G& g,
+                                            const View& v)
+{
+    long long oy[3], oz[3];
+#pragma unroll
+    for (int ci = 0; ci < 3; ++ci) {
+        const int c = ci - 1;
+        int sy = y - c, sz = z - c;
+        if (sy < 0) sy = g.per[1] ? g.n[1] - 1 : 0;
+        else if (sy >= g.n[1]) sy = g.per[1] ? 0 : g.n[1] - 1;
+        if (D == 2) sz = z;
+        else if (sz < 0) sz = g.per[2] ? g.n[2] - 1 : 0;
+        else if (sz >= g.n[2]) sz = g.per[2] ? 0 : g.n[2] - 1;
+        oy[ci] = (long long)sy * v.sy;
+        oz[ci] = (long long)sz * v.sz;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) rows[a][b] = oy[a] + oz[b];
+}
+
+// x of the pull source for c = -1, 0, +1.
+template <class G>
+__device__ __forceinline__ void source_x(int (&xs)[3], int x, const G& g)
+{
+#pragma unroll
+    for (int ci = 0; ci < 3; ++ci) {
+        int sx = x - (ci - 1);
+        if (sx < 0) sx = g.per[0] ? g.n[0] - 1 : 0;
+        else if (sx >= g.n[0]) sx = g.per[0] ? 0 : g.n[0] - 1;
+        xs[ci] = sx;
+    }
+}
+
 template <class L, int q>
 struct Unroll {
     template <class Fn>
@@ -312,6 +366,14 @@ __device__ __forceinline__ void moments(const double (&f)[L::Q], const double F[
 // (plain) and :62-92 (forced).  `emit(q, value)` receives each collided
 // population, so the step kernels store them to HBM and the two-step kernel
 // to LDS through one copy of the arithmetic.
+//
+// Populations q and opp(q) are collided together.  Their lattice velocities
+// are opposite, so c.u, c.F and every product of two of them change sign or
+// not, exactly, in IEEE arithmetic with round-to-nearest: c_opp.u = -(c_q.u),
+// 3*(-a) = -(3a), (4.5*(-a))*(-a) = (4.5a)*a, (-a)*(-b) = a*b, and
+// x + (-y) is the operation x - y.  Each shared term is computed once, and
+// every population gets the bits of the reference's term-by-term formula
+// (Equilibrium.H:49-56, Collision.H:42-48).
 template <class L, bool FORCED, class Emit>
 __device__ __forceinline__ void collide(const double (&f)[L::Q], const double omega, const double F0,
                                         const double F1, const double F2, Emit&& emit)
@@ -320,6 +382,7 @@ __device__ __forceinline__ void collide(const double (&f)[L::Q], const double om
     double rho, u0, u1, u2;
     moments<L, FORCED>(f, F, rho, u0, u1, u2);
     const double usq = usq_of<L>(u0, u1, u2);
+    const double usq15 = 1.5 * usq;
     const double omw = 1 - omega;
     double pref = 0.0, uF = 0.0;
     if constexpr (FORCED) {
@@ -328,10 +391,31 @@ __device__ __forceinline__ void collide(const double (&f)[L::Q], const double om
     }
     for_q<L>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        const double fe = feq<L, q>(rho, u0, u1, u2, usq);
-        double out = omw * f[q] + omega * fe;
-        if constexpr (FORCED) out = out + guo<L, q>(u0, u1, u2, F0, F1, F2, pref, uF);
-        emit(qc, out);
+        constexpr int o = L::opp(q);
+        if constexpr (q == o) {            // rest population: c = 0
+            const double fe = feq<L, q>(rho, u0, u1, u2, usq);
+            double out = omw * f[q] + omega * fe;
+            if constexpr (FORCED) out = out + guo<L, q>(u0, u1, u2, F0, F1, F2, pref, uF);
+            emit(qc, out);
+        } else if constexpr (q < o) {      // q and its opposite
+            const double cu = cdot<L, q>(u0, u1, u2);
+            const double t3 = 3.0 * cu;
+            const double t45 = 4.5 * cu * cu;
+            const double wr = L::w(q) * rho;
+            const double fq = wr * (1.0 + t3 + t45 - usq15);
+            const double fo = wr * (1.0 - t3 + t45 - usq15);
+            double oq = omw * f[q] + omega * fq;
+            double oo = omw * f[o] + omega * fo;
+            if constexpr (FORCED) {
+                const double cF = cdot<L, q>(F0, F1, F2);
+                const double p3 = pref * L::w(q) * 3.0;
+                const double tcF = t3 * cF;
+                oq = oq + p3 * (cF + tcF - uF);
+                oo = oo + p3 * (tcF - cF - uF);
+            }
+            emit(qc, oq);
+            emit(std::integral_constant<int, o>{}, oo);
+        }
     });
 }
 

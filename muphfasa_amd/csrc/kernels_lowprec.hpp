@@ -268,60 +268,6 @@ __device__ __forceinline__ void collide_store_dev(const float (&d)[L::Q], T* __r
     });
 }
 
-// Word of T at a 32-bit byte offset from a wave-uniform base: the load or
-// store addresses the base held in scalar registers plus one 32-bit vector
-// offset, with no 64-bit address arithmetic per population.
-template <class W>
-__device__ __forceinline__ const W* at_bytes(const W* base, unsigned off)
-{
-    return (const W*)((const char*)base + off);
-}
-template <class W>
-__device__ __forceinline__ W* at_bytes(W* base, unsigned off)
-{
-    return (W*)((char*)base + off);
-}
-
-// Element offsets of the pull-source rows of one cell whose y and z are the
-// same across its wavefront: rows[cy+1][cz+1] = y' * sy + z' * sz with the
-// source coordinates wrapped (periodic) or clamped (walls; the clamped value
-// is replaced by the wall value).
-template <int D, class G>
-__device__ __forceinline__ void source_rows(long long (&rows)[3][3], int y, int z, const G& g,
-                                            const View& v)
-{
-    long long oy[3], oz[3];
-#pragma unroll
-    for (int ci = 0; ci < 3; ++ci) {
-        const int c = ci - 1;
-        int sy = y - c, sz = z - c;
-        if (sy < 0) sy = g.per[1] ? g.n[1] - 1 : 0;
-        else if (sy >= g.n[1]) sy = g.per[1] ? 0 : g.n[1] - 1;
-        if (D == 2) sz = z;
-        else if (sz < 0) sz = g.per[2] ? g.n[2] - 1 : 0;
-        else if (sz >= g.n[2]) sz = g.per[2] ? 0 : g.n[2] - 1;
-        oy[ci] = (long long)sy * v.sy;
-        oz[ci] = (long long)sz * v.sz;
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) rows[a][b] = oy[a] + oz[b];
-}
-
-// x of the pull source for c = -1, 0, +1.
-template <class G>
-__device__ __forceinline__ void source_x(int (&xs)[3], int x, const G& g)
-{
-#pragma unroll
-    for (int ci = 0; ci < 3; ++ci) {
-        int sx = x - (ci - 1);
-        if (sx < 0) sx = g.per[0] ? g.n[0] - 1 : 0;
-        else if (sx >= g.n[0]) sx = g.per[0] ? 0 : g.n[0] - 1;
-        xs[ci] = sx;
-    }
-}
-
 // Pull of one cell through wave-uniform source rows (see source_rows).
 template <class L, bool WALLS, class T, class G>
 __device__ __forceinline__ void pull_dev_rows(float (&d)[L::Q], const T* __restrict__ fin, int x, int y,

@@ -1,0 +1,330 @@
+// kernels_step2.hpp -- two time steps per pass on a 3-D box (temporal
+// blocking), periodic or with walls, with the intermediate state held in LDS
+// at the cell that will pull it.
+//
+// A block owns a 64 x TY column of output cells and marches through its
+// z-range.  Phase A computes the first step on the column grown by one cell
+// in x and y, plane by plane; phase B pulls the second step and writes it to
+// HBM.  Population q that phase A leaves in grown cell g of plane p is pulled
+// by phase B at cell g + c_q (x, y) of plane p + c_q,z, so only the values
+// whose destination lies in the tile are kept, and they are kept at that
+// destination: slot [q][cell of the tile].  Every plane of every population
+// then needs (64+2)*TY words instead of 66*(TY+2), which makes room for a
+// taller tile: its grown column re-reads fewer rows, and the re-read rows
+// are what the pass is bound by (L1-to-L2 transactions, not arithmetic).
+// Phase B reads the slot of its own cell for every q.
+//
+// At a wall the second step's ghost value of population q at cell x is a
+// function of x's own intermediate populations (src/LBM/BoundaryConditions.H:
+// 124-189 and src/LBM/BCTypes/), which phase A holds in registers when it
+// computes x; it writes the ghost value into the slot that phase B of x reads
+// for q, the slot no grown cell pushes into because the pull source lies
+// outside the domain.  Per cell the arithmetic is that of k_step (pull,
+// collide, wall values), so a pass equals two single steps bit for bit.
+//
+// Schedule.  Iteration j computes phase A of plane j+1 (from loads issued in
+// the iteration before), issues the loads of plane j+2, and after a barrier
+// computes phase B of plane j; a second barrier closes the iteration.
+#pragma once
+
+#include "kernels_common.hpp"
+
+namespace mfa {
+
+// Rings of planes by c_z class k = c_z + 1.  Phase B of plane z reads class 0
+// from source plane z+1 (written in the same iteration, before the barrier),
+// class 1 from plane z (one iteration earlier) and class 2 from plane z-1
+// (two iterations earlier): rings of 1, 2 and 3 planes.  The ghost values of
+// a wall are written by the cell that pulls them, at its own plane z, into
+// the slot of source plane z - c_z; for class 0 that is plane z+1, one
+// iteration before phase B of z reads it, while the slot of plane z is still
+// being read: with walls class 0 gets a second slot.
+template <class L, int TC, bool WALLS>
+struct PushRings {
+    static_assert(L::D == 3, "three-dimensional lattices only");
+    static constexpr int N0 = cz_count<L>(0), N1 = cz_count<L>(1), N2 = cz_count<L>(2);
+    static constexpr int S0 = WALLS ? 2 : 1, S1 = 2, S2 = 3;
+    static constexpr int PLANES = S0 * N0 + S1 * N1 + S2 * N2;
+    double c0[S0][N0][TC];
+    double c1[S1][N1][TC];
+    double c2[S2][N2][TC];
+    // ring slot of one plane in each class (or of the three planes phase B
+    // reads); computed once per plane, wave-uniform
+    struct Pos {
+        int k0, k1, k2;
+    };
+    // slots of plane p (p >= Z0 - 2; the offset 12 is a multiple of every ring length)
+    static __device__ __forceinline__ Pos plane(int p, int Z0)
+    {
+        const int rel = p - Z0 + 12;
+        return Pos{rel % S0, rel % S1, rel % S2};
+    }
+    // slots phase B of plane z reads: class k from plane z + 1 - k
+    static __device__ __forceinline__ Pos pulled_by(int z, int Z0)
+    {
+        const int rel = z - Z0 + 12;
+        return Pos{(rel + 1) % S0, rel % S1, (rel - 1) % S2};
+    }
+    template <int q>
+    __device__ __forceinline__ double& at(const Pos& ps, int cell)
+    {
+        constexpr int k = L::c(q, 2) + 1, r = cz_rank<L>(q);
+        if constexpr (k == 0) return c0[ps.k0][r][cell];
+        else if constexpr (k == 1) return c1[ps.k1][r][cell];
+        else return c2[ps.k2][r][cell];
+    }
+};
+
+template <int TY_>
+struct PushTile {
+    static constexpr int TX = 64, TY = TY_, TC = TX * TY;
+    static constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
+    static constexpr int THREADS = (EC + 63) / 64 * 64;
+    // slots of one tile row, padded by a cell on either side: a wavefront
+    // of a grown row pushes its 64 lanes to distinct words
+    static constexpr int SX = TX + 2, SC = SX * TY;
+    // one block per CU: its waves over the CU's four SIMDs
+    static constexpr int WAVES_PER_EU = (THREADS / 64 + 3) / 4;
+};
+
+// Grown-column cell (ex, ey) of thread tid in phase A: the tile's own rows
+// first, in phase B's order (thread tid < TC computes output cell
+// (tid % 64, tid / 64) in both phases), then the rows below and above the
+// tile, then the two side columns.  Every wavefront but the last covers one
+// row of 64 cells, so its loads are aligned runs.
+template <class T>
+__device__ __forceinline__ void grown_cell(int tid, int& ex, int& ey)
+{
+    if (tid < T::TC) {
+        ex = tid % T::TX + 1;
+        ey = tid / T::TX + 1;
+    } else if (tid < T::TC + 2 * T::TX) {
+        const int t = tid - T::TC;
+        ex = t % T::TX + 1;
+        ey = t < T::TX ? 0 : T::TY + 1;
+    } else {
+        const int t = tid - T::TC - 2 * T::TX;
+        ex = (t & 1) ? T::TX + 1 : 0;
+        ey = t >> 1;
+    }
+}
+
+// Wall values of one cell: for every population q whose pull source x - c_q
+// lies behind a non-periodic face (edge flags lo/hi of the cell), emit(q, v)
+// receives the ghost value the reference's BCManager writes there
+// (src/LBM/BoundaryConditions.H:124-189; BCTypes/BounceBack.H:56-63,
+// OutflowKernel.H:26-38, PressureOutletKernel.H:29-64), computed from s, the
+// populations of the cell itself.  The same expressions as wall_value_at,
+// with the density and momentum of s that a pressure outlet needs summed
+// once per cell rather than once per population.
+template <class L, class G, class Emit>
+__device__ __forceinline__ void wall_fill(const double (&s)[L::Q], const G& g, const int lo[3],
+                                          const int hi[3], bool any_outlet, Emit&& emit)
+{
+    double rho = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
+    if (any_outlet) {
+        for_q<L>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const double fm = s[m];
+            rho += fm;
+            j0 += L::c(m, 0) * fm;
+            j1 += L::c(m, 1) * fm;
+            if constexpr (L::D == 3) j2 += L::c(m, 2) * fm;
+        });
+    }
+    for_q<L>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        int code[3] = {0, 0, 0};
+        bool wall = false;
+#pragma unroll
+        for (int d = 0; d < L::D; ++d) {
+            const int c = L::c(q, d);
+            if (c == 1 && lo[d]) { code[d] = -1; wall = true; }
+            if (c == -1 && hi[d]) { code[d] = 1; wall = true; }
+        }
+        if (!wall) return;
+        const int idx = (code[0] + 1) + 3 * (code[1] + 1) + 9 * (code[2] + 1) * (L::D == 3);
+        const int t = g.rtype[idx];
+        constexpr int m = L::opp(q);
+        double v;
+        if (t == MFA_BC_MOVING_WALL || t == MFA_BC_NO_SLIP || t == MFA_BC_INFLOW) {
+            v = s[m] + bb_contrib<L, m>(g.rrho[idx], g.ru[idx][0], g.ru[idx][1], g.ru[idx][2]);
+        } else if (t == MFA_BC_OUTFLOW) {
+            v = s[q];
+        } else if (t == MFA_BC_PRESSURE_OUTLET) {
+            const double inv_rho = 1.0 / rho;
+            const double cu = cdot<L, q>(j0, j1, j2) * inv_rho;
+            double uu;
+            if constexpr (L::D == 2) uu = (j0 * j0 + j1 * j1) * inv_rho * inv_rho;
+            else uu = (j0 * j0 + j1 * j1 + j2 * j2) * inv_rho * inv_rho;
+            const double feq_even =
+                L::w(q) * g.rrho[idx] * (1.0 + 0.5 * cu * cu * 3.0 * 3.0 - 0.5 * uu * 3.0);
+            v = -s[m] + 2.0 * feq_even;
+        } else {
+            v = __builtin_nan("");
+        }
+        emit(qc, v);
+    });
+}
+
+template <class L, bool FORCED, bool WALLS, class T>
+__global__ __launch_bounds__(T::THREADS) __attribute__((amdgpu_waves_per_eu(T::WAVES_PER_EU)))
+void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const Geo g, const View vi,
+              const View vo, const double omega, const double F0, const double F1, const double F2,
+              const int TZ)
+{
+    __shared__ PushRings<L, T::SC, WALLS> lds;
+    // A value whose destination lies outside the tile goes to a dummy row
+    // shared by every population and plane, so the writes need no branch.
+    __shared__ double dummy[T::SX];
+    // The boundary tables, indexed at run time by the ghost code, are read
+    // from LDS: indexed in the kernel argument they would be copied to
+    // scratch memory.
+    __shared__ signed char s_rtype[WALLS ? 27 : 1];
+    __shared__ double s_rrho[WALLS ? 27 : 1];
+    __shared__ double s_ru[WALLS ? 27 : 1][3];
+    using Rings = PushRings<L, T::SC, WALLS>;
+    using Pos = typename Rings::Pos;
+    const int tid = threadIdx.x;
+    GeoRef gr;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        gr.n[d] = g.n[d];
+        gr.per[d] = g.per[d];
+    }
+    gr.walls = g.walls;
+    gr.rtype = s_rtype;
+    gr.rrho = s_rrho;
+    gr.ru = s_ru;
+    bool any_outlet = false;
+    if constexpr (WALLS) {
+#pragma unroll
+        for (int i = 0; i < 27; ++i) any_outlet = any_outlet || g.rtype[i] == MFA_BC_PRESSURE_OUTLET;
+        if (tid == 0) {
+#pragma unroll
+            for (int i = 0; i < 27; ++i) {
+                s_rtype[i] = g.rtype[i];
+                s_rrho[i] = g.rrho[i];
+                s_ru[i][0] = g.ru[i][0];
+                s_ru[i][1] = g.ru[i][1];
+                s_ru[i][2] = g.ru[i][2];
+            }
+        }
+        __syncthreads();
+    }
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const bool px = !WALLS || g.per[0], py = !WALLS || g.per[1], pz = !WALLS || g.per[2];
+    // The hardware deals consecutive blocks round-robin to the 8 XCDs.  Give
+    // each XCD one contiguous run of tiles (x fastest) instead, so that tiles
+    // whose grown columns overlap run in the same L2 and their shared rims
+    // and the cache lines their row ends straddle are fetched from HBM once.
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned per = nb / kXcds, rem = nb % kXcds, xcd = b % kXcds;
+    const unsigned t = xcd * per + (xcd < rem ? xcd : rem) + b / kXcds;
+    const int X0 = (int)(t % gridDim.x) * T::TX, Y0 = (int)((t / gridDim.x) % gridDim.y) * T::TY;
+    const int Z0 = (int)(t / (gridDim.x * gridDim.y)) * TZ;
+    const int Z1 = Z0 + TZ < nz ? Z0 + TZ : nz;
+
+    // phase A: one cell of the grown column (periodic images; a grown cell
+    // behind a wall is not needed)
+    int ex, ey;
+    grown_cell<T>(tid, ex, ey);
+    bool inA = tid < T::EC;
+    int gx = X0 - 1 + ex, gy = Y0 - 1 + ey;
+    if (px) gx = (gx % nx + nx) % nx;
+    else inA = inA && gx >= 0 && gx < nx;
+    if (py) gy = (gy % ny + ny) % ny;
+    else inA = inA && gy >= 0 && gy < ny;
+    // phase B: one output cell
+    const int ox = tid % T::TX, oy = tid / T::TX;
+    const bool inB = tid < T::TC && X0 + ox < nx && Y0 + oy < ny;
+    const int own = ox + 1 + T::SX * oy;    // slot of the output cell
+
+    auto has_plane = [&](int p) { return pz || (p >= 0 && p < nz); };
+    // edge flags of a cell of the domain at (x, y, z)
+    auto edges = [&](int x, int y, int z, int (&lo)[3], int (&hi)[3]) {
+        const int pos[3] = {x, y, z};
+        bool edge = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = !g.per[d] && pos[d] == 0;
+            hi[d] = !g.per[d] && pos[d] == g.n[d] - 1;
+            edge = edge || lo[d] || hi[d];
+        }
+        return edge;
+    };
+    auto loadA = [&](double (&P)[L::Q], int p) {
+        if (!inA || !has_plane(p)) return;
+        const int pw = (p % nz + nz) % nz;
+        pull<L, 0, false, 0>(P, fin, gx, gy, pw, gr, vi);
+        if constexpr (WALLS) {
+            // pulls from behind a wall take the first step's ghost values,
+            // from the populations of this cell
+            int lo[3], hi[3];
+            if (edges(gx, gy, pw, lo, hi)) {
+                const long long self = gx + gy * vi.sy + pw * vi.sz;
+                double s[L::Q];
+                for_q<L>([&](auto mc) {
+                    constexpr int m = decltype(mc)::value;
+                    s[m] = fin[m * vi.sq + self];
+                });
+                wall_fill<L>(s, gr, lo, hi, any_outlet,
+                             [&](auto qc, double v) { P[decltype(qc)::value] = v; });
+            }
+        }
+    };
+    auto push = [&](auto qc, const Pos& ps, double out) {
+        constexpr int q = decltype(qc)::value;
+        const int dx = ex - 1 + L::c(q, 0), dy = ey - 1 + L::c(q, 1);
+        double* dst = (dx >= 0 && dx < T::TX && dy >= 0 && dy < T::TY)
+                          ? &lds.template at<q>(ps, dx + 1 + T::SX * dy)
+                          : &dummy[ex];
+        *dst = out;
+    };
+    auto storeA = [&](const double (&P)[L::Q], int p) {
+        const Pos pw = Rings::plane(p, Z0);
+        if (!inA || !has_plane(p)) return;
+        if constexpr (!WALLS) {
+            collide<L, FORCED>(P, omega, F0, F1, F2, [&](auto qc, double out) { push(qc, pw, out); });
+        } else {
+            double o[L::Q];
+            collide<L, FORCED>(P, omega, F0, F1, F2,
+                               [&](auto qc, double out) { o[decltype(qc)::value] = out; });
+            for_q<L>([&](auto qc) { push(qc, pw, o[decltype(qc)::value]); });
+            // the second step's ghost values at a tile cell on an edge go to
+            // the slots phase B of this plane reads
+            int lo[3], hi[3];
+            if (edges(X0 + ox, Y0 + oy, p, lo, hi) && inB) {
+                const Pos pr = Rings::pulled_by(p, Z0);
+                wall_fill<L>(o, gr, lo, hi, any_outlet, [&](auto qc, double v) {
+                    lds.template at<decltype(qc)::value>(pr, own) = v;
+                });
+            }
+        }
+    };
+    auto phaseB = [&](int z) {
+        const Pos pr = Rings::pulled_by(z, Z0);
+        if (!inB) return;
+        double f[L::Q];
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = lds.template at<q>(pr, own);
+        });
+        collide_store<L, FORCED>(f, fout, (X0 + ox) + (long long)(Y0 + oy) * vo.sy + (long long)z * vo.sz,
+                                 vo, omega, F0, F1, F2);
+    };
+
+    // iterations Z0-2 and Z0-1 compute the two planes below the block's range
+    double P[L::Q];
+    loadA(P, Z0 - 1);
+    for (int j = Z0 - 2; j < Z1; ++j) {
+        storeA(P, j + 1);
+        if (j + 2 <= Z1) loadA(P, j + 2);
+        __syncthreads();
+        if (j >= Z0) phaseB(j);
+        __syncthreads();
+    }
+}
+
+}  // namespace mfa

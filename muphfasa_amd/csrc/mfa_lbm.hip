@@ -18,11 +18,13 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <type_traits>
 
 #include "kernels_common.hpp"
+#include "kernels_step2.hpp"
 
 namespace mfa {
 
@@ -319,89 +321,15 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double* __re
     }
 }
 
-// ------------------------------------------------------ two-step kernel --
-// Two fused time steps per pass on a fully periodic 3-D box (temporal
-// blocking).  A block owns a TX x TY column of output cells and marches
-// through its z-range.  Phase A computes the intermediate state (the first
-// step) on the column grown by one cell in x and y, plane by plane, into
-// LDS rings (CzRings); phase B pulls the second step from those planes and
-// writes it to HBM.  The intermediate state never leaves the chip, so a
-// pair of steps reads the source lattice once (plus the one-cell rim, which
-// neighbouring blocks also read and L2 mostly serves) and writes once.  Per
-// cell the arithmetic is that of k_step (pull, collide), so two-step results
-// equal two single steps bit for bit.  The next plane's pulls are issued
-// before the current plane's second step, so they are in flight meanwhile.
+// ------------------------------------------------------ two-step kernels --
+// 3-D boxes: k_step2p (kernels_step2.hpp).  The 2-D kernel below keeps its
+// TX x TY tile grown by one cell in LDS.
 template <int TX_, int TY_>
 struct Step2Tile {                          // (TX+2) x (TY+2) grown column
     static constexpr int TX = TX_, TY = TY_;
     static constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
     static constexpr int THREADS = (EC + 63) / 64 * 64;
 };
-
-template <class L, bool FORCED, class T>
-__global__ __launch_bounds__(T::THREADS) void k_step2(
-    const double* __restrict__ fin, double* __restrict__ fout, const Geo g, const View vi,
-    const View vo, const double omega, const double F0, const double F1, const double F2,
-    const int TZ)
-{
-    __shared__ CzRings<L, double, T::EC> lds;
-    const int tid = threadIdx.x;
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    // The hardware deals consecutive blocks round-robin to the 8 XCDs.  Give
-    // each XCD one contiguous run of tiles (x fastest) instead, so that tiles
-    // whose grown columns overlap run in the same L2: their shared rims and
-    // the cache lines their row ends straddle are fetched from HBM once
-    // (measured: 0.736 -> 0.638 ms per step at 256^3 D3Q19).
-    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const unsigned per = nb / kXcds, rem = nb % kXcds, k = b % kXcds;
-    const unsigned t = k * per + (k < rem ? k : rem) + b / kXcds;
-    const int bx = (int)(t % gridDim.x), by = (int)((t / gridDim.x) % gridDim.y);
-    const int bz = (int)(t / (gridDim.x * gridDim.y));
-    const int X0 = bx * T::TX, Y0 = by * T::TY, Z0 = bz * TZ;
-    const int Z1 = Z0 + TZ < nz ? Z0 + TZ : nz;
-    // phase A: one cell of the grown column (periodic images, any box size)
-    const bool inA = tid < T::EC;
-    const int exA = tid % T::EX, eyA = tid / T::EX;
-    const int gx = ((X0 - 1 + exA) % nx + nx) % nx;
-    const int gy = ((Y0 - 1 + eyA) % ny + ny) % ny;
-    // phase B: one output cell
-    const int ox = tid % T::TX, oy = tid / T::TX;
-    const bool inB = tid < T::TX * T::TY && X0 + ox < nx && Y0 + oy < ny;
-    double P[L::Q];
-    auto loadA = [&](int z) {
-        if (inA) pull<L, 0, false, 0>(P, fin, gx, gy, ((z % nz) + nz) % nz, g, vi);
-    };
-    auto storeA = [&](int z) {
-        if (!inA) return;
-        collide<L, FORCED>(P, omega, F0, F1, F2, [&](auto qc, double out) {
-            constexpr int q = decltype(qc)::value;
-            lds.template at<q>(z, Z0, tid) = out;
-        });
-    };
-    loadA(Z0 - 1);
-    storeA(Z0 - 1);
-    loadA(Z0);
-    storeA(Z0);
-    loadA(Z0 + 1);
-    for (int z = Z0; z < Z1; ++z) {
-        storeA(z + 1);
-        if (z + 2 <= Z1) loadA(z + 2);
-        __syncthreads();
-        if (inB) {
-            double f[L::Q];
-            for_q<L>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
-                f[q] = lds.template at<q>(z - cz, Z0, (ox + 1 - cx) + T::EX * (oy + 1 - cy));
-            });
-            collide_store<L, FORCED>(f, fout,
-                                     (X0 + ox) + (long long)(Y0 + oy) * vo.sy + (long long)z * vo.sz,
-                                     vo, omega, F0, F1, F2);
-        }
-        __syncthreads();
-    }
-}
 
 // Two steps per pass on a 2-D box, periodic or with walls.  A block owns a
 // TX x TY tile of output cells.  Phase A computes the first step on the tile
@@ -626,6 +554,29 @@ static void launch_shape(const int n[3], dim3& grid, dim3& block, int grow)
 // planes are recomputed by the block below: 2/64 extra work).
 constexpr int kStep2Planes = 64;
 
+// z-planes per block of the 3-D two-step kernel (one block per CU: its LDS
+// fills the CU).  Each block also computes the two planes below its range,
+// and blocks run in rounds of one per CU, so the time goes as
+// rounds * (planes + 2); the split of the z-range that minimises it, e.g.
+// 256^3 in 64 x 8 columns (128 of them on 256 CUs): two blocks of 128
+// planes per column, one round.
+static int step2_planes(int tiles, int nz, int ncu)
+{
+    if (ncu < 1) ncu = 1;
+    long long best = -1;
+    int best_tz = nz;
+    for (int zs = 1; zs <= 64 && zs <= nz; ++zs) {
+        const int tz = (nz + zs - 1) / zs;
+        const long long blocks = (long long)tiles * ((nz + tz - 1) / tz);
+        const long long cost = ((blocks + ncu - 1) / ncu) * (long long)(tz + 2);
+        if (best < 0 || cost < best) {
+            best = cost;
+            best_tz = tz;
+        }
+    }
+    return best_tz;
+}
+
 static std::mutex g_cfg_mu;
 static const mfa_launch k_default_cfg[4] = {
     {128, 2, 1, 2}, {128, 2, 1, 2}, {128, 2, 1, 2}, {128, 2, 1, 2}};
@@ -727,22 +678,38 @@ static int check_2d(const mfa_field* f, int lat)
     return MFA_OK;
 }
 
-// Reduction workspace: one per device, allocated once, kept until exit.
-static std::mutex g_ws_mu;
-static double* g_ws[64] = {nullptr};
-
-static int workspace(double** out)
+// Compute units of the current device, read once per device.
+static int cu_count(int* out)
 {
+    static std::mutex mu;
+    static int cus[64] = {0};
     int dev = 0;
     MFA_HIP_CHECK(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return fail(MFA_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!cus[dev]) MFA_HIP_CHECK(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+    *out = cus[dev];
+    return MFA_OK;
+}
+
+// Reduction workspace: one per device and stream, allocated on first use and
+// kept until exit, so that reductions queued on different streams do not
+// share their partial sums.
+static std::mutex g_ws_mu;
+static std::map<std::pair<int, hipStream_t>, double*> g_ws;
+
+static int workspace(hipStream_t st, double** out)
+{
+    int dev = 0;
+    MFA_HIP_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_ws_mu);
-    if (!g_ws[dev]) {
+    double*& ws = g_ws[std::make_pair(dev, st)];
+    if (!ws) {
         void* p = nullptr;
         MFA_HIP_CHECK(hipMalloc(&p, sizeof(double) * kRedBlocks * kRedMaxOut));
-        g_ws[dev] = (double*)p;
+        ws = (double*)p;
     }
-    *out = g_ws[dev];
+    *out = ws;
     return MFA_OK;
 }
 
@@ -879,38 +846,51 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
         MFA_HIP_CHECK(hipGetLastError());
         return (int)MFA_OK;
     }
-    if (geo.walls)
-        return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: 3-D boxes with walls are not offered (use mfa_step)");
-    const int TZ = geo.n[2] < kStep2Planes ? geo.n[2] : kStep2Planes;
-    if (fin->dtype != MFA_F64)
+    if (fin->dtype != MFA_F64) {
+        if (geo.walls)
+            return fail(MFA_ERR_UNSUPPORTED,
+                        "mfa_step2: 3-D boxes with walls need FP64 storage (use mfa_step)");
+        const int TZ = geo.n[2] < kStep2Planes ? geo.n[2] : kStep2Planes;
         return step2_dev(fin, fout, geo, lat, omega, force, TZ, (hipStream_t)stream);
-
+    }
     const View vi = view_of(*fin), vo = view_of(*fout);
     const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0, F2 = force ? force[2] : 0.0;
-    auto run = [&](auto Lc) {
+    int ncu = 0;
+    if ((rc = cu_count(&ncu))) return rc;
+    auto run = [&](auto Lc, auto Tc, auto walls) {
         using L = decltype(Lc);
-        // Columns (measured, forced): D3Q27 384^3 32 x 8, 2.86 ms per step
-        // against 3.09 with 16 x 16 and 4.36 for two single steps; D3Q19
-        // 256^3 64 x 4, 0.583 ms against 0.586 with 32 x 8, 0.642 with
-        // 16 x 16 and 0.852 for two single steps.  Wider rows touch fewer
-        // cache lines per wavefront; the per-c_z rings leave LDS room for them.
-        using T = std::conditional_t<L::Q == 27, Step2Tile<32, 8>, Step2Tile<64, 4>>;
+        using T = decltype(Tc);
+        constexpr bool W = decltype(walls)::value;
+        const int tiles = ((geo.n[0] + T::TX - 1) / T::TX) * ((geo.n[1] + T::TY - 1) / T::TY);
+        const int TZ = step2_planes(tiles, geo.n[2], ncu);
         const dim3 grid((geo.n[0] + T::TX - 1) / T::TX, (geo.n[1] + T::TY - 1) / T::TY,
                         (geo.n[2] + TZ - 1) / TZ);
         const double* a = (const double*)fin->data;
         double* b = (double*)fout->data;
         if (force)
-            hipLaunchKernelGGL((k_step2<L, true, T>), grid, dim3(T::THREADS), 0, (hipStream_t)stream, a, b,
+            hipLaunchKernelGGL((k_step2p<L, true, W, T>), grid, dim3(T::THREADS), 0, (hipStream_t)stream, a, b,
                                geo, vi, vo, omega, F0, F1, F2, TZ);
         else
-            hipLaunchKernelGGL((k_step2<L, false, T>), grid, dim3(T::THREADS), 0, (hipStream_t)stream, a,
+            hipLaunchKernelGGL((k_step2p<L, false, W, T>), grid, dim3(T::THREADS), 0, (hipStream_t)stream, a,
                                b, geo, vi, vo, omega, F0, F1, F2, TZ);
         MFA_HIP_CHECK(hipGetLastError());
         return (int)MFA_OK;
     };
-    if (lat == MFA_D3Q15) return run(Lattice<MFA_D3Q15>{});
-    if (lat == MFA_D3Q19) return run(Lattice<MFA_D3Q19>{});
-    return run(Lattice<MFA_D3Q27>{});
+    // Columns (D3Q19 256^3 forced, ms per step): periodic 64 x 8, 0.513,
+    // against 0.554 for 64 x 4 and 0.577 for the grown-column layout of
+    // 64 x 4; D3Q15 64 x 8, 0.379; D3Q27 (27 rings of planes) 64 x 4, 2.81
+    // at 384^3.  With walls class 0 needs a second ring slot and the wall
+    // values hold the cell's populations in registers: 64 x 4.
+    using Yes = std::true_type;
+    using No = std::false_type;
+    if (geo.walls) {
+        if (lat == MFA_D3Q15) return run(Lattice<MFA_D3Q15>{}, PushTile<4>{}, Yes{});
+        if (lat == MFA_D3Q19) return run(Lattice<MFA_D3Q19>{}, PushTile<4>{}, Yes{});
+        return run(Lattice<MFA_D3Q27>{}, PushTile<4>{}, Yes{});
+    }
+    if (lat == MFA_D3Q15) return run(Lattice<MFA_D3Q15>{}, PushTile<8>{}, No{});
+    if (lat == MFA_D3Q19) return run(Lattice<MFA_D3Q19>{}, PushTile<8>{}, No{});
+    return run(Lattice<MFA_D3Q27>{}, PushTile<4>{}, No{});
 }
 
 int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, double omega,
@@ -1135,7 +1115,7 @@ int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void*
     if (what == MFA_RED_COMP_SUM && f->ncomp > kRedMaxOut)
         return fail(MFA_ERR_INVALID, "mfa_reduce: more than 27 components");
     double* ws = nullptr;
-    if ((rc = workspace(&ws))) return rc;
+    if ((rc = workspace((hipStream_t)stream, &ws))) return rc;
     const Geo geo = geo_of(*f, nullptr, lattice);
     const View v = view_of(*f);
     hipStream_t st = (hipStream_t)stream;
@@ -1182,7 +1162,7 @@ int mfa_residual(const mfa_field* a, const mfa_field* b, double* out_dev, void* 
         return fail(MFA_ERR_INVALID, "mfa_residual: shapes differ");
     if (!out_dev) return fail(MFA_ERR_INVALID, "mfa_residual: null output");
     double* ws = nullptr;
-    if ((rc = workspace(&ws))) return rc;
+    if ((rc = workspace((hipStream_t)stream, &ws))) return rc;
     const Geo geo = geo_of(*a, nullptr, MFA_D3Q19);
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_residual, dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)a->data,

@@ -36,7 +36,7 @@ def _pick(d, needle):
 
 
 def summarize(fetch_csv, write_csv, n, Q=19, elem=8, step="k_step", copy="k_copy",
-              copy_bytes=None, commands=(), steps_per_launch=1, workload=None):
+              copy_bytes=None, commands=(), steps_per_launch=1, workload=None, commit=None):
     fe = per_kernel(fetch_csv, "FETCH_SIZE")
     wr = per_kernel(write_csv, "WRITE_SIZE")
     ks, kc = _pick(fe, step), _pick(fe, copy)
@@ -60,6 +60,7 @@ def summarize(fetch_csv, write_csv, n, Q=19, elem=8, step="k_step", copy="k_copy
         "hbm_read_bytes_per_launch": rd, "hbm_write_bytes_per_launch": wt,
         "hbm_bytes_per_launch": rd + wt, "algorithmic_bytes_per_launch": algo,
         "traffic_over_algorithmic": (rd + wt) / algo, "commands": list(commands),
+        "commit": commit,
     }
 
 
@@ -91,12 +92,13 @@ def main():
     p.add_argument("--step", default="k_step", help="name fragment of the measured kernel")
     p.add_argument("--steps-per-launch", type=int, default=1)
     p.add_argument("--workload", default=None)
+    p.add_argument("--commit", default=None, help="source commit the counters were measured at")
     a = p.parse_args()
     if a.stats:
         print(stats_table(a.stats, a.top))
         return
     r = summarize(a.fetch_csv, a.write_csv, a.n, step=a.step, commands=a.command,
-                  steps_per_launch=a.steps_per_launch, workload=a.workload)
+                  steps_per_launch=a.steps_per_launch, workload=a.workload, commit=a.commit)
     s = json.dumps(r, indent=1)
     if a.out:
         with open(a.out, "w") as fh:

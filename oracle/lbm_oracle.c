@@ -14,6 +14,7 @@
 #include <math.h>
 #include <stddef.h>
 #include <string.h>
+#include <stdlib.h>
 #include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -524,13 +525,18 @@ int orc_run(int lattice, double* fa, double* fb, const int* n, double omega,
 {
     double* cur = fa;
     double* nxt = fb;
+    int walls = 0;
+    for (int d = 0; d < k_lat[lattice].D; ++d)
+        if (!periodic[d]) walls = 1;
     for (int s = 0; s < nsteps; ++s) {
         orc_step(lattice, cur, nxt, n, omega, F, nthreads);
         double* t = cur;
         cur = nxt;
         nxt = t;
         orc_fill_periodic(lattice, cur, n, periodic);
-        if (orc_apply_bc(lattice, cur, n, periodic, rtype, rrho, ru) != 0) return -2;
+        /* a fully periodic box has no ghost behind a wall: the boundary
+         * fill would visit every ghost and write none */
+        if (walls && orc_apply_bc(lattice, cur, n, periodic, rtype, rrho, ru) != 0) return -2;
     }
     return cur == fa ? 0 : 1;
 }
@@ -546,4 +552,123 @@ double orc_time_run(int lattice, double* fa, double* fb, const int* n, double om
     clock_gettime(CLOCK_MONOTONIC, &t1);
     if (r < 0) return -1.0;
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* ---- the reference's box layout, for the CPU baseline.  The domain of a
+ *      fully periodic box is cut into boxes of at most mgs cells per side
+ *      (AMReX max_grid_size; 64 by default, src/IO/InputParser.cpp), each
+ *      with a one-cell ghost layer.  A step pull-collides every box (OpenMP
+ *      over boxes, as MFIter + omp parallel run pullCollideFusedMF,
+ *      src/LBM/Collision.cpp:95-143), swaps, and fills every ghost cell of
+ *      every box from the valid cell of its periodic image in the box that
+ *      owns it (FillBoundary, src/Simulation/SimulationSetup.cpp:221-234).
+ *      Starts from the valid cells of the single-box fab f and writes the
+ *      final state back into them; returns the seconds taken by the nsteps
+ *      steps that follow nwarm untimed ones, or < 0 on bad arguments. ---- */
+typedef struct box_t {
+    int lo[3], n[3];
+    fab_t fb;
+    double *a, *b;
+} box_t;
+
+static void fill_boxes(const orc_lattice* L, box_t* bx, int nbox, const int* n, int mgs,
+                       const int* nb, int use_b)
+{
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int bi = 0; bi < nbox; ++bi) {
+        box_t* B = &bx[bi];
+        double* dst = use_b ? B->b : B->a;
+        for (int k = -B->fb.g[2]; k < B->n[2] + B->fb.g[2]; ++k)
+            for (int j = -B->fb.g[1]; j < B->n[1] + B->fb.g[1]; ++j)
+                for (int i = -B->fb.g[0]; i < B->n[0] + B->fb.g[0]; ++i) {
+                    const int c[3] = {i, j, k};
+                    int ghost = 0, G[3], ob[3], l[3];
+                    for (int d = 0; d < 3; ++d) {
+                        if (c[d] < 0 || c[d] >= B->n[d]) ghost = 1;
+                        G[d] = ((B->lo[d] + c[d]) % n[d] + n[d]) % n[d];
+                        ob[d] = G[d] / mgs;
+                        l[d] = G[d] - ob[d] * mgs;
+                    }
+                    if (!ghost) continue;
+                    const box_t* S = &bx[ob[0] + nb[0] * (ob[1] + nb[1] * ob[2])];
+                    const double* src = use_b ? S->b : S->a;
+                    for (int q = 0; q < L->Q; ++q)
+                        dst[fidx(&B->fb, i, j, k, q)] = src[fidx(&S->fb, l[0], l[1], l[2], q)];
+                }
+    }
+}
+
+double orc_time_run_boxes(int lattice, double* f, const int* n, int mgs, double omega,
+                          const double* F, int nwarm, int nsteps, int nthreads)
+{
+    const orc_lattice* L = &k_lat[lattice];
+    if (mgs < 1 || nwarm < 0 || nsteps < 1) return -1.0;
+    if (L->D == 2 && n[2] != 1) return -1.0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    int nb[3];
+    for (int d = 0; d < 3; ++d) nb[d] = (d < L->D) ? (n[d] + mgs - 1) / mgs : 1;
+    const int nbox = nb[0] * nb[1] * nb[2];
+    box_t* bx = (box_t*)calloc((size_t)nbox, sizeof(box_t));
+    if (!bx) return -1.0;
+    fab_t whole = make_fab(L, n);
+    int ok = 1;
+    for (int bk = 0; bk < nb[2]; ++bk)
+        for (int bj = 0; bj < nb[1]; ++bj)
+            for (int bi = 0; bi < nb[0]; ++bi) {
+                box_t* B = &bx[bi + nb[0] * (bj + nb[1] * bk)];
+                const int bb[3] = {bi, bj, bk};
+                for (int d = 0; d < 3; ++d) {
+                    B->lo[d] = (d < L->D) ? bb[d] * mgs : 0;
+                    B->n[d] = (d < L->D) ? ((n[d] - B->lo[d]) < mgs ? n[d] - B->lo[d] : mgs) : 1;
+                }
+                B->fb = make_fab(L, B->n);
+                B->a = (double*)malloc(sizeof(double) * (size_t)B->fb.sq * L->Q);
+                B->b = (double*)malloc(sizeof(double) * (size_t)B->fb.sq * L->Q);
+                if (!B->a || !B->b) { ok = 0; continue; }
+                for (int q = 0; q < L->Q; ++q)
+                    for (int k = 0; k < B->n[2]; ++k)
+                        for (int j = 0; j < B->n[1]; ++j)
+                            for (int i = 0; i < B->n[0]; ++i)
+                                B->a[fidx(&B->fb, i, j, k, q)] =
+                                    f[fidx(&whole, B->lo[0] + i, B->lo[1] + j, B->lo[2] + k, q)];
+            }
+    double secs = -1.0;
+    if (ok) {
+        fill_boxes(L, bx, nbox, n, mgs, nb, 0);
+        int cur_b = 0;   /* the current state lives in a (0) or b (1) */
+        struct timespec t0, t1;
+        for (int s = 0; s < nwarm + nsteps; ++s) {
+            if (s == nwarm) clock_gettime(CLOCK_MONOTONIC, &t0);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+            for (int bi = 0; bi < nbox; ++bi)
+                orc_step(lattice, cur_b ? bx[bi].b : bx[bi].a, cur_b ? bx[bi].a : bx[bi].b, bx[bi].n,
+                         omega, F, 1);
+            cur_b = !cur_b;
+            fill_boxes(L, bx, nbox, n, mgs, nb, cur_b);
+        }
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        secs = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+        for (int bi = 0; bi < nbox; ++bi) {
+            const box_t* B = &bx[bi];
+            const double* src = cur_b ? B->b : B->a;
+            for (int q = 0; q < L->Q; ++q)
+                for (int k = 0; k < B->n[2]; ++k)
+                    for (int j = 0; j < B->n[1]; ++j)
+                        for (int i = 0; i < B->n[0]; ++i)
+                            f[fidx(&whole, B->lo[0] + i, B->lo[1] + j, B->lo[2] + k, q)] =
+                                src[fidx(&B->fb, i, j, k, q)];
+        }
+    }
+    for (int bi = 0; bi < nbox; ++bi) {
+        free(bx[bi].a);
+        free(bx[bi].b);
+    }
+    free(bx);
+    return secs;
 }

@@ -74,6 +74,14 @@ double orc_time_run(int lattice, double* fa, double* fb, const int* n,
                     const int* rtype, const double* rrho, const double* ru,
                     int nsteps, int nthreads);
 
+/* the same loop on a fully periodic domain cut into boxes of at most mgs
+ * cells per side, each with its own ghost layer (the reference's
+ * max_grid_size layout), OpenMP over boxes; starts from and returns the
+ * valid cells of the single-box fab f.  Seconds of the nsteps steps timed
+ * after nwarm untimed ones, or < 0. */
+double orc_time_run_boxes(int lattice, double* f, const int* n, int mgs, double omega,
+                          const double* F, int nwarm, int nsteps, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

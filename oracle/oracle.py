@@ -38,6 +38,14 @@ def build() -> str:
 _lib = None
 
 
+def use_library(path: str) -> None:
+    """Load the oracle from another build of lbm_oracle.c (the CPU baseline
+    builds one with -march=native on the host it times)."""
+    global _LIB_PATH, _lib
+    _LIB_PATH = path
+    _lib = None
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -263,6 +271,23 @@ def run(lat, f, n, omega, F, periodic, resolved, nsteps, nthreads=0):
     if r < 0:
         raise ValueError("oracle run failed")
     return fa if r == 0 else fb
+
+
+def time_run_boxes(lat, f, n, mgs, omega, F, nwarm, nsteps, nthreads=0):
+    """The fully periodic advance() loop on boxes of at most mgs cells per
+    side (the reference's max_grid_size layout), OpenMP over boxes; f (a
+    ghosted fab) is advanced in place.  Returns the seconds of the nsteps
+    timed steps."""
+    Fa, Fp = _force(F)
+    L = lib()
+    L.orc_time_run_boxes.restype = ctypes.c_double
+    L.orc_time_run_boxes.argtypes = [ctypes.c_int, _dp, _ip, ctypes.c_int, ctypes.c_double, _dp,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    assert f.flags.c_contiguous and f.dtype == np.float64
+    t = L.orc_time_run_boxes(lat.id, _d(f), _i(n3(n)), mgs, omega, Fp, nwarm, nsteps, nthreads)
+    if t < 0:
+        raise ValueError("oracle boxed run failed")
+    return t
 
 
 def time_run(lat, fa, fb, n, omega, F, periodic, resolved, nsteps, nthreads=0):

@@ -256,25 +256,29 @@ def test_row_interleaved_layout_matches_oracle(dev, lat, bc):
     assert np.array_equal(to_host(v), vel_e)
 
 
-STEP2_SIZES = [[13, 7, 6], [64, 3, 2], [40, 33, 70], [1, 5, 3]]
+STEP2_SIZES = [[13, 7, 6], [64, 3, 2], [40, 33, 70], [1, 5, 3], [70, 20, 150]]
 
 
 @pytest.mark.parametrize("lat", ["D3Q15", "D3Q19", "D3Q27"])
 @pytest.mark.parametrize("si", range(len(STEP2_SIZES)))
 @pytest.mark.parametrize("forced", [False, True])
 @pytest.mark.parametrize("layout", ["component", "row"])
-def test_step2_equals_two_oracle_steps(dev, lat, si, forced, layout):
+@pytest.mark.parametrize("bc", ["periodic", "noslip", "lid", "xper_walls", "inflow_pressure",
+                                "inflow_outflow"])
+def test_step2_equals_two_oracle_steps(dev, lat, si, forced, layout, bc):
     """mfa_step2 (two steps per pass, intermediate state in LDS) gives the
     oracle's bits after two steps: boxes smaller than its tile, not a
-    multiple of it, and more z-planes than one block marches (70 > 64)."""
+    multiple of it, one cell wide, and taller than one block's z-range; walls
+    of every kind on any face (the second step's ghost values come from the
+    intermediate state)."""
     import torch
     L = O.Lattice(lat)
     n = STEP2_SIZES[si]
-    print(f"case step2 lattice={lat} n={n} forced={forced} layout={layout}", flush=True)
-    g, ref, per = bc_tables(L, "periodic")
+    print(f"case step2 lattice={lat} n={n} forced={forced} layout={layout} bc={bc}", flush=True)
+    g, ref, per = bc_tables(L, bc)
     F = [2e-5, -1e-5, 3e-6] if forced else None
     omega = 1.0 / 0.8
-    f0 = random_state(L, n, seed=zlib.crc32(f"step2/{lat}/{si}".encode()))
+    f0 = random_state(L, n, seed=zlib.crc32(f"step2/{lat}/{si}/{bc}".encode()))
     _, mid = _oracle_step(L, f0, n, per, ref, omega, F)
     _, expect = _oracle_step(L, mid, n, per, ref, omega, F)
     if layout == "row":
@@ -324,14 +328,15 @@ def test_step2_2d_equals_two_oracle_steps(dev, bc, si, forced):
         assert np.array_equal(got, O.valid(L, expect)), layout
 
 
-def test_step2_refuses_3d_walls_and_2d_16bit(dev):
+def test_step2_refuses_3d_walls_in_fp32_and_2d_16bit(dev):
     import torch
     L = O.Lattice("D3Q19")
     g, _, _ = bc_tables(L, "lid")
-    a = torch.zeros((19, 4, 4, 4), dtype=torch.float64, device=dev)
+    a = torch.zeros((19, 4, 4, 4), dtype=torch.float32, device=dev)
     b = torch.zeros_like(a)
     with pytest.raises(abi.MfaError) as e:
-        abi.step2(abi.field(a, [4, 4, 4], 0, 3), abi.field(b, [4, 4, 4], 0, 3), g, 1.0)
+        abi.step2(abi.field(a, [4, 4, 4], 0, 3, abi.DTYPE["f32"]),
+                  abi.field(b, [4, 4, 4], 0, 3, abi.DTYPE["f32"]), g, 1.0)
     assert e.value.code == -2
     L2 = O.Lattice("D2Q9")
     g2, _, _ = bc_tables(L2, "periodic")

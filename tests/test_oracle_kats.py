@@ -435,3 +435,26 @@ def test_masked_step_bounces_back_and_conserves_mass():
     fe = O.run_masked(L, fm, n, 1.0 / 0.7, [0, 0], res, mask, 40)
     r1 = O.moments(L, fe, n, streamed=False)[0]
     assert abs(r1[fluid].sum() - m0) / m0 < 1e-12
+
+
+@pytest.mark.parametrize("lat,n,mgs", [("D3Q19", [20, 18, 22], 8), ("D2Q9", [37, 21], 16),
+                                       ("D3Q27", [9, 10, 11], 4), ("D3Q15", [7, 5, 6], 64)])
+def test_boxed_run_equals_single_box(lat, n, mgs):
+    """The CPU baseline's box layout (boxes of at most max_grid_size cells,
+    each with its own ghost layer, filled from the neighbouring boxes) gives
+    the single box's bits after the same steps: ragged last boxes, a box
+    bigger than the domain, forced."""
+    L = O.Lattice(lat)
+    rng = np.random.default_rng(3)
+    f = O.new_fab(L, n)
+    v = O.valid(L, f)
+    v[...] = L.w[:, None, None, None] * (1 + 0.05 * rng.standard_normal(v.shape))
+    F = [1e-5, -2e-6, 3e-6] if L.D == 3 else [1e-5, -2e-6]
+    per = [1] * L.D + [0] * (3 - L.D)
+    ref = O.resolve_bc(L.D, [O.BC["Periodic"]] * (2 * L.D))
+    g = f.copy()
+    O.fill_periodic(L, g, n, per)
+    expect = O.run(L, g, n, 1.0 / 0.8, F, per, ref, 5)
+    b = f.copy()
+    assert O.time_run_boxes(L, b, n, mgs, 1.0 / 0.8, F, 2, 3) >= 0
+    assert np.array_equal(O.valid(L, b), O.valid(L, expect))
