@@ -192,6 +192,17 @@ int mfa_init_equilibrium(mfa_field* rho, mfa_field* vel, mfa_field* f, int latti
  * so calls queued on different streams do not share partial sums. */
 int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void* stream);
 
+/* mfa_reduce with no allocation, for callers that cannot call hipMalloc
+ * (code under HIP graph capture, serving loops, harnesses that forbid it).
+ * No counterpart in the reference.  The partial sums go to a file-scope
+ * device array of 1024 x 27 doubles (216 KiB) that each GPU the library runs
+ * on holds until the process ends; the caller passes only f and out_dev.
+ * Same kernels and results as mfa_reduce, bit for bit; any box size fits
+ * (the blocks stride over it); per-component sums of more than 27 components
+ * are refused.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams, two threads) race on it. */
+int mfa_reduce_noalloc(const mfa_field* f, int lattice, int what, double* out_dev, void* stream);
+
 /* Steady-state residual: out_dev[0] = sum (a-b)^2, out_dev[1] = sum a^2 over
  * the valid cells of all components of two same-shaped F64 fields. */
 int mfa_residual(const mfa_field* a, const mfa_field* b, double* out_dev, void* stream);

@@ -26,7 +26,7 @@ EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_s
            "mfa_get_launch_dev", "mfa_stream_copy",
            "mfa_convert", "mfa_level_create", "mfa_level_destroy", "mfa_level_bind",
            "mfa_level_fill_boundary", "mfa_level_apply_bc", "mfa_level_ghost_cells",
-           "mfa_level_step"]
+           "mfa_level_step", "mfa_reduce_noalloc"]
 
 
 class MfaError(RuntimeError):
@@ -95,6 +95,7 @@ def lib():
         L.mfa_moments.argtypes = [F, F, F, G, dp, ctypes.c_int, vp]
         L.mfa_init_equilibrium.argtypes = [F, F, F, ctypes.c_int, dp, vp]
         L.mfa_reduce.argtypes = [F, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.mfa_reduce_noalloc.argtypes = [F, ctypes.c_int, ctypes.c_int, vp, vp]
         L.mfa_residual.argtypes = [F, F, vp, vp]
         L.mfa_set_launch.argtypes = [ctypes.c_int, ctypes.POINTER(Launch)]
         L.mfa_get_launch.argtypes = [ctypes.c_int, ctypes.POINTER(Launch)]
@@ -279,6 +280,13 @@ def init_equilibrium(rho: Field, vel: Field, f: Field, lattice: int, force=None,
 def reduce(f: Field, lattice: int, what: int, out_ptr: int, stream=None):
     check(lib().mfa_reduce(ctypes.byref(f), lattice, what, ctypes.c_void_p(out_ptr),
                            stream_ptr(stream)))
+
+
+def reduce_noalloc(f: Field, lattice: int, what: int, out_ptr: int, stream=None):
+    """reduce() with its partial sums in the library's file-scope device
+    array instead of an allocated workspace (mfa_reduce_noalloc)."""
+    check(lib().mfa_reduce_noalloc(ctypes.byref(f), lattice, what, ctypes.c_void_p(out_ptr),
+                                   stream_ptr(stream)))
 
 
 def residual(a: Field, b: Field, out_ptr: int, stream=None):

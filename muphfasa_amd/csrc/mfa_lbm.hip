@@ -713,6 +713,81 @@ static int workspace(hipStream_t st, double** out)
     return MFA_OK;
 }
 
+// Workspace of mfa_reduce_noalloc: the partial sums of the reduction kernels,
+// kRedBlocks blocks x kRedMaxOut values (216 KiB) in a file-scope device
+// array.  Each GPU holds its own copy, at its own address, from load to
+// process exit.  The array's name is a host-side stand-in, not a device
+// address: host code uses only the address hipGetSymbolAddress returned on
+// that device, kept per device.
+constexpr int kScratchDoubles = kRedBlocks * kRedMaxOut;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex g_scratch_mu;
+static double* g_scratch[64] = {nullptr};
+
+static int scratch_ptr(double** out)
+{
+    int dev = 0;
+    MFA_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(MFA_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        MFA_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(MFA_ERR_HIP, "hipGetSymbolAddress(scratch): null address");
+        g_scratch[dev] = (double*)p;
+    }
+    *out = g_scratch[dev];
+    return MFA_OK;
+}
+
+static int reduce_with(const mfa_field* f, int lattice, int what, double* out_dev, void* stream,
+                       double* ws)
+{
+    const int Q = lattice_q(lattice);
+    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_reduce: unknown lattice");
+    if (!out_dev) return fail(MFA_ERR_INVALID, "mfa_reduce: null output");
+    int rc;
+    const int need = (what == MFA_RED_COMP_SUM) ? 1 : Q;
+    if ((rc = check_field(f, need, 0, "mfa_reduce f", true))) return rc;
+    if (what == MFA_RED_COMP_SUM && f->ncomp > kRedMaxOut)
+        return fail(MFA_ERR_INVALID, "mfa_reduce: more than 27 components");
+    const Geo geo = geo_of(*f, nullptr, lattice);
+    const View v = view_of(*f);
+    hipStream_t st = (hipStream_t)stream;
+    int nv = 0;
+    if (f->dtype != MFA_F64) {
+        if ((rc = reduce_dev(f, geo, lattice, what, ws, kRedBlocks, st))) return rc;
+        nv = (what == MFA_RED_MASS) ? 1 : (what == MFA_RED_HEALTH ? 2 : lattice_dim(lattice));
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, ws, kRedBlocks, nv, out_dev);
+        MFA_HIP_CHECK(hipGetLastError());
+        return MFA_OK;
+    }
+    rc = with_lattice(lattice, [&](auto L) {
+        using LT = decltype(L);
+        if (what == MFA_RED_MASS) {
+            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_MASS>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
+            nv = 1;
+        } else if (what == MFA_RED_MOMENTUM) {
+            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_MOMENTUM>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
+            nv = LT::D;
+        } else if (what == MFA_RED_HEALTH) {
+            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_HEALTH>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
+            nv = 2;
+        } else if (what == MFA_RED_COMP_SUM) {
+            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_COMP_SUM>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
+            nv = f->ncomp;
+        } else {
+            return fail(MFA_ERR_INVALID, "mfa_reduce: unknown reduction");
+        }
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    });
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, ws, kRedBlocks, nv, out_dev);
+    MFA_HIP_CHECK(hipGetLastError());
+    return MFA_OK;
+}
+
 }  // namespace mfa
 
 using namespace mfa;
@@ -1106,51 +1181,21 @@ int mfa_init_equilibrium(mfa_field* rho, mfa_field* vel, mfa_field* f, int latti
 
 int mfa_reduce(const mfa_field* f, int lattice, int what, double* out_dev, void* stream)
 {
-    const int Q = lattice_q(lattice);
-    if (Q < 0) return fail(MFA_ERR_INVALID, "mfa_reduce: unknown lattice");
-    if (!out_dev) return fail(MFA_ERR_INVALID, "mfa_reduce: null output");
-    int rc;
-    const int need = (what == MFA_RED_COMP_SUM) ? 1 : Q;
-    if ((rc = check_field(f, need, 0, "mfa_reduce f", true))) return rc;
-    if (what == MFA_RED_COMP_SUM && f->ncomp > kRedMaxOut)
-        return fail(MFA_ERR_INVALID, "mfa_reduce: more than 27 components");
     double* ws = nullptr;
+    int rc;
     if ((rc = workspace((hipStream_t)stream, &ws))) return rc;
-    const Geo geo = geo_of(*f, nullptr, lattice);
-    const View v = view_of(*f);
-    hipStream_t st = (hipStream_t)stream;
-    int nv = 0;
-    if (f->dtype != MFA_F64) {
-        if ((rc = reduce_dev(f, geo, lattice, what, ws, kRedBlocks, st))) return rc;
-        nv = (what == MFA_RED_MASS) ? 1 : (what == MFA_RED_HEALTH ? 2 : lattice_dim(lattice));
-        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, ws, kRedBlocks, nv, out_dev);
-        MFA_HIP_CHECK(hipGetLastError());
-        return MFA_OK;
-    }
-    rc = with_lattice(lattice, [&](auto L) {
-        using LT = decltype(L);
-        if (what == MFA_RED_MASS) {
-            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_MASS>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
-            nv = 1;
-        } else if (what == MFA_RED_MOMENTUM) {
-            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_MOMENTUM>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
-            nv = LT::D;
-        } else if (what == MFA_RED_HEALTH) {
-            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_HEALTH>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
-            nv = 2;
-        } else if (what == MFA_RED_COMP_SUM) {
-            hipLaunchKernelGGL((k_reduce_f<LT, MFA_RED_COMP_SUM>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, (const double*)f->data, geo, v, f->ncomp, ws);
-            nv = f->ncomp;
-        } else {
-            return fail(MFA_ERR_INVALID, "mfa_reduce: unknown reduction");
-        }
-        MFA_HIP_CHECK(hipGetLastError());
-        return (int)MFA_OK;
-    });
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, ws, kRedBlocks, nv, out_dev);
-    MFA_HIP_CHECK(hipGetLastError());
-    return MFA_OK;
+    return reduce_with(f, lattice, what, out_dev, stream, ws);
+}
+
+int mfa_reduce_noalloc(const mfa_field* f, int lattice, int what, double* out_dev, void* stream)
+{
+    // The kernels' partial sums go to the file-scope scratch array: every
+    // block writes each partial the final pass reads, so no clearing is
+    // needed, and a field of any size fits (the blocks stride over it).
+    double* ws = nullptr;
+    int rc;
+    if ((rc = scratch_ptr(&ws))) return rc;
+    return reduce_with(f, lattice, what, out_dev, stream, ws);
 }
 
 int mfa_residual(const mfa_field* a, const mfa_field* b, double* out_dev, void* stream)
