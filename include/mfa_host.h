@@ -49,6 +49,14 @@ int mfa_sim_health(mfa_sim* s, double out[2]);
 int mfa_inputs_units(const char* inputs_text, int noverrides, const char* const* overrides,
                      double out[12]);
 
+/* Host-only (no GPU): the populations of a muphfasa_checkpoint_v1 directory
+ * (src/IO/Checkpoint.cpp:105-141, readCheckpoint), in whatever box layout and
+ * ghost width its VisMF "f" was written with, as ncomp planes of the n[0..dim)
+ * domain's valid cells [comp][z][y][x]; the step and time of its Header.
+ * The restart of mfa_sim_initialize reads checkpoints the same way. */
+int mfa_checkpoint_read(const char* dir, int dim, const int n[3], int ncomp, double* f, int* step,
+                        double* time);
+
 /* src/main.cpp: argv[1] = inputs file, then key=value overrides. */
 int mfa_run_main(int argc, const char* const* argv);
 

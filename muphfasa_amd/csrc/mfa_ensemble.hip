@@ -762,28 +762,35 @@ int mfa_ensemble_run(mfa_ensemble* e, void* fa, void* fb, const unsigned char* m
     // into a hipGraph on a private stream and replayed.
     if (!e || nsteps < 0) return ens_fail(MFA_ERR_INVALID, "bad arguments");
     if (!use_graph || nsteps == 0) return run_launches(e, fa, fb, masks, nsteps, stream);
-    hipStream_t cap;
-    ENS_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-    hipEvent_t ev;
-    ENS_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    ENS_HIP(hipEventRecord(ev, (hipStream_t)stream));
-    ENS_HIP(hipStreamWaitEvent(cap, ev, 0));
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    ENS_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    const int rc = run_launches(e, fa, fb, masks, nsteps, cap);
-    const hipError_t ce = hipStreamEndCapture(cap, &graph);
+    // the capture stream, event, graph and executable are destroyed on every
+    // way out, errors included
+    struct Handles {
+        hipStream_t cap = nullptr;
+        hipEvent_t ev = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        ~Handles()
+        {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (ev) (void)hipEventDestroy(ev);
+            if (cap) (void)hipStreamDestroy(cap);
+        }
+    } h;
+    ENS_HIP(hipStreamCreateWithFlags(&h.cap, hipStreamNonBlocking));
+    ENS_HIP(hipEventCreateWithFlags(&h.ev, hipEventDisableTiming));
+    ENS_HIP(hipEventRecord(h.ev, (hipStream_t)stream));
+    ENS_HIP(hipStreamWaitEvent(h.cap, h.ev, 0));
+    ENS_HIP(hipStreamBeginCapture(h.cap, hipStreamCaptureModeThreadLocal));
+    const int rc = run_launches(e, fa, fb, masks, nsteps, h.cap);
+    const hipError_t ce = hipStreamEndCapture(h.cap, &h.graph);
     if (rc != MFA_OK) return rc;
     if (ce != hipSuccess) return ens_fail(MFA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-    ENS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    ENS_HIP(hipGraphLaunch(exec, cap));
-    ENS_HIP(hipEventRecord(ev, cap));
-    ENS_HIP(hipStreamWaitEvent((hipStream_t)stream, ev, 0));
-    ENS_HIP(hipStreamSynchronize(cap));
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    (void)hipEventDestroy(ev);
-    (void)hipStreamDestroy(cap);
+    ENS_HIP(hipGraphInstantiate(&h.exec, h.graph, nullptr, nullptr, 0));
+    ENS_HIP(hipGraphLaunch(h.exec, h.cap));
+    ENS_HIP(hipEventRecord(h.ev, h.cap));
+    ENS_HIP(hipStreamWaitEvent((hipStream_t)stream, h.ev, 0));
+    ENS_HIP(hipStreamSynchronize(h.cap));
     return MFA_OK;
 }
 

@@ -474,25 +474,30 @@ static int check_field(const mfa_field* f, int ncomp_min, int need_grow, const c
 }
 
 // Byte range [lo, hi] a field's kernels may touch: every component over the
-// valid box grown by its ghost layers (a z layer only when the box has more
-// than one plane, so 2-D views are not widened past their allocation).
-static void byte_range(const mfa_field& f, long long& lo, long long& hi)
+// valid box grown by its ghost layers, in the lattice's D dimensions (a 3-D
+// lattice's halo-mode kernels read the z ghost planes even of a box one
+// plane thick; a 2-D view has no z ghost layer).
+static void byte_range(const mfa_field& f, int D, long long& lo, long long& hi)
 {
     const long long es = f.dtype == MFA_F64 ? 8 : (f.dtype == MFA_F32 ? 4 : 2);
-    const long long g = f.ngrow, gz = f.n[2] > 1 ? g : 0;
+    const long long g = f.ngrow, gz = D == 3 ? g : 0;
     const long long base = (long long)(intptr_t)f.data;
     lo = base - es * (g + g * f.sy + gz * f.sz);
     hi = base + es * ((f.n[0] - 1 + g) + (f.n[1] - 1 + g) * f.sy + (f.n[2] - 1 + gz) * f.sz +
                       (long long)(f.ncomp - 1) * f.sq + 1) - 1;
 }
 
+// The same for the multi-box level (mfa_level.hip).
+void field_range(const mfa_field& f, int D, long long& lo, long long& hi) { byte_range(f, D, lo, hi); }
+
 // A kernel that reads one field while it writes another races if the two
 // share memory, so their byte ranges must be disjoint.
-static int check_disjoint(const mfa_field* a, const mfa_field* b, const char* what)
+static int check_disjoint(const mfa_field* a, const mfa_field* b, int lattice, const char* what)
 {
     long long alo, ahi, blo, bhi;
-    byte_range(*a, alo, ahi);
-    byte_range(*b, blo, bhi);
+    const int D = lattice == MFA_D2Q9 ? 2 : 3;
+    byte_range(*a, D, alo, ahi);
+    byte_range(*b, D, blo, bhi);
     if (alo <= bhi && blo <= ahi)
         return fail(MFA_ERR_INVALID, std::string(what) + ": input and output overlap in memory");
     return MFA_OK;
@@ -895,7 +900,7 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
         return rc;
     if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
         return fail(MFA_ERR_INVALID, "mfa_step2: fin/fout boxes differ");
-    if ((rc = check_disjoint(fin, fout, "mfa_step2"))) return rc;
+    if ((rc = check_disjoint(fin, fout, lat, "mfa_step2"))) return rc;
     if (fin->dtype != fout->dtype) return fail(MFA_ERR_INVALID, "mfa_step2: fin/fout storage types differ");
     if ((rc = check_2d(fin, lat))) return rc;
     const Geo geo = geo_of(*fin, g, lat);
@@ -981,7 +986,7 @@ int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, double
         return rc;
     if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
         return fail(MFA_ERR_INVALID, "mfa_step: fin/fout boxes differ");
-    if ((rc = check_disjoint(fin, fout, "mfa_step"))) return rc;
+    if ((rc = check_disjoint(fin, fout, lat, "mfa_step"))) return rc;
     if (fin->dtype != fout->dtype) return fail(MFA_ERR_INVALID, "mfa_step: fin/fout storage types differ");
     if ((rc = check_2d(fin, lat))) return rc;
     const Geo geo = geo_of(*fin, g, lat);
@@ -1015,7 +1020,7 @@ int mfa_convert(const mfa_field* src, mfa_field* dst, int lattice, void* stream)
     if (std::memcmp(src->n, dst->n, sizeof(src->n))) return fail(MFA_ERR_INVALID, "mfa_convert: boxes differ");
     if ((src->dtype == MFA_F64) == (dst->dtype == MFA_F64))
         return fail(MFA_ERR_INVALID, "mfa_convert: exactly one side must be FP64");
-    if ((rc = check_disjoint(src, dst, "mfa_convert"))) return rc;
+    if ((rc = check_disjoint(src, dst, lattice, "mfa_convert"))) return rc;
     if ((rc = check_2d(src, lattice))) return rc;
     return convert_dev(src, dst, geo_of(*src, nullptr, lattice), lattice, (hipStream_t)stream);
 }
@@ -1049,7 +1054,7 @@ int mfa_pull_collide(const mfa_field* fin, mfa_field* fout, int lattice, double 
         return rc;
     if (std::memcmp(fin->n, fout->n, sizeof(fin->n)))
         return fail(MFA_ERR_INVALID, "mfa_pull_collide: fin/fout boxes differ");
-    if ((rc = check_disjoint(fin, fout, "mfa_pull_collide"))) return rc;
+    if ((rc = check_disjoint(fin, fout, lattice, "mfa_pull_collide"))) return rc;
     if ((rc = check_2d(fin, lattice))) return rc;
     const Geo geo = geo_of(*fin, nullptr, lattice);
     return with_lattice(lattice, [&](auto L) {

@@ -22,6 +22,7 @@
 namespace mfa {
 
 int field_check(const mfa_field* f, int ncomp_min, int need_grow, const char* what);
+void field_range(const mfa_field& f, int D, long long& lo, long long& hi);
 Geo geo_for(const int n[3], const mfa_geom_bc* gb, int lat);
 
 struct BoxView {                  // one box's populations, relative to valid cell (0,0,0)
@@ -188,7 +189,22 @@ struct mfa_level {
     BoxView* d_views = nullptr;             // kLevelSlots x nboxes
     bool bound[kLevelSlots] = {false, false, false, false};
     std::vector<BoxView> views[kLevelSlots];
+    // byte ranges of the bound boxes, sorted by start
+    std::vector<std::pair<long long, long long>> ranges[kLevelSlots];
 };
+
+// Do two sets of byte ranges, each sorted by start and disjoint within
+// itself, share a byte?
+static bool ranges_meet(const std::vector<std::pair<long long, long long>>& a,
+                        const std::vector<std::pair<long long, long long>>& b)
+{
+    std::vector<std::pair<long long, long long>> all(a);
+    all.insert(all.end(), b.begin(), b.end());
+    std::sort(all.begin(), all.end());
+    for (size_t i = 1; i < all.size(); ++i)
+        if (all[i].first <= all[i - 1].second) return true;
+    return false;
+}
 
 static int lattice_q_lv(int lat)
 {
@@ -364,10 +380,19 @@ int mfa_level_bind(mfa_level* lv, int slot, const mfa_field* boxes)
                 return fail(MFA_ERR_INVALID, "mfa_level_bind: box " + std::to_string(b) + " has other extents");
         v[b] = BoxView{(double*)f->data, f->sy, f->sz, f->sq};
     }
+    // the boxes of a slot are written together by one launch: they must not
+    // share memory
+    std::vector<std::pair<long long, long long>> r(lv->nboxes);
+    for (int b = 0; b < lv->nboxes; ++b) field_range(boxes[b], lv->D, r[b].first, r[b].second);
+    std::sort(r.begin(), r.end());
+    for (int b = 1; b < lv->nboxes; ++b)
+        if (r[b].first <= r[b - 1].second)
+            return fail(MFA_ERR_INVALID, "mfa_level_bind: two boxes of the slot overlap in memory");
     const hipError_t e = hipMemcpy(lv->d_views + (size_t)slot * lv->nboxes, v.data(),
                                    v.size() * sizeof(BoxView), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(MFA_ERR_HIP, std::string("mfa_level_bind: ") + hipGetErrorString(e));
     lv->views[slot] = v;
+    lv->ranges[slot] = r;
     lv->bound[slot] = true;
     return MFA_OK;
 }
@@ -398,9 +423,8 @@ int mfa_level_step(const mfa_level* lv, int slot_in, int slot_out, double omega,
         if (s < 0 || s >= kLevelSlots || !lv->bound[s])
             return fail(MFA_ERR_INVALID, "mfa_level_step: slot not bound");
     if (slot_in == slot_out) return fail(MFA_ERR_INVALID, "mfa_level_step: slot_in == slot_out");
-    for (int b = 0; b < lv->nboxes; ++b)
-        if (lv->views[slot_in][b].data == lv->views[slot_out][b].data)
-            return fail(MFA_ERR_INVALID, "mfa_level_step: a box aliases itself");
+    if (ranges_meet(lv->ranges[slot_in], lv->ranges[slot_out]))
+        return fail(MFA_ERR_INVALID, "mfa_level_step: an input box and an output box overlap in memory");
     return with_lattice_lv(lv->lattice, [&](auto Lc) {
         using L = decltype(Lc);
         Geo g;

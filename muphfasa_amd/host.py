@@ -16,7 +16,7 @@ EXE_PATH = os.path.join(HERE, "lib", "muphfasa")
 EXPORTS = ["mfa_sim_error", "mfa_sim_create", "mfa_sim_destroy", "mfa_sim_initialize",
            "mfa_sim_advance", "mfa_sim_update_macroscopic", "mfa_sim_write_output", "mfa_sim_sync",
            "mfa_sim_info", "mfa_sim_copy_f", "mfa_sim_copy_macro", "mfa_sim_health", "mfa_inputs_units",
-           "mfa_run_main"]
+           "mfa_run_main", "mfa_checkpoint_read"]
 _lib = None
 
 
@@ -43,6 +43,7 @@ def lib():
         L.mfa_sim_health.argtypes = [ctypes.c_void_p, dp]
         L.mfa_inputs_units.argtypes = [cp, ctypes.c_int, ctypes.POINTER(cp), dp]
         L.mfa_run_main.argtypes = [ctypes.c_int, ctypes.POINTER(cp)]
+        L.mfa_checkpoint_read.argtypes = [cp, ctypes.c_int, ip, ctypes.c_int, dp, ip, dp]
         _lib = L
     return _lib
 

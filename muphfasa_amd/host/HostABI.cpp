@@ -1,5 +1,6 @@
 // HostABI.cpp -- C ABI of the host (include/mfa_host.h) and the run loop of
 // the solver executable (src/main.cpp:14-38 of the reference).
+#include <algorithm>
 #include <cstdio>
 #include <memory>
 #include <string>
@@ -127,6 +128,21 @@ int mfa_inputs_units(const char* inputs_text, int noverrides, const char* const*
                               u.physViscosity(), u.machNumber(), u.toLatticeAcceleration(a[0]),
                               u.toLatticeAcceleration(a[1]), u.toLatticeAcceleration(a[2])};
         for (int i = 0; i < 12; ++i) out[i] = v[i];
+    });
+}
+
+int mfa_checkpoint_read(const char* dir, int dim, const int n[3], int ncomp, double* f, int* step,
+                        double* time)
+{
+    return guarded([&] {
+        if (!dir || !n || !f || !step || !time) throw std::runtime_error("mfa_checkpoint_read: null argument");
+        if (dim != 2 && dim != 3) throw std::runtime_error("mfa_checkpoint_read: dim must be 2 or 3");
+        BoxGeom g;
+        g.dim = dim;
+        for (int d = 0; d < 3; ++d) g.n[d] = d < dim ? n[d] : 1;
+        std::vector<double> v;
+        readCheckpoint(dir, v, g, ncomp, *step, *time);
+        std::copy(v.begin(), v.end(), f);
     });
 }
 

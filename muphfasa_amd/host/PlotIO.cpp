@@ -193,9 +193,176 @@ std::vector<std::string> checkpointMismatches(const std::string& dir, const Chec
     return problems;
 }
 
+namespace {
+
+// Integers of an AMReX Box or IntVect text, "((0,0,0) (7,5,3) (0,0,0))".
+std::vector<long long> text_ints(const std::string& text)
+{
+    std::string t = text;
+    for (char& ch : t)
+        if (ch == '(' || ch == ')' || ch == ',') ch = ' ';
+    std::istringstream s(t);
+    std::vector<long long> v;
+    long long x;
+    while (s >> x) v.push_back(x);
+    return v;
+}
+
+struct VBox {
+    long long lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+};
+
+// ((lo) (hi) (type)) in dim dimensions
+bool parse_box(const std::string& text, int dim, VBox& b)
+{
+    const auto v = text_ints(text);
+    if ((int)v.size() != 3 * dim) return false;
+    for (int d = 0; d < dim; ++d) {
+        b.lo[d] = v[d];
+        b.hi[d] = v[dim + d];
+        if (v[2 * dim + d] != 0) return false;   // cell-centred only
+    }
+    return true;
+}
+
+long long volume(const VBox& b, int dim)
+{
+    long long v = 1;
+    for (int d = 0; d < dim; ++d) v *= b.hi[d] - b.lo[d] + 1;
+    return v;
+}
+
+}  // namespace
+
+std::string readVisMF(const std::string& path, const BoxGeom& g, int ncomp, std::vector<double>* f)
+{
+    // VisMF::Write's header (version 1): version, how, ncomp, ghost width
+    // (an int, or an IntVect when it differs by direction), the BoxArray of
+    // valid boxes, one "FabOnDisk: file offset" per box, then min/max.
+    const int dim = g.dim;
+    std::ifstream h(path + "_H");
+    if (!h) return "no header " + path + "_H";
+    int vers = 0, how = 0, nc = 0;
+    if (!(h >> vers >> how >> nc)) return "malformed header " + path + "_H";
+    if (vers != 1) return "VisMF header version " + std::to_string(vers) + " in " + path + "_H (expected 1)";
+    if (nc != ncomp)
+        return path + " holds " + std::to_string(nc) + " components but this run needs " + std::to_string(ncomp);
+    std::string tok;
+    h >> std::ws;
+    std::getline(h, tok);
+    const auto gv = text_ints(tok);
+    long long ng[3] = {0, 0, 0};
+    if (gv.size() == 1) {
+        for (int d = 0; d < dim; ++d) ng[d] = gv[0];
+    } else if ((int)gv.size() == dim) {
+        for (int d = 0; d < dim; ++d) ng[d] = gv[d];
+    } else {
+        return "malformed ghost width in " + path + "_H";
+    }
+    for (int d = 0; d < dim; ++d)
+        if (ng[d] < 0) return "negative ghost width in " + path + "_H";
+    // BoxArray: "(N 0" then N boxes then ")"
+    std::string open;
+    long long nbox = -1, zero = -1;
+    if (!(h >> open) || open.rfind("(", 0) != 0) return "malformed box array in " + path + "_H";
+    if (open.size() > 1) nbox = std::stoll(open.substr(1));
+    else h >> nbox;
+    h >> zero;
+    if (!h || nbox < 1 || nbox > (1LL << 24)) return "malformed box array in " + path + "_H";
+    std::vector<VBox> boxes((size_t)nbox);
+    for (auto& b : boxes) {
+        std::string line;
+        h >> std::ws;
+        std::getline(h, line);
+        if (!h || !parse_box(line, dim, b)) return "malformed box in " + path + "_H";
+    }
+    h >> tok;
+    if (tok != ")") return "malformed box array in " + path + "_H";
+    long long nfod = -1;
+    if (!(h >> nfod) || nfod != nbox) return "the number of FABs in " + path + "_H differs from the boxes";
+    std::vector<std::pair<std::string, long long>> fods((size_t)nbox);
+    for (auto& fo : fods) {
+        std::string label;
+        if (!(h >> label >> fo.first >> fo.second) || label != "FabOnDisk:" || fo.second < 0)
+            return "malformed FabOnDisk entry in " + path + "_H";
+    }
+    // The boxes must lie in the domain, not overlap, and together hold all
+    // of its cells (src/IO/Checkpoint.cpp:131-135 checks the last two).
+    long long cells = 0, domain = 1;
+    for (int d = 0; d < dim; ++d) domain *= g.n[d];
+    for (size_t i = 0; i < boxes.size(); ++i) {
+        const VBox& b = boxes[i];
+        for (int d = 0; d < dim; ++d)
+            if (b.lo[d] < 0 || b.hi[d] >= g.n[d] || b.hi[d] < b.lo[d])
+                return "a box of " + path + " lies outside the domain";
+        cells += volume(b, dim);
+    }
+    for (size_t i = 0; i < boxes.size(); ++i)
+        for (size_t j = i + 1; j < boxes.size(); ++j) {
+            bool meet = true;
+            for (int d = 0; d < dim; ++d)
+                meet = meet && boxes[i].lo[d] <= boxes[j].hi[d] && boxes[j].lo[d] <= boxes[i].hi[d];
+            if (meet) return "two boxes of " + path + " overlap";
+        }
+    if (cells != domain) return "data in " + path + " does not cover the domain (incomplete checkpoint?)";
+    const std::string dir = path.substr(0, path.find_last_of('/') + 1);
+    if (f) f->assign((size_t)domain * ncomp, 0.0);
+    for (size_t i = 0; i < boxes.size(); ++i) {
+        const VBox& b = boxes[i];
+        const std::string fname = dir + fods[i].first;
+        std::ifstream d(fname, std::ios::binary);
+        if (!d) return "data file " + fname + " is missing";
+        d.seekg(fods[i].second);
+        std::string fh;
+        if (!d || !std::getline(d, fh) || fh.rfind("FAB ", 0) != 0)
+            return "data in " + path + " is not a FAB file (nothing at offset " +
+                   std::to_string(fods[i].second) + " of " + fname + ")";
+        // FAB <RealDescriptor><box> <ncomp>: native little-endian IEEE doubles only
+        const size_t bs = fh.find(")))");
+        if (bs == std::string::npos || fh.compare(4, bs + 3 - 4, kFabDouble) != 0)
+            return "FAB in " + fname + " does not hold little-endian IEEE doubles";
+        const std::string rest = fh.substr(bs + 3);
+        const size_t sp = rest.find_last_of(' ');
+        VBox fb;
+        if (sp == std::string::npos || !parse_box(rest.substr(0, sp), dim, fb))
+            return "malformed FAB header in " + fname;
+        long long fnc = -1;
+        try {
+            fnc = std::stoll(rest.substr(sp + 1));
+        } catch (...) {
+        }
+        if (fnc != ncomp) return "FAB in " + fname + " has another component count";
+        for (int dd = 0; dd < dim; ++dd)
+            if (fb.lo[dd] != b.lo[dd] - ng[dd] || fb.hi[dd] != b.hi[dd] + ng[dd])
+                return "FAB in " + fname + " is not its box grown by the ghost width";
+        long long e[3] = {1, 1, 1};
+        for (int dd = 0; dd < dim; ++dd) e[dd] = fb.hi[dd] - fb.lo[dd] + 1;
+        const long long fpts = e[0] * e[1] * e[2];
+        std::vector<double> buf((size_t)(fpts * ncomp));
+        d.read((char*)buf.data(), (std::streamsize)(sizeof(double) * buf.size()));
+        if (!d)
+            return "data in " + path + " does not cover the domain (incomplete checkpoint?): the FAB at offset " +
+                   std::to_string(fods[i].second) + " of " + fname + " is short";
+        if (!f) continue;
+        const long long n0 = g.n[0], n1 = g.n[1];
+        const long long bz0 = dim == 3 ? b.lo[2] : 0, bz1 = dim == 3 ? b.hi[2] : 0;
+        for (int c = 0; c < ncomp; ++c)
+            for (long long z = bz0; z <= bz1; ++z)
+                for (long long y = b.lo[1]; y <= b.hi[1]; ++y)
+                    for (long long x = b.lo[0]; x <= b.hi[0]; ++x) {
+                        const long long lx = x - fb.lo[0], ly = y - fb.lo[1], lz = dim == 3 ? z - fb.lo[2] : 0;
+                        (*f)[(size_t)(((c * (dim == 3 ? g.n[2] : 1) + z) * n1 + y) * n0 + x)] =
+                            buf[(size_t)(((c * e[2] + lz) * e[1] + ly) * e[0] + lx)];
+                    }
+    }
+    return {};
+}
+
 void readCheckpoint(const std::string& dir, std::vector<double>& f, const BoxGeom& g, int ncomp,
                     int& step, double& time)
 {
+    // src/IO/Checkpoint.cpp:105-141: the Header, then the populations of
+    // VisMF "f" in the checkpoint's own box layout, valid cells only
     CheckpointHeader h;
     const std::string err = readCheckpointHeader(dir, h, g.dim);
     if (!err.empty()) throw std::runtime_error("readCheckpoint: " + err);
@@ -205,19 +372,8 @@ void readCheckpoint(const std::string& dir, std::vector<double>& f, const BoxGeo
         for (size_t i = 1; i < problems.size(); ++i) msg += "; " + problems[i];
         throw std::runtime_error(msg);
     }
-    const std::string dname = dir + "/f_D_00000";
-    std::ifstream d(dname, std::ios::binary);
-    if (!d) throw std::runtime_error("readCheckpoint: data in " + dir + "/f is missing");
-    std::string fabhdr;
-    std::getline(d, fabhdr);
-    if (fabhdr.rfind("FAB ", 0) != 0)
-        throw std::runtime_error("readCheckpoint: data in " + dir + "/f is not a FAB file");
-    const long long npts = (long long)g.n[0] * g.n[1] * (g.dim == 3 ? g.n[2] : 1);
-    f.assign((size_t)npts * ncomp, 0.0);
-    d.read((char*)f.data(), (std::streamsize)(sizeof(double) * f.size()));
-    if (!d)
-        throw std::runtime_error("readCheckpoint: data in " + dir +
-                                 "/f does not cover the domain (incomplete checkpoint?)");
+    const std::string verr = readVisMF(dir + "/f", g, ncomp, &f);
+    if (!verr.empty()) throw std::runtime_error("readCheckpoint: " + verr);
     step = h.step;
     time = h.time;
 }

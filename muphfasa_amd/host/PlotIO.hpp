@@ -3,7 +3,8 @@
 // (src/IO/Plotfile.cpp:16-75: HyperCLaw-V1.1 Header, Level_0/Cell_H,
 // Level_0/Cell_D_00000 with native little-endian FP64 FABs) and the
 // muphfasa_checkpoint_v1 directory (src/IO/Checkpoint.cpp:26-154: text
-// Header + VisMF "f").  One box per level (this engine runs one box per GPU).
+// Header + VisMF "f").  Written as one box per level (this engine runs one
+// box per GPU); read in any box layout and ghost width.
 #pragma once
 
 #include <array>
@@ -38,6 +39,11 @@ void writeCheckpoint(const std::string& dir, const double* f, int ncomp, const B
 std::string readCheckpointHeader(const std::string& dir, CheckpointHeader& h, int dim);
 std::vector<std::string> checkpointMismatches(const std::string& dir, const CheckpointHeader& h,
                                               const BoxGeom& g, int ncomp);
+// The valid cells of a VisMF MultiFab (<path>_H and its FAB files) in any box
+// layout and ghost width VisMF::Write produces, copied into f as ncomp planes
+// of the domain; f == nullptr only checks.  Returns "" or the reason the data
+// cannot be used.
+std::string readVisMF(const std::string& path, const BoxGeom& g, int ncomp, std::vector<double>* f);
 // Reads the populations (valid cells) into f (ncomp planes); throws on error.
 void readCheckpoint(const std::string& dir, std::vector<double>& f, const BoxGeom& g, int ncomp,
                     int& step, double& time);

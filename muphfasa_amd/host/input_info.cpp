@@ -188,19 +188,10 @@ int report(const std::string& file, const ParmParse& pp)
         if (err.empty() && Q) problems = checkpointMismatches(restart, h, g, Q);
         if (!err.empty()) problems.insert(problems.begin(), err);
         if (problems.empty()) {
-            // one FAB: a text header line, then Q planes of FP64 values
-            std::ifstream d(restart + "/f_D_00000", std::ios::binary);
-            std::string fabhdr;
-            if (!d) {
-                problems.push_back("data in " + restart + "/f is missing");
-            } else if (!std::getline(d, fabhdr) || fabhdr.rfind("FAB ", 0) != 0) {
-                problems.push_back("data in " + restart + "/f is not a FAB file");
-            } else {
-                const std::streamoff start = d.tellg();
-                d.seekg(0, std::ios::end);
-                if ((double)(d.tellg() - start) < cells * Q * 8.0)
-                    problems.push_back("data in " + restart + "/f does not cover the domain (incomplete checkpoint?)");
-            }
+            // the populations as the restart reads them: every FAB of VisMF
+            // "f" in the checkpoint's own box layout and ghost width
+            const std::string verr = readVisMF(restart + "/f", g, Q, nullptr);
+            if (!verr.empty()) problems.push_back(verr);
         }
         if (problems.empty()) {
             row("initial state", fmt("restart from %s (written at step %d, t_lat = %.17g); the run resumes at "

@@ -241,3 +241,31 @@ def test_overlapping_input_and_output_are_rejected_before_launch():
     with pytest.raises(abi.MfaError) as e:
         abi.check(L.mfa_convert(ctypes.byref(a), ctypes.byref(b), abi.LATTICE["D3Q19"], None))
     assert "overlap" in str(e.value)
+
+
+def test_pull_collide_refuses_an_output_in_the_z_ghost_of_a_one_plane_box():
+    """A 3-D lattice's halo-mode step reads the z ghost planes even of a box
+    one plane thick, so an output placed right after the input's last
+    component but inside its z = +1 ghost plane overlaps the input.  The call
+    is refused on the host, before any GPU work (the pointers are never
+    dereferenced)."""
+    Q, n = 19, [4, 4, 1]
+    e = [n[0] + 2, n[1] + 2, n[2] + 2]
+    fin = abi.Field()
+    fin.n[0], fin.n[1], fin.n[2] = n
+    fin.ngrow = 1
+    fin.sy, fin.sz, fin.sq = e[0], e[0] * e[1], e[0] * e[1] * e[2]
+    fin.ncomp, fin.dtype = Q, 0
+    base = 1 << 40
+    fin.data = base
+    # last valid element of the last component, then its z = +1 ghost plane
+    last_valid = (n[0] - 1) + (n[1] - 1) * fin.sy + (n[2] - 1) * fin.sz + (Q - 1) * fin.sq
+    fout = abi.Field()
+    fout.n[0], fout.n[1], fout.n[2] = n
+    fout.ngrow = 0
+    fout.sy, fout.sz, fout.sq = n[0], n[0] * n[1], n[0] * n[1] * n[2]
+    fout.ncomp, fout.dtype = Q, 0
+    fout.data = base + 8 * (last_valid + 1 + 1 + fin.sy)    # beyond the x/y ghosts of that plane
+    rc = abi.lib().mfa_pull_collide(ctypes.byref(fin), ctypes.byref(fout), abi.LATTICE["D3Q19"], 1.0,
+                                    None, None)
+    assert rc == -1 and b"overlap" in abi.lib().mfa_last_error()

@@ -121,6 +121,35 @@ def test_restart_is_bit_identical(tmp_path):
         c.initialize()
 
 
+def test_restart_from_a_reference_checkpoint_of_several_boxes(tmp_path):
+    """A checkpoint in the reference's layout (tests/unit/test_restart.cpp:
+    219-226 restarts across box layouts): the populations of 4 boxes of
+    max_grid_size 16, each FAB grown by one ghost cell, in two data files.
+    The engine restarts its one box from it, bit for bit, and advances it
+    like the state written by this engine itself."""
+    from tests.test_host_cpu import write_reference_checkpoint
+    base = _read("lid_cavity_128").replace("128 128", "32 32") + \
+        f"\namr.check_file = {tmp_path}/chk\namr.chk_int = 10\nsim.max_step = 30\n"
+    a = host.Simulation(base)
+    a.initialize()
+    for step in range(0, 21):
+        a.advance()
+        a.write_output(step)
+    f20 = a.f()
+    for step in range(21, 31):
+        a.advance()
+    boxes = [([0, 0], [15, 15]), ([16, 0], [31, 15]), ([0, 16], [15, 31]), ([16, 16], [31, 31])]
+    d = write_reference_checkpoint(str(tmp_path / "ref_chk"), [32, 32], 9, f20.reshape(9, 32, 32),
+                                   boxes, ngrow=1, step=20, time=21.0)
+    b = host.Simulation(base, [f"amr.restart={d}", "amr.chk_int=0"])
+    b.initialize()
+    assert b.start_step == 21 and b.time == 21.0
+    assert np.array_equal(b.f(), f20)
+    for step in range(21, 31):
+        b.advance()
+    assert np.array_equal(b.f(), a.f())
+
+
 def test_closed_box_mass_conservation_with_refreshed_moments():
     """tests/unit/test_mass_conservation.cpp with the moments refreshed (the
     reference compares two stale rho fields): NoSlip box, tau=1, 50 steps."""

@@ -184,6 +184,21 @@ def test_level_rejects_bad_layouts_and_bindings(dev):
             lv.bind(0, [abi.field(t, [5, 8], 1, 2), abi.field(t, [5, 8], 1, 2)])
         with pytest.raises(abi.MfaError):
             lv.fill_boundary(1)                                                  # slot never bound
+        # the boxes of one slot, and an input and an output box, must not share memory
+        u = torch.zeros((2, 9, 1, 10, 6), dtype=torch.float64, device=dev)
+        shared = abi.field(u[0], [4, 8], 1, 2)
+        with pytest.raises(abi.MfaError, match="overlap"):
+            lv.bind(0, [shared, shared])
+        flat = torch.zeros(2 * 9 * 10 * 6 + 1, dtype=torch.float64, device=dev)
+        half = flat[:9 * 10 * 6].view(9, 1, 10, 6)
+        skew = flat[9 * 10 * 6 - 6:2 * 9 * 10 * 6 - 6].view(9, 1, 10, 6)    # one row into `half`
+        with pytest.raises(abi.MfaError, match="overlap"):
+            lv.bind(0, [abi.field(half, [4, 8], 1, 2), abi.field(skew, [4, 8], 1, 2)])
+        lv.bind(0, [abi.field(u[0], [4, 8], 1, 2), abi.field(u[1], [4, 8], 1, 2)])
+        w = torch.zeros((9, 1, 10, 6), dtype=torch.float64, device=dev)
+        lv.bind(1, [abi.field(w, [4, 8], 1, 2), abi.field(u[1], [4, 8], 1, 2)])  # box 1 in both slots
+        with pytest.raises(abi.MfaError, match="overlap"):
+            lv.step(0, 1, 1.0)
     finally:
         lv.close()
 

@@ -116,25 +116,74 @@ def test_inputinfo_advisories_and_refusals():
     assert r.returncode == 1 and "Both units.Re and units.nu_phys" in r.stdout
 
 
-def _fake_checkpoint(d, n, ncomp, step=20, time=21.0, version="muphfasa_checkpoint_v1", box=None,
-                     data_doubles=None, fab=True):
-    """A checkpoint directory in the muphfasa_checkpoint_v1 layout (text
-    Header; VisMF f_H / f_D_00000 with one FAB) for a 2-D box of n cells."""
+DESC = "((8, (64 11 52 0 1 12 0 1023)),(8, (1 2 3 4 5 6 7 8)))"
+
+
+def _ivect(v):
+    return "(" + ",".join(str(int(x)) for x in v) + ")"
+
+
+def _box(lo, hi):
+    return "(" + _ivect(lo) + " " + _ivect(hi) + " " + _ivect([0] * len(lo)) + ")"
+
+
+def write_reference_checkpoint(d, n, ncomp, f, boxes, ngrow=1, files=2, step=20, time=21.0,
+                               version="muphfasa_checkpoint_v1", header_box=None, ngrow_text=None,
+                               desc=DESC, fab_grow=None, short=None, skip_file=False, fab=True):
+    """A checkpoint directory laid out as the reference's writeCheckpoint
+    writes it (src/IO/Checkpoint.cpp:26-59): text Header, then VisMF::Write
+    of the populations: f_H listing the valid boxes, the ghost width and each
+    box's FAB by file and byte offset; each FAB holds the box grown by the
+    ghost width (ghost cells filled with NaN here), ncomp planes, x fastest,
+    spread over `files` data files.  f: (ncomp, [n2,] n1, n0) valid cells.
+    The keyword arguments plant one defect each."""
     import numpy as np
     os.makedirs(d, exist_ok=True)
-    hi = f"({n[0] - 1},{n[1] - 1})"
-    box = box if box is not None else f"((0,0) {hi} (0,0))"
+    dim = len(n)
+    hb = header_box if header_box is not None else _box([0] * dim, [x - 1 for x in n])
     with open(os.path.join(d, "Header"), "w") as h:
-        h.write(f"{version}\n{step}\n{time!r}\n{box}\n{ncomp}\n(0,0)\n")
-    if data_doubles is None:
-        data_doubles = n[0] * n[1] * ncomp
-    if data_doubles >= 0:
-        with open(os.path.join(d, "f_D_00000"), "wb") as f:
-            if fab:
-                f.write(f"FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (1 2 3 4 5 6 7 8)))((0,0) {hi} (0,0)) "
-                        f"{ncomp}\n".encode())
-            f.write(np.full(data_doubles, 1.0 / 9.0).tobytes())
+        h.write(f"{version}\n{step}\n{time!r}\n{hb}\n{ncomp}\n{_ivect([ngrow] * dim)}\n")
+    fg = ngrow if fab_grow is None else fab_grow
+    fods, offsets = [], {}
+    for i, (lo, hi) in enumerate(boxes):
+        name = f"f_D_{i % files:05d}"
+        path = os.path.join(d, name)
+        glo = [x - fg for x in lo]
+        ghi = [x + fg for x in hi]
+        ext = [b - a + 1 for a, b in zip(glo, ghi)]
+        data = np.full([ncomp] + ext[::-1], np.nan)
+        sl = tuple(slice(lo[k] - glo[k], lo[k] - glo[k] + hi[k] - lo[k] + 1) for k in range(dim))[::-1]
+        src = tuple(slice(lo[k], hi[k] + 1) for k in range(dim))[::-1]
+        data[(slice(None),) + sl] = f[(slice(None),) + src]
+        raw = data.astype("<f8").tobytes()
+        if short is not None and i == short:
+            raw = raw[:-8]
+        off = offsets.get(name, 0)
+        if not (skip_file and i == len(boxes) - 1):
+            with open(path, "ab") as fh:
+                head = (f"FAB {desc}{_box(glo, ghi)} {ncomp}\n" if fab else "") .encode()
+                fh.write(head + raw)
+                offsets[name] = off + len(head) + len(raw)
+        fods.append((name, off))
+    with open(os.path.join(d, "f_H"), "w") as h:
+        h.write(f"1\n0\n{ncomp}\n{ngrow_text if ngrow_text is not None else ngrow}\n")
+        h.write(f"({len(boxes)} 0\n")
+        for lo, hi in boxes:
+            h.write(_box(lo, hi) + "\n")
+        h.write(")\n")
+        h.write(f"{len(boxes)}\n")
+        for name, off in fods:
+            h.write(f"FabOnDisk: {name} {off}\n")
+        h.write("\n")
     return d
+
+
+def _fake_checkpoint(d, n, ncomp, **kw):
+    """A one-box 2-D checkpoint of n cells with populations 1/9."""
+    import numpy as np
+    f = np.full((ncomp, n[1], n[0]), 1.0 / 9.0)
+    return write_reference_checkpoint(d, n, ncomp, f, [([0, 0], [n[0] - 1, n[1] - 1])], ngrow=0,
+                                      files=1, **kw)
 
 
 def test_inputinfo_validates_restart_checkpoints(tmp_path):
@@ -151,14 +200,87 @@ def test_inputinfo_validates_restart_checkpoints(tmp_path):
     cases = {
         "no Header file found": lambda d: os.makedirs(d) or d,
         "unrecognized checkpoint version": lambda d: _fake_checkpoint(d, n, 9, version="chk_v0"),
-        "malformed Header": lambda d: _fake_checkpoint(d, n, 9, box="(0 0 15 15)"),
+        "malformed Header": lambda d: _fake_checkpoint(d, n, 9, header_box="(0 0 15 15)"),
         "domain mismatch": lambda d: _fake_checkpoint(d, [32, 32], 9),
         "holds 19 distribution components": lambda d: _fake_checkpoint(d, n, 19),
-        "does not cover the domain": lambda d: _fake_checkpoint(d, n, 9, data_doubles=16 * 16 * 9 - 1),
-        "is missing": lambda d: _fake_checkpoint(d, n, 9, data_doubles=-1),
+        "does not cover the domain": lambda d: _fake_checkpoint(d, n, 9, short=0),
+        "is missing": lambda d: _fake_checkpoint(d, n, 9, skip_file=True),
         "is not a FAB file": lambda d: _fake_checkpoint(d, n, 9, fab=False),
     }
     for i, (msg, make) in enumerate(cases.items()):
         d = make(str(tmp_path / f"bad{i}"))
         r = _inputinfo(inp, *args, f"amr.restart={d}")
         assert r.returncode == 1 and msg in r.stdout and "would refuse" in r.stdout, (msg, r.stdout)
+
+
+def _checkpoint_read(d, n, ncomp):
+    import ctypes
+    import numpy as np
+    L = host.lib()
+    dim = len(n)
+    f = np.zeros((ncomp,) + tuple(n[::-1]))
+    step, time = ctypes.c_int(), ctypes.c_double()
+    nn = (ctypes.c_int * 3)(*(list(n) + [1] * (3 - dim)))
+    rc = L.mfa_checkpoint_read(d.encode(), dim, nn, ncomp,
+                               f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               ctypes.byref(step), ctypes.byref(time))
+    return rc, f, step.value, time.value, L.mfa_sim_error().decode()
+
+
+@pytest.mark.parametrize("dim,n,ncomp,boxes,ngrow,ngrow_text", [
+    (3, [8, 6, 5], 19, [([0, 0, 0], [3, 2, 4]), ([4, 0, 0], [7, 2, 4]), ([0, 3, 0], [3, 5, 4]),
+                        ([4, 3, 0], [7, 5, 4])], 1, None),
+    (2, [13, 7], 9, [([0, 0], [4, 6]), ([5, 0], [12, 2]), ([5, 3], [12, 6])], 1, "(1,1)"),
+    (3, [5, 4, 3], 27, [([0, 0, 0], [4, 3, 1]), ([0, 0, 2], [4, 3, 2])], 2, None),
+])
+def test_restart_reads_reference_checkpoints_of_several_boxes(tmp_path, dim, n, ncomp, boxes, ngrow,
+                                                              ngrow_text):
+    """A checkpoint in the layout the reference writes (several boxes of
+    max_grid_size, each FAB grown by the ghost width, FABs at byte offsets
+    in two data files; tests/unit/test_restart.cpp:219-226 restarts across
+    box layouts) is read into this engine's one box: every valid cell in its
+    place, no ghost value (NaN in the files) taken."""
+    import numpy as np
+    rng = np.random.default_rng(dim * 100 + ncomp)
+    f = rng.standard_normal((ncomp,) + tuple(n[::-1]))
+    d = write_reference_checkpoint(str(tmp_path / "chk"), n, ncomp, f, boxes, ngrow=ngrow,
+                                   ngrow_text=ngrow_text, step=37, time=38.5)
+    rc, got, step, time, err = _checkpoint_read(d, n, ncomp)
+    assert rc == 0, err
+    assert np.array_equal(got, f)
+    assert (step, time) == (37, 38.5)
+
+
+def test_restart_refuses_malformed_reference_checkpoints(tmp_path):
+    """Each defect of the populations' VisMF data is refused with its reason
+    (the reference aborts in VisMF::Read or in readCheckpoint's coverage
+    check, src/IO/Checkpoint.cpp:131-135), and muphfasa_inputinfo reports the
+    same checkpoint unusable."""
+    import numpy as np
+    n, ncomp = [8, 6, 5], 19
+    boxes = [([0, 0, 0], [3, 5, 4]), ([4, 0, 0], [7, 5, 4])]
+    f = np.ones((ncomp,) + tuple(n[::-1]))
+    cases = {
+        "is not its box grown by the ghost width": dict(fab_grow=0),
+        "does not cover the domain": dict(short=1),
+        "little-endian IEEE doubles": dict(desc=DESC.replace("(1 2 3 4 5 6 7 8)", "(8 7 6 5 4 3 2 1)")),
+        "is missing": dict(skip_file=True),
+    }
+    for i, (msg, kw) in enumerate(cases.items()):
+        d = write_reference_checkpoint(str(tmp_path / f"c{i}"), n, ncomp, f, boxes, **kw)
+        rc, _, _, _, err = _checkpoint_read(d, n, ncomp)
+        assert rc != 0 and msg in err, (msg, err)
+    # boxes that overlap or leave a gap
+    for i, bx in enumerate([[([0, 0, 0], [4, 5, 4]), ([4, 0, 0], [7, 5, 4])],
+                            [([0, 0, 0], [3, 5, 4]), ([5, 0, 0], [7, 5, 4])]]):
+        d = write_reference_checkpoint(str(tmp_path / f"b{i}"), n, ncomp, f, bx)
+        rc, _, _, _, err = _checkpoint_read(d, n, ncomp)
+        assert rc != 0 and ("overlap" in err or "does not cover the domain" in err), err
+    inp = os.path.join(INP, "periodic_d3q19_256")
+    bad = write_reference_checkpoint(str(tmp_path / "insp"), n, ncomp, f, boxes, fab_grow=0)
+    good = write_reference_checkpoint(str(tmp_path / "insp_ok"), n, ncomp, f, boxes)
+    args = ["grid.ncells=8 6 5", "geometry.prob_hi=8 6 5"]
+    r = _inputinfo(inp, *args, f"amr.restart={good}")
+    assert r.returncode == 0 and "resumes at step 21" in r.stdout, r.stdout
+    r = _inputinfo(inp, *args, f"amr.restart={bad}")
+    assert r.returncode == 1 and "grown by the ghost width" in r.stdout, r.stdout
