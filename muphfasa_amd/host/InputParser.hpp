@@ -6,7 +6,7 @@
 //
 // Keys added by this engine (no reference counterpart):
 //   lbm.storage   = f64 | f32 | f16dev | bf16dev   (population storage type)
-//   lbm.init      = rest | noise | tgv3d           (named initial condition,
+//   lbm.init      = rest | noise | tgv3d | tgv2d   (named initial condition,
 //                   replacing the reference's link-time initData override)
 //   lbm.init_amp, lbm.seed                          (parameters of lbm.init)
 #pragma once

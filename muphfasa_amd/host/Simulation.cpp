@@ -49,7 +49,6 @@ static int bc_id(const std::string& name)
 void initData(const InputParser& params, const UnitConverter& units, const Geometry& g,
               std::vector<double>& rho, std::vector<double>& vel)
 {
-    (void)units;
     const long long nc = (long long)g.n[0] * g.n[1] * g.n[2];
     rho.assign(nc, 1.0);
     vel.assign((size_t)g.dim * nc, 0.0);
@@ -86,7 +85,27 @@ void initData(const InputParser& params, const UnitConverter& units, const Geome
                 }
         return;
     }
-    throw std::runtime_error("unknown lbm.init '" + name + "' (expected rest | noise | tgv3d)");
+    if (name == "tgv2d") {
+        // tests/regression/tgv2d/InitData.cpp:26-52 with TgvAnalytic.H:34-50 at t = 0: one
+        // vortex pair per side, k = 2 pi / n_x, amplitude units.ref_velocity in lattice units,
+        // cell centres at i + 0.5,
+        //   u = -u0 cos(kx) sin(ky),  v = u0 sin(kx) cos(ky),
+        //   rho = 1 - u0^2 / (4 cs2) (cos 2kx + cos 2ky).
+        if (g.dim != 2) throw std::runtime_error("lbm.init = tgv2d needs a 2-D lattice");
+        const double u0 = units.toLatticeVelocity(params.refVelocity());
+        const double k = 2.0 * M_PI / g.n[0];
+        for (int j = 0; j < g.n[1]; ++j)
+            for (int i = 0; i < g.n[0]; ++i) {
+                const double x = i + 0.5, y = j + 0.5;
+                const long long c = i + (long long)g.n[0] * j;
+                vel[c] = -u0 * std::cos(k * x) * std::sin(k * y);
+                vel[nc + c] = u0 * std::sin(k * x) * std::cos(k * y);
+                rho[c] = 1.0 - u0 * u0 / (4.0 / 3.0) * (std::cos(2 * k * x) + std::cos(2 * k * y));
+            }
+        return;
+    }
+    throw std::runtime_error("unknown lbm.init '" + name +
+                             "' (expected rest | noise | tgv3d | tgv2d)");
 }
 
 class SimulationImpl final : public ISimulation {
