@@ -53,6 +53,21 @@ int mfa_ensemble_init(mfa_ensemble* e, void* f, const unsigned char* masks, cons
  * lid cavity); the average of two consecutive states does not (2.6e-11). */
 int mfa_ensemble_init_warm(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks,
                            const double* rho, const double* vel, void* stream);
+/* Start from per-cell (rho, u) with options (flags, OR-ed):
+ *   MFA_START_NEQ      f = feq(rho, u) plus the first-order Chapman-Enskog
+ *                      non-equilibrium part of u, -3 (tau - 1) w_q rho Q_q:S
+ *                      (Q_q = c_q c_q - I/3, S the strain rate by central
+ *                      differences, second-order one-sided through half-way
+ *                      walls and solids, first-order towards open faces);
+ *                      without it, f = feq(rho, u) as mfa_ensemble_init;
+ *   MFA_START_AVERAGE  then one step fa -> fb and fa = (fa + fb) / 2 (fb is
+ *                      scratch), as mfa_ensemble_init_warm.
+ * mfa_ensemble_init_warm(...) is mfa_ensemble_init_start(..., MFA_START_AVERAGE).
+ * The reference initialises feq only (src/Simulation/SimulationSetup.cpp:200-219);
+ * the non-equilibrium part is this engine's addition for predicted starts. */
+enum { MFA_START_NEQ = 1, MFA_START_AVERAGE = 2 };
+int mfa_ensemble_init_start(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks,
+                            const double* rho, const double* vel, int flags, void* stream);
 int mfa_ensemble_step(mfa_ensemble* e, const void* fin, void* fout, const unsigned char* masks,
                       void* stream);
 /* Two steps fin -> fout in one launch, bit-identical to two

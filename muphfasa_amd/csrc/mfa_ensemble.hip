@@ -333,6 +333,125 @@ __global__ __launch_bounds__(256) void k_batch_init(T* __restrict__ fout, const 
     }
 }
 
+// One neighbour of a cell along an axis, for the strain-rate stencil: a fluid
+// cell (its velocity), a half-way wall at distance 1/2 (the wall's velocity),
+// or an open face (no value: zero-gradient).
+enum NbKind { NB_FLUID = 0, NB_WALL = 1, NB_OPEN = 2 };
+
+// d u_b / d x_a at a cell from its two neighbours along a (values m, p of the
+// component b, kinds km, kp) and its own value c: central differences between
+// fluid cells; second-order one-sided differences through a half-way wall
+// (the quadratic through the wall value at 1/2 and the two cells on the
+// other side); first-order one-sided towards an open face.
+__device__ __forceinline__ double ddx(int km, double m, double c, int kp, double p)
+{
+    if (km == NB_FLUID && kp == NB_FLUID) return 0.5 * (p - m);
+    if (km == NB_FLUID && kp == NB_WALL) return (4.0 / 3.0) * p - c - m * (1.0 / 3.0);
+    if (km == NB_WALL && kp == NB_FLUID) return p * (1.0 / 3.0) + c - (4.0 / 3.0) * m;
+    if (km == NB_WALL && kp == NB_WALL) return p - m;
+    if (km == NB_FLUID) return kp == NB_OPEN ? c - m : 0.0;
+    if (kp == NB_FLUID) return p - c;
+    if (km == NB_WALL) return 2.0 * (c - m);
+    if (kp == NB_WALL) return 2.0 * (p - c);
+    return 0.0;
+}
+
+// First-order Chapman-Enskog start: f_q = feq_q(rho, u) - 3 (tau - 1) w_q rho
+// Q_q:S, with Q_q = c_q c_q - I/3 and S the strain rate of the given velocity
+// field.  This is the post-collision non-equilibrium part that a BGK lattice
+// carries at a smooth flow (tau = 1/omega; the stored populations of a pull
+// step are post-collision).  Neighbours across a non-periodic face or in a
+// solid are half-way walls (NoSlip at rest; MovingWall and Inflow at their
+// resolved velocity); Outflow and PressureOutlet faces are open.  Solid cells
+// get the rest equilibrium.
+template <class L, class T, bool MASKED>
+__global__ __launch_bounds__(256) void k_batch_init_neq(T* __restrict__ fout,
+                                                        const unsigned char* __restrict__ masks,
+                                                        const Tile* __restrict__ tiles, int ntiles,
+                                                        const LatInfo* __restrict__ lats,
+                                                        const BcTable* __restrict__ bc,
+                                                        const double* __restrict__ rho,
+                                                        const double* __restrict__ vel,
+                                                        long long total_cells)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        const LatInfo li = lats[tl.lat];
+        const int x = tl.x0 + threadIdx.x, y = tl.y0 + threadIdx.y;
+        if (x >= li.nx || y >= li.ny) continue;
+        const View v = pop_view<L>(li);
+        const long long self = x + (long long)y * v.sy;
+        const long long cell = li.cell_off + x + (long long)y * li.nx;
+        T* b = fout + li.pop_off;
+        if (MASKED && masks[cell]) {
+            for_q<L>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const double fe = feq<L, q>(1.0, 0.0, 0.0, 0.0, 0.0);
+                if constexpr (std::is_same<T, double>::value) b[q * v.sq + self] = fe;
+                else b[q * v.sq + self] = from_f<T>((float)(fe - L::w(q)));
+            });
+            continue;
+        }
+        const double r = rho[cell];
+        const double u0 = vel[cell], u1 = vel[total_cells + cell];
+        // g[a][c] = d u_c / d x_a
+        double g[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            int kind[2];
+            double val[2][2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                int nx = x + (a == 0 ? 2 * s - 1 : 0), ny = y + (a == 1 ? 2 * s - 1 : 0);
+                const int n_a = a == 0 ? li.nx : li.ny;
+                const int pos = a == 0 ? nx : ny;
+                kind[s] = NB_FLUID;
+                val[s][0] = val[s][1] = 0.0;
+                if (pos < 0 || pos >= n_a) {
+                    // face ghost class: code -1/+1 along a, 0 along the other axis
+                    const int idx = a == 0 ? (s ? 2 : 0) + 3 : 1 + 3 * (s ? 2 : 0);
+                    const int rt = bc->rtype[idx];
+                    if (rt == MFA_BC_PERIODIC) {
+                        if (a == 0) nx = (nx + li.nx) % li.nx;
+                        else ny = (ny + li.ny) % li.ny;
+                    } else if (rt == MFA_BC_OUTFLOW || rt == MFA_BC_PRESSURE_OUTLET) {
+                        kind[s] = NB_OPEN;
+                        continue;
+                    } else {
+                        kind[s] = NB_WALL;
+                        if (rt != MFA_BC_NO_SLIP) {
+                            val[s][0] = bc->ru[idx][0];
+                            val[s][1] = bc->ru[idx][1];
+                        }
+                        continue;
+                    }
+                }
+                const long long nc = li.cell_off + nx + (long long)ny * li.nx;
+                if (MASKED && masks[nc]) {
+                    kind[s] = NB_WALL;
+                    continue;
+                }
+                val[s][0] = vel[nc];
+                val[s][1] = vel[total_cells + nc];
+            }
+            g[a][0] = ddx(kind[0], val[0][0], u0, kind[1], val[1][0]);
+            g[a][1] = ddx(kind[0], val[0][1], u1, kind[1], val[1][1]);
+        }
+        const double sxx = g[0][0], syy = g[1][1], sxy = 0.5 * (g[0][1] + g[1][0]);
+        const double tr3 = (sxx + syy) * (1.0 / 3.0);
+        const double pre = -3.0 * (1.0 / li.omega - 1.0) * r;
+        const double usq = usq_of<L>(u0, u1, 0.0);
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = L::c(q, 0), cy = L::c(q, 1);
+            const double qs = cx * cx * sxx + 2.0 * cx * cy * sxy + cy * cy * syy - tr3;
+            const double fe = feq<L, q>(r, u0, u1, 0.0, usq) + pre * L::w(q) * qs;
+            if constexpr (std::is_same<T, double>::value) b[q * v.sq + self] = fe;
+            else b[q * v.sq + self] = from_f<T>((float)(fe - L::w(q)));
+        });
+    }
+}
+
 // near flag of each tile: does the tile or its rim of `rim` cells hold a
 // solid cell?
 __global__ __launch_bounds__(256) void k_tile_near(const unsigned char* __restrict__ masks,
@@ -682,9 +801,39 @@ int mfa_ensemble_init(mfa_ensemble* e, void* f, const unsigned char* masks, cons
 int mfa_ensemble_init_warm(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks,
                            const double* rho, const double* vel, void* stream)
 {
-    if (!e || !fa || !fb || fa == fb) return ens_fail(MFA_ERR_INVALID, "bad populations");
-    int rc = mfa_ensemble_init(e, fa, masks, rho, vel, stream);
-    if (rc == MFA_OK) rc = mfa_ensemble_step(e, fa, fb, masks, stream);
+    return mfa_ensemble_init_start(e, fa, fb, masks, rho, vel, MFA_START_AVERAGE, stream);
+}
+
+int mfa_ensemble_init_start(mfa_ensemble* e, void* fa, void* fb, const unsigned char* masks,
+                            const double* rho, const double* vel, int flags, void* stream)
+{
+    if (!e || !fa) return ens_fail(MFA_ERR_INVALID, "null argument");
+    if (flags & ~(MFA_START_NEQ | MFA_START_AVERAGE)) return ens_fail(MFA_ERR_INVALID, "unknown start flags");
+    if ((flags & MFA_START_AVERAGE) && (!fb || fa == fb))
+        return ens_fail(MFA_ERR_INVALID, "the two-state average needs a second, distinct buffer");
+    if ((flags & MFA_START_NEQ) && (!rho || !vel))
+        return ens_fail(MFA_ERR_INVALID, "the non-equilibrium start needs rho and u");
+    int rc = MFA_OK;
+    if (flags & MFA_START_NEQ) {
+        if (e->masked && !masks) return ens_fail(MFA_ERR_INVALID, "ensemble was created with masks");
+        if (e->masked) rc = compute_near(e, masks, (hipStream_t)stream);
+        if (rc == MFA_OK && e->ntiles)
+            rc = with_type(e->dtype, [&](auto tv) {
+                using T = decltype(tv);
+                using L = Lattice<MFA_D2Q9>;
+                const dim3 block(e->tx, e->ty);
+                if (e->masked)
+                    hipLaunchKernelGGL((k_batch_init_neq<L, T, true>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (T*)fa, masks, e->d_tiles, e->ntiles, e->d_lats, e->d_bc, rho, vel, e->total_cells);
+                else
+                    hipLaunchKernelGGL((k_batch_init_neq<L, T, false>), dim3(grid_for(e)), block, 0, (hipStream_t)stream, (T*)fa, masks, e->d_tiles, e->ntiles, e->d_lats, e->d_bc, rho, vel, e->total_cells);
+                ENS_HIP(hipGetLastError());
+                return (int)MFA_OK;
+            });
+    } else {
+        rc = mfa_ensemble_init(e, fa, masks, rho, vel, stream);
+    }
+    if (rc != MFA_OK || !(flags & MFA_START_AVERAGE)) return rc;
+    rc = mfa_ensemble_step(e, fa, fb, masks, stream);
     if (rc != MFA_OK || !e->ntiles) return rc;
     return with_type(e->dtype, [&](auto tv) {
         using T = decltype(tv);

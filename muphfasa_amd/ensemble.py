@@ -42,18 +42,23 @@ def lib():
         L.mfa_ensemble_set_grid.argtypes = [vp, ctypes.c_int]
         L.mfa_ensemble_init.argtypes = [vp, vp, vp, vp, vp, vp]
         L.mfa_ensemble_init_warm.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.mfa_ensemble_init_start.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]
         L.mfa_ensemble_step.argtypes = [vp, vp, vp, vp, vp]
         L.mfa_ensemble_step2.argtypes = [vp, vp, vp, vp, vp]
         L.mfa_ensemble_run.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
         L.mfa_ensemble_moments.argtypes = [vp, vp, vp, vp, vp, vp]
         L.mfa_ensemble_residual.argtypes = [vp, vp, vp, vp, dp, vp]
         for n in ("mfa_ensemble_info", "mfa_ensemble_set_active", "mfa_ensemble_set_grid",
-                  "mfa_ensemble_init", "mfa_ensemble_init_warm", "mfa_ensemble_step", "mfa_ensemble_step2",
+                  "mfa_ensemble_init", "mfa_ensemble_init_warm", "mfa_ensemble_init_start",
+                  "mfa_ensemble_step", "mfa_ensemble_step2",
                   "mfa_ensemble_run",
                   "mfa_ensemble_moments", "mfa_ensemble_residual"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+START_NEQ, START_AVERAGE = 1, 2   # mfa_ensemble.h MFA_START_*
 
 
 def _check(rc):
@@ -131,11 +136,17 @@ class Ensemble:
     def stream(self):
         return abi.stream_ptr()
 
-    def init(self, rho=None, vel=None, average=False):
-        """f = feq(rho, u) (rest state when both are None).  average=True is
-        the warm start of mfa_ensemble_init_warm: one step, then the mean of
-        the two consecutive states."""
-        if average:
+    def init(self, rho=None, vel=None, average=False, neq=False):
+        """f = feq(rho, u) (rest state when both are None).  neq=True adds the
+        first-order Chapman-Enskog non-equilibrium part of u
+        (mfa_ensemble_init_start, MFA_START_NEQ); average=True then takes one
+        step and the mean of the two consecutive states (the warm start of
+        mfa_ensemble_init_warm)."""
+        if neq:
+            flags = START_NEQ | (START_AVERAGE if average else 0)
+            _check(lib().mfa_ensemble_init_start(self.h, _p(self.fa), _p(self.fb), _p(self.mask),
+                                                 _p(rho), _p(vel), flags, self.stream()))
+        elif average:
             _check(lib().mfa_ensemble_init_warm(self.h, _p(self.fa), _p(self.fb), _p(self.mask),
                                                 _p(rho), _p(vel), self.stream()))
         else:

@@ -171,6 +171,63 @@ def test_warm_init_average_matches_oracle(dev):
         assert np.array_equal(ens.lattice_populations(l), expect), l
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_neq_start_matches_the_chapman_enskog_statement(dev, dtype):
+    """mfa_ensemble_init_start(MFA_START_NEQ): feq of the given fields plus
+    -3 (tau - 1) w rho Q:S, against tests/ce_ref.py (the same statement in
+    numpy) on two masked cavities with the lid moving; mass and momentum
+    of every fluid cell are those of (rho, u) (the added part has neither).
+    With MFA_START_AVERAGE it is followed by one step and the mean of the
+    two states, as the plain warm start."""
+    import torch
+    import ce_ref
+    sizes = [(30, 26), (20, 22)]
+    masks = [np.zeros((26, 30), np.uint8), np.zeros((22, 20), np.uint8)]
+    masks[0][8:14, 10:19] = 1
+    masks[1][3:6, 12:16] = 1
+    masks[1][10, 0:3] = 1
+    omegas = [1.0 / 0.6, 1.0 / 0.53]
+    rng = np.random.default_rng(9)
+    rhos, vels = [], []
+    for (nx, ny), m in zip(sizes, masks):
+        yy, xx = np.meshgrid(np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+        r = 1.0 + 0.01 * np.sin(xx / 3.0) * np.cos(yy / 5.0)
+        v = 0.05 * np.stack([np.sin(yy / 4.0) + 0.3 * np.cos(xx / 2.0), np.cos(xx / 3.0) * np.sin(yy / 7.0)])
+        v += 1e-3 * rng.standard_normal(v.shape)
+        r[m == 1] = 1.0
+        v[:, m == 1] = 0.0
+        rhos.append(r)
+        vels.append(v)
+    ens = E.Ensemble(sizes, omegas, dtype=dtype, masks=masks)
+    rho_d = torch.from_numpy(np.concatenate([r.ravel() for r in rhos])).to(dev)
+    vel_d = torch.from_numpy(np.concatenate([np.concatenate([v[0].ravel() for v in vels]),
+                                             np.concatenate([v[1].ravel() for v in vels])])).to(dev)
+    ens.init(rho_d, vel_d, neq=True)
+    torch.cuda.synchronize()
+    walls = [(0.0, 0.0), (0.0, 0.0), (0.0, 0.0), (0.1, 0.0)]
+    tol = 1e-15 if dtype == "f64" else 2e-7
+    for l in range(2):
+        want = ce_ref.ce_start(rhos[l], vels[l], masks[l], omegas[l], walls)
+        got = ens.lattice_populations(l)[:, 0]
+        if dtype == "f32":
+            got = got + ce_ref.W[:, None, None]
+        assert np.abs(got - want).max() < tol, (l, np.abs(got - want).max())
+        fl = masks[l] == 0
+        neq = want - ce_ref.feq(rhos[l], vels[l])
+        assert np.abs(neq[:, fl]).max() > 1e-4                # the part is there ...
+        assert np.abs(neq.sum(axis=0)[fl]).max() < 1e-15      # ... and carries no mass
+        assert np.abs(np.einsum("qa,qyx->ayx", ce_ref.C, neq)[:, fl]).max() < 1e-15
+    if dtype == "f64":
+        f0 = [ens.lattice_populations(l).copy() for l in range(2)]
+        ens2 = E.Ensemble(sizes, omegas, dtype=dtype, masks=masks)
+        ens2.init(rho_d, vel_d, neq=True, average=True)
+        ens.step()
+        torch.cuda.synchronize()
+        for l in range(2):
+            f1 = ens.lattice_populations(l)          # the state after one step
+            assert np.array_equal(ens2.lattice_populations(l), 0.5 * (f0[l] + f1)), l
+
+
 def _masked_moments_reference(L, fab, n, mask):
     """rho, u of the pulled populations with half-way bounce-back at solid
     neighbours (a population pulled from a solid cell is the cell's own

@@ -1,9 +1,12 @@
 """What the trained network gains over the method without it, end to end on
 a small scale: the batched engine generates steady 64x64 obstacle cavities,
 the BF16-MFMA network is trained on them by the engine's own kernels, and
-held-out cavities are run to the 1e-8 steady criterion from rest (cold) and
-from the network's prediction (warm, mfa_ensemble_init_warm).  The warm start
-must need fewer steps in total and land on the cold start's fields."""
+held-out cavities are run to the 1e-8 steady criterion from rest (cold), from
+the network's prediction (warm, mfa_ensemble_init_warm; warm_neq, with the
+Chapman-Enskog non-equilibrium part of the prediction added) and from the
+cold start's own converged fields (exact, exact_neq).  The warm starts must
+need fewer steps in total and land on the cold start's fields; from exact
+fields the non-equilibrium part must save steps over feq alone."""
 import math
 
 import numpy as np
@@ -49,9 +52,9 @@ def test_trained_network_shortens_the_run_to_steady_state(dev):
           f"{t_train:.1f} s", flush=True)
     assert hist[-1][1] < 0.05 * hist[0][1]
     summary, per_case = W.evaluate(net, n_eval, tol=1e-8, max_steps=150000, N=N,
-                                   modes=("warm", "cold"), cases=_cases(n_eval, 99))
-    print({k: v for k, v in summary.items() if "warm" in k or "cold" in k or "pred" in k},
-          flush=True)
+                                   modes=("cold", "warm", "warm_neq", "exact", "exact_neq"),
+                                   cases=_cases(n_eval, 99))
+    print({k: v for k, v in summary.items() if k != "bands"}, flush=True)
     assert summary["cold_converged"] >= n_eval - 1
     assert summary["warm_both_converged"] >= n_eval - 1
     # the gain: fewer steps in total, and more than 1 case in 2 faster
@@ -62,3 +65,15 @@ def test_trained_network_shortens_the_run_to_steady_state(dev):
     # and the same converged fields (max |u_warm - u_cold| / U)
     assert summary["warm_field_diff_max"] < 1e-6
     assert summary["pred_err_l2_median"] < 0.2
+    # the Chapman-Enskog start of the prediction: the same converged fields,
+    # and fewer steps than the cold start
+    both_n = (per_case["status_warm_neq"] == "converged") & (per_case["status_cold"] == "converged")
+    assert both_n.sum() >= n_eval - 1
+    assert per_case["steps_warm_neq"][both_n].sum() < per_case["steps_cold"][both_n].sum()
+    assert summary["warm_neq_field_diff_max"] < 1e-6
+    # from the exact fields the non-equilibrium part saves steps over feq alone
+    both_x = ((per_case["status_exact"] == "converged") & (per_case["status_exact_neq"] == "converged")
+              & (per_case["status_cold"] == "converged"))
+    assert per_case["steps_exact_neq"][both_x].sum() < per_case["steps_exact"][both_x].sum()
+    assert per_case["steps_exact"][both_x].sum() < per_case["steps_cold"][both_x].sum()
+    assert summary["exact_neq_field_diff_max"] < 1e-6

@@ -1,21 +1,29 @@
-"""What any warm start can gain under the 1e-8 steady criterion, measured
-with the oracle on a small lid cavity (64^2, Re = 100, lid u = 0.1, the
-criterion of muphfasa_amd.ensemble.run_to_steady: relative L2 change of u
-over 1 000 steps below 1e-8).
+"""What starts from the EXACT steady fields gain under the 1e-8 steady
+criterion, measured with the oracle on one small lid cavity (64^2, Re = 100,
+lid u = 0.1, the criterion of muphfasa_amd.ensemble.run_to_steady: relative
+L2 change of u over 1 000 steps below 1e-8).  These are measurements of one
+case, kept so that a change in the start recipes shows up; the gains of the
+configuration-5 cavities (256^2, Re 100-3200, obstacles) are measured on the
+GPU by muphfasa_amd.warmstart, start by start and Re band by Re band.
 
-* Starting from the equilibrium of the EXACT steady (rho, u) still takes a
-  large fraction of the cold start's steps: the populations' non-equilibrium
-  part is missing and has to relax through the slow modes.  This bounds the
-  speed-up of any predictor whose output initialises feq (the warm start
-  here): it is below 3x on this case even for a perfect prediction.
+* The equilibrium of the exact steady (rho, u) gains about 1.4x: the
+  populations' non-equilibrium part is missing and has to relax through the
+  slow modes.
+* Adding the first-order Chapman-Enskog non-equilibrium part (the engine's
+  MFA_START_NEQ start, restated in tests/ce_ref.py) gains about 2.9x.
 * A closed cavity keeps its initial mass and its steady velocity scales with
   it: a start whose mass differs from the cold start's converges to a
   different field, the same start with its mass restored does not.
+* The Chapman-Enskog part is the non-equilibrium part of a steady plane
+  Couette flow (a closed form for BGK) up to terms of order u times the
+  strain rate.
 """
 import numpy as np
 import pytest
 
 from oracle import oracle as O
+
+import ce_ref
 
 N, RE, U = 64, 100.0, 0.1
 L = O.Lattice("D2Q9")
@@ -65,13 +73,12 @@ def cold():
     return _to_steady(_fab(np.ones((N, N)), np.zeros((2, N, N))))
 
 
-def test_equilibrium_start_from_exact_steady_fields_is_below_3x(cold):
+def test_equilibrium_start_from_exact_steady_fields(cold):
     s_cold, rho, u = cold
     assert 10000 < s_cold < 100000
     s_warm, _, u_w = _to_steady(_fab(rho, u))
     print(f"cold {s_cold} steps, feq(exact steady) {s_warm} steps: {s_cold / s_warm:.2f}x")
-    assert s_warm < s_cold                     # it helps ...
-    assert s_cold / s_warm < 3.0               # ... but not 3x, even with a perfect prediction
+    assert 1.2 < s_cold / s_warm < 1.6         # measured 1.37x on this case
     assert np.abs(u_w - u).max() < 1e-6 * U    # and lands on the same field
 
 
@@ -115,39 +122,46 @@ def test_average_of_two_states_lands_on_the_cold_field(cold):
     assert abs(s_avg - s_plain) <= 1000
 
 
-def _strain(u):
-    """S_ab of a cavity velocity field (2,N,N) by central differences, with
-    the half-way walls (u = 0, lid u = (U, 0)) in one-sided second-order
-    stencils at the boundary cells."""
-    g = np.zeros((2, 2, N, N))
-    walls = {0: ((0.0, 0.0), (0.0, 0.0)), 1: ((0.0, 0.0), (U, 0.0))}   # axis -> (lo, hi) wall u
-    for b in range(2):
-        for a, ax in ((0, 1), (1, 0)):
-            c = np.moveaxis(u[b], ax, 0)
-            d = np.zeros_like(c)
-            d[1:-1] = 0.5 * (c[2:] - c[:-2])
-            lo, hi = walls[a][0][b], walls[a][1][b]
-            d[0] = -4.0 / 3.0 * lo + c[0] + c[1] / 3.0
-            d[-1] = 4.0 / 3.0 * hi - c[-1] - c[-2] / 3.0
-            g[a, b] = np.moveaxis(d, 0, ax)
-    return 0.5 * (g + g.transpose(1, 0, 2, 3))
-
-
-def test_first_order_nonequilibrium_start_from_exact_fields_is_still_below_3x(cold):
-    """Adding the Chapman-Enskog non-equilibrium part of the exact steady
-    fields, f* = feq - 3 (tau - 1) w rho Q:S, to the equilibrium start helps
-    more than feq alone, and still stays below 3x at this Reynolds number."""
+def test_first_order_nonequilibrium_start_from_exact_fields(cold):
+    """The equilibrium start plus the Chapman-Enskog non-equilibrium part of
+    the exact steady fields, f* = feq - 3 (tau - 1) w rho Q:S: about twice
+    the gain of feq alone on this case, landing on the same field."""
     s_cold, rho, u = cold
-    S = _strain(u)
-    f = _fab(rho, u)
-    v = O.valid(L, f)
-    for q in range(L.Q):
-        QS = sum((C[q, a] * C[q, b] - (1.0 / 3.0 if a == b else 0.0)) * S[a, b]
-                 for a in range(2) for b in range(2))
-        v[q, 0] -= 3.0 * (TAU - 1.0) * L.w[q] * rho * QS
+    f = O.new_fab(L, n)
+    O.valid(L, f)[:, 0] = ce_ref.ce_start(rho, u, np.zeros((N, N), np.uint8), 1.0 / TAU,
+                                          [(0.0, 0.0), (0.0, 0.0), (0.0, 0.0), (U, 0.0)])
     O.apply_bc(L, f, n, [0, 0, 0], _res())
     s_neq, _, u_neq = _to_steady(f)
     print(f"cold {s_cold}, feq + first-order non-equilibrium of the exact fields {s_neq}: "
           f"{s_cold / s_neq:.2f}x")
-    assert 1.5 < s_cold / s_neq < 3.0
+    assert 2.5 < s_cold / s_neq < 3.3          # measured 2.89x on this case
     assert np.abs(u_neq - u).max() < 1e-6 * U
+
+
+def test_chapman_enskog_part_of_a_steady_couette_flow():
+    """Plane Couette (walls at y, lid u = 1e-3, periodic x) run to its steady
+    state on the oracle: the non-equilibrium part of the stored populations is
+    -3 (tau - 1) w rho Q:S of the steady field to within 1 % (the neglected
+    terms are of order u times the strain rate).  A wrong factor (tau in
+    place of tau - 1) or sign misses by more than 25 %."""
+    nx, ny, Uc, tau = 4, 16, 1e-3, 0.8
+    fu = np.zeros((6, 3))
+    fu[3, 0] = Uc
+    res = O.resolve_bc(2, [5, 5, 1, 0], np.ones(6), fu)
+    f, _, _ = O.init_equilibrium(L, np.ones((1, ny, nx)), np.zeros((2, 1, ny, nx)), [nx, ny])
+    O.fill_periodic(L, f, [nx, ny], [1, 0, 0])
+    O.apply_bc(L, f, [nx, ny], [1, 0, 0], res)
+    f = O.run(L, f, [nx, ny], 1.0 / tau, None, [1, 0], res, 40000)
+    rho, u = O.moments(L, f, [nx, ny], streamed=False)
+    rho, u = rho[0], u[:, 0]
+    assert np.abs(u[0, :, 0] - Uc * (np.arange(ny) + 0.5) / ny).max() < 1e-11 * Uc
+    fs = O.valid(L, f)[:, 0]
+    neq = fs - ce_ref.feq(rho, u)
+    walls = [(0.0, 0.0), (0.0, 0.0), (0.0, 0.0), (Uc, 0.0)]
+    mask = np.zeros((ny, nx), np.uint8)
+    err = np.abs(fs - ce_ref.ce_start(rho, u, mask, 1.0 / tau, walls, (True, False))).max()
+    print(f"non-equilibrium {np.abs(neq).max():.3e}, Chapman-Enskog error {err:.3e}")
+    assert err < 0.01 * np.abs(neq).max()
+    for bad in (1.0 / (tau + 1.0), 1.0 / (2.0 - tau)):    # tau in place of tau - 1; sign flipped
+        e = np.abs(fs - ce_ref.ce_start(rho, u, mask, bad, walls, (True, False))).max()
+        assert e > 0.25 * np.abs(neq).max()
