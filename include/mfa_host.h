@@ -29,6 +29,13 @@ int mfa_sim_advance(mfa_sim* s, int nsteps);        /* nsteps x ISimulation::adv
 int mfa_sim_update_macroscopic(mfa_sim* s, int step);
 int mfa_sim_write_output(mfa_sim* s, int step);
 int mfa_sim_sync(mfa_sim* s);
+/* Time per stage since creation, in this order: advance (device time of the
+ * steps; calls = steps), updateMacroscopic (device time), writePlotfile and
+ * writeCheckpoint (host time, device-to-host copies included), initialize
+ * (host time).  Synchronises the stream.  Stands for the reference's
+ * BL_PROFILE regions and its declared reportStageTiming
+ * (src/Simulation/SimulationSetup.H:110, SimulationSetup.cpp:122,171). */
+int mfa_sim_stage_times(mfa_sim* s, double ms[5], long long calls[5]);
 
 /* lattice id, cells n[3], omega, lattice force F[3], time, start step */
 int mfa_sim_info(mfa_sim* s, int* lattice, int n[3], double* omega, double force[3],

@@ -64,6 +64,20 @@ int mfa_level_fill_boundary(const mfa_level* lv, int slot, void* stream);
 int mfa_level_step(const mfa_level* lv, int slot_in, int slot_out, double omega, const double* force,
                    void* stream);
 
+/* The same step without the exchange: what a box pulls from outside itself
+ * is read straight from the valid cells of the box that owns it (periodic
+ * images included), so slot_in needs no mfa_level_fill_boundary and its
+ * ghost cells are never read.  Equal, bit for bit, to
+ * mfa_level_fill_boundary + mfa_level_step.  Serves a domain periodic in
+ * every dimension whose boxes form a tensor-product grid (the cut positions
+ * along each dimension are shared by all boxes, as AMReX's chopping by
+ * max_grid_size gives); MFA_ERR_UNSUPPORTED otherwise
+ * (mfa_level_direct_available tells beforehand).  No counterpart in the
+ * reference, where FillBoundary always precedes the step. */
+int mfa_level_step_direct(const mfa_level* lv, int slot_in, int slot_out, double omega,
+                          const double* force, void* stream);
+int mfa_level_direct_available(const mfa_level* lv);
+
 /* PhysBCFunct on every box: each ghost cell outside the periodic-grown
  * domain gets, for every population whose source g + c_q is a cell of the
  * box's grown region that lies in the domain, the wall value of its

@@ -26,7 +26,8 @@ EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_s
            "mfa_get_launch_dev", "mfa_stream_copy",
            "mfa_convert", "mfa_level_create", "mfa_level_destroy", "mfa_level_bind",
            "mfa_level_fill_boundary", "mfa_level_apply_bc", "mfa_level_ghost_cells",
-           "mfa_level_step", "mfa_reduce_noalloc"]
+           "mfa_level_step", "mfa_reduce_noalloc", "mfa_level_step_direct",
+           "mfa_level_direct_available"]
 
 
 class MfaError(RuntimeError):
@@ -111,6 +112,8 @@ def lib():
         L.mfa_level_apply_bc.argtypes = [vp, ctypes.c_int, G, vp]
         L.mfa_level_ghost_cells.argtypes = [vp]
         L.mfa_level_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, vp]
+        L.mfa_level_step_direct.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, vp]
+        L.mfa_level_direct_available.argtypes = [vp]
         for name in EXPORTS[2:]:
             getattr(L, name).restype = ctypes.c_int
         L.mfa_level_ghost_cells.restype = ctypes.c_longlong
@@ -270,6 +273,17 @@ class Level:
 
     def step(self, slot_in: int, slot_out: int, omega: float, force=None, stream=None):
         check(lib().mfa_level_step(self.h, slot_in, slot_out, omega, _dbl(force), stream_ptr(stream)))
+
+    @property
+    def direct_available(self) -> bool:
+        """Whether step_direct serves this layout (periodic domain, boxes on a
+        tensor-product grid)."""
+        return bool(lib().mfa_level_direct_available(self.h))
+
+    def step_direct(self, slot_in: int, slot_out: int, omega: float, force=None, stream=None):
+        """The step reading other boxes' valid cells directly (no exchange)."""
+        check(lib().mfa_level_step_direct(self.h, slot_in, slot_out, omega, _dbl(force),
+                                          stream_ptr(stream)))
 
 
 def init_equilibrium(rho: Field, vel: Field, f: Field, lattice: int, force=None, stream=None):

@@ -115,6 +115,87 @@ __global__ __launch_bounds__(256) void k_level_step(const BoxView* __restrict__ 
     collide_store<L, FORCED>(f, b.data, x + y * vo.sy + (long long)z * vo.sz, vo, omega, F0, F1, F2);
 }
 
+// The neighbours of a box on a periodic tensor-product grid of boxes, seen
+// from the box's own frame: entry (cx+1) + 3 (cy+1) + 9 (cz+1) serves the
+// cells x - c of the box that fall outside it in the directions given by the
+// sign of each c (entry 13 is the box itself).  org is the neighbour's data
+// pointer shifted so that org[q sq + x + y sy + z sz] is the neighbour's
+// population q at the box-local cell (x, y, z), periodic images included.
+struct DirectEntry {
+    const double* org;
+    long long sy, sz, sq;
+};
+
+// The step of every box reading what it pulls from outside itself straight
+// from the owning box's valid cells (no ghost exchange, ghost cells never
+// read).  NC cells along z per thread, all their loads issued before the
+// first collision, as in the one-box step.  Blocks of 64 x 4 columns; box b
+// owns blocks [bstart[b], bstart[b+1]); the 27 entries of the block's box
+// are staged in LDS.  The values pulled are those the exchange would have
+// put in the ghosts, and the collision is collide_store's, so the result
+// equals mfa_level_fill_boundary + mfa_level_step bit for bit.
+template <class L, bool FORCED, int NC>
+__global__ __launch_bounds__(256) void k_level_direct(const DirectEntry* __restrict__ table,
+                                                      const BoxView* __restrict__ vout,
+                                                      const int* __restrict__ boxn,
+                                                      const long long* __restrict__ bstart, const int nboxes,
+                                                      const double omega, const double F0, const double F1,
+                                                      const double F2)
+{
+    __shared__ DirectEntry T[27];
+    const long long nblk = bstart[nboxes];
+    const long long wg = blockIdx.x + (long long)gridDim.x * blockIdx.y;
+    if (wg >= nblk) return;                       // uniform over the block
+    const long long per = nblk / kXcds, rem = nblk % kXcds, k = wg % kXcds;
+    const long long blk = k * per + (k < rem ? k : rem) + wg / kXcds;
+    int lo = 0, hi = nboxes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const int tid = threadIdx.x + 64 * threadIdx.y;
+    if (tid < 27) T[tid] = table[27 * (long long)lo + tid];
+    __syncthreads();
+    const int nx = boxn[3 * lo], ny = boxn[3 * lo + 1], nz = boxn[3 * lo + 2];
+    const int tx = (nx + 63) / 64, ty = (ny + 3) / 4;
+    long long r = blk - bstart[lo];
+    const int bx = (int)(r % tx);
+    r /= tx;
+    const int by = (int)(r % ty);
+    const int z0 = (int)(r / ty) * NC;
+    const int x = bx * 64 + (int)threadIdx.x, y = by * 4 + (int)threadIdx.y;
+    if (x >= nx || y >= ny) return;
+    const DirectEntry own = T[13];
+    double f[NC][L::Q];
+#pragma unroll
+    for (int kk = 0; kk < NC; ++kk) {
+        const int z = z0 + kk;
+        if (NC > 1 && z >= nz) break;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
+            const int sx = x - cx, sy = y - cy, sz = z - cz;
+            const int code = (sx < 0 ? 0 : (sx >= nx ? 2 : 1)) + 3 * (sy < 0 ? 0 : (sy >= ny ? 2 : 1)) +
+                             9 * (sz < 0 ? 0 : (sz >= nz ? 2 : 1));
+            if (__builtin_expect(code == 13, 1)) {
+                f[kk][q] = own.org[q * own.sq + sx + sy * own.sy + sz * own.sz];
+            } else {
+                const DirectEntry e = T[code];
+                f[kk][q] = e.org[q * e.sq + sx + sy * e.sy + sz * e.sz];
+            }
+        });
+    }
+    const BoxView b = vout[lo];
+    const View vo{b.sy, b.sz, b.sq};
+#pragma unroll
+    for (int kk = 0; kk < NC; ++kk) {
+        const int z = z0 + kk;
+        if (NC > 1 && z >= nz) break;
+        collide_store<L, FORCED>(f[kk], b.data, x + y * vo.sy + (long long)z * vo.sz, vo, omega, F0, F1, F2);
+    }
+}
+
 // Physical boundary fill of one box whose valid cells start at lo in a
 // domain of dn cells (g.n holds the box extents).  A ghost cell outside the
 // domain in a non-periodic dimension takes, for each population q whose
@@ -191,7 +272,18 @@ struct mfa_level {
     std::vector<BoxView> views[kLevelSlots];
     // byte ranges of the bound boxes, sorted by start
     std::vector<std::pair<long long, long long>> ranges[kLevelSlots];
+    // direct step: the boxes form a tensor-product grid on a domain periodic
+    // in every dimension; grid position of each box and the box at each
+    // grid position
+    bool direct = false;
+    int gn[3] = {1, 1, 1};
+    std::vector<int> gpos, at;              // nboxes x 3; gn[0] gn[1] gn[2]
+    long long* d_dstart = nullptr;          // direct-step blocks per box, prefix sums
+    long long direct_blocks = 0;
+    DirectEntry* d_direct = nullptr;        // kLevelSlots x nboxes x 27
 };
+
+constexpr int kDirectNC = 2;
 
 // Do two sets of byte ranges, each sorted by start and disjoint within
 // itself, share a byte?
@@ -224,6 +316,8 @@ static void free_level(mfa_level* lv)
     if (lv->d_boxn) (void)hipFree(lv->d_boxn);
     if (lv->d_bstart) (void)hipFree(lv->d_bstart);
     if (lv->d_views) (void)hipFree(lv->d_views);
+    if (lv->d_dstart) (void)hipFree(lv->d_dstart);
+    if (lv->d_direct) (void)hipFree(lv->d_direct);
     delete lv;
 }
 
@@ -335,6 +429,44 @@ int mfa_level_create(int lattice, const int domain_n[3], const int periodic[3], 
         bstart[b + 1] = bstart[b] + (long long)((lv->n[3 * b] + 63) / 64) * ((lv->n[3 * b + 1] + 3) / 4) *
                                         lv->n[3 * b + 2];
     lv->step_blocks = bstart[nboxes];
+    // a tensor-product grid: the distinct box starts along each dimension
+    // cut it, every box spans one interval of each, and every grid position
+    // holds exactly one box
+    bool all_per = true;
+    for (int d = 0; d < D; ++d) all_per = all_per && per[d];
+    std::vector<int> cuts[3];
+    bool grid = all_per;
+    for (int d = 0; d < 3 && grid; ++d) {
+        for (int b = 0; b < nboxes; ++b) cuts[d].push_back(lv->lo[3 * b + d]);
+        std::sort(cuts[d].begin(), cuts[d].end());
+        cuts[d].erase(std::unique(cuts[d].begin(), cuts[d].end()), cuts[d].end());
+        lv->gn[d] = (int)cuts[d].size();
+        cuts[d].push_back(dn[d]);
+    }
+    if (grid && (long long)lv->gn[0] * lv->gn[1] * lv->gn[2] != nboxes) grid = false;
+    if (grid) {
+        lv->gpos.assign(3 * nboxes, 0);
+        lv->at.assign(nboxes, -1);
+        for (int b = 0; b < nboxes && grid; ++b) {
+            int idx = 0, mul = 1;
+            for (int d = 0; d < 3; ++d) {
+                const int i = (int)(std::lower_bound(cuts[d].begin(), cuts[d].end(), lv->lo[3 * b + d]) -
+                                    cuts[d].begin());
+                if (cuts[d][i + 1] - cuts[d][i] != lv->n[3 * b + d]) grid = false;
+                lv->gpos[3 * b + d] = i;
+                idx += i * mul;
+                mul *= lv->gn[d];
+            }
+            if (grid && lv->at[idx] >= 0) grid = false;
+            if (grid) lv->at[idx] = b;
+        }
+    }
+    lv->direct = grid;
+    std::vector<long long> dstart(nboxes + 1, 0);
+    for (int b = 0; b < nboxes; ++b)
+        dstart[b + 1] = dstart[b] + (long long)((lv->n[3 * b] + 63) / 64) * ((lv->n[3 * b + 1] + 3) / 4) *
+                                        ((lv->n[3 * b + 2] + kDirectNC - 1) / kDirectNC);
+    lv->direct_blocks = dstart[nboxes];
     hipError_t e = hipGetDevice(&lv->device);
     if (e == hipSuccess && lv->ntags) e = hipMalloc((void**)&lv->d_tags, tags.size() * sizeof(CopyTag));
     if (e == hipSuccess) e = hipMalloc((void**)&lv->d_cstart, cstart.size() * sizeof(long long));
@@ -349,6 +481,13 @@ int mfa_level_create(int lattice, const int domain_n[3], const int periodic[3], 
         e = hipMemcpy(lv->d_boxn, lv->n.data(), lv->n.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(lv->d_bstart, bstart.data(), bstart.size() * sizeof(long long), hipMemcpyHostToDevice);
+    if (e == hipSuccess && lv->direct) {
+        e = hipMalloc((void**)&lv->d_dstart, dstart.size() * sizeof(long long));
+        if (e == hipSuccess)
+            e = hipMalloc((void**)&lv->d_direct, (size_t)kLevelSlots * nboxes * 27 * sizeof(DirectEntry));
+        if (e == hipSuccess)
+            e = hipMemcpy(lv->d_dstart, dstart.data(), dstart.size() * sizeof(long long), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         free_level(lv);
         return fail(e == hipErrorOutOfMemory ? MFA_ERR_NOMEM : MFA_ERR_HIP,
@@ -388,8 +527,35 @@ int mfa_level_bind(mfa_level* lv, int slot, const mfa_field* boxes)
     for (int b = 1; b < lv->nboxes; ++b)
         if (r[b].first <= r[b - 1].second)
             return fail(MFA_ERR_INVALID, "mfa_level_bind: two boxes of the slot overlap in memory");
-    const hipError_t e = hipMemcpy(lv->d_views + (size_t)slot * lv->nboxes, v.data(),
-                                   v.size() * sizeof(BoxView), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(lv->d_views + (size_t)slot * lv->nboxes, v.data(),
+                             v.size() * sizeof(BoxView), hipMemcpyHostToDevice);
+    if (e == hipSuccess && lv->direct) {
+        // the 27 neighbour entries of every box for the direct step
+        std::vector<DirectEntry> t((size_t)lv->nboxes * 27);
+        for (int b = 0; b < lv->nboxes; ++b)
+            for (int code = 0; code < 27; ++code) {
+                const int c[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+                int idx = 0, mul = 1;
+                long long off[3];
+                for (int d = 0; d < 3; ++d) {
+                    int i = lv->gpos[3 * b + d] + c[d];
+                    long long wrap = 0;
+                    if (i < 0) { i += lv->gn[d]; wrap = lv->dn[d]; }
+                    if (i >= lv->gn[d]) { i -= lv->gn[d]; wrap = -(long long)lv->dn[d]; }
+                    idx += i * mul;
+                    mul *= lv->gn[d];
+                    off[d] = wrap;                 // lo_b - lo_a is added below
+                }
+                const int a = lv->at[idx];
+                for (int d = 0; d < 3; ++d) off[d] += lv->lo[3 * b + d] - lv->lo[3 * a + d];
+                // 2-D boxes have no z neighbours: their cz is always 0
+                const BoxView& va = v[a];
+                t[(size_t)b * 27 + code] = DirectEntry{va.data + off[0] + off[1] * va.sy + off[2] * va.sz,
+                                                       va.sy, va.sz, va.sq};
+            }
+        e = hipMemcpy(lv->d_direct + (size_t)slot * lv->nboxes * 27, t.data(), t.size() * sizeof(DirectEntry),
+                      hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) return fail(MFA_ERR_HIP, std::string("mfa_level_bind: ") + hipGetErrorString(e));
     lv->views[slot] = v;
     lv->ranges[slot] = r;
@@ -442,6 +608,42 @@ int mfa_level_step(const mfa_level* lv, int slot_in, int slot_out, double omega,
         else
             hipLaunchKernelGGL((k_level_step<L, false>), dim3(gx, gy), dim3(64, 4), 0, (hipStream_t)stream, vi,
                                vo, lv->d_boxn, lv->d_bstart, lv->nboxes, g, omega, F0, F1, F2);
+        MFA_HIP_CHECK(hipGetLastError());
+        return (int)MFA_OK;
+    });
+}
+
+int mfa_level_direct_available(const mfa_level* lv) { return lv && lv->direct ? 1 : 0; }
+
+int mfa_level_step_direct(const mfa_level* lv, int slot_in, int slot_out, double omega, const double* force,
+                          void* stream)
+{
+    if (!lv) return fail(MFA_ERR_INVALID, "mfa_level_step_direct: null level");
+    if (!lv->direct)
+        return fail(MFA_ERR_UNSUPPORTED, "mfa_level_step_direct: needs a domain periodic in every dimension "
+                                         "whose boxes form a tensor-product grid (use the exchange and "
+                                         "mfa_level_step)");
+    for (int s : {slot_in, slot_out})
+        if (s < 0 || s >= kLevelSlots || !lv->bound[s])
+            return fail(MFA_ERR_INVALID, "mfa_level_step_direct: slot not bound");
+    if (slot_in == slot_out) return fail(MFA_ERR_INVALID, "mfa_level_step_direct: slot_in == slot_out");
+    if (ranges_meet(lv->ranges[slot_in], lv->ranges[slot_out]))
+        return fail(MFA_ERR_INVALID, "mfa_level_step_direct: an input box and an output box overlap in memory");
+    return with_lattice_lv(lv->lattice, [&](auto Lc) {
+        using L = decltype(Lc);
+        const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0,
+                     F2 = (force && L::D == 3) ? force[2] : 0.0;
+        const long long nb = lv->direct_blocks;
+        const unsigned gx = (unsigned)std::min<long long>(nb, 65535);
+        const unsigned gy = (unsigned)((nb + gx - 1) / gx);
+        const DirectEntry* t = lv->d_direct + (size_t)slot_in * lv->nboxes * 27;
+        const BoxView* vo = lv->d_views + (size_t)slot_out * lv->nboxes;
+        if (force)
+            hipLaunchKernelGGL((k_level_direct<L, true, kDirectNC>), dim3(gx, gy), dim3(64, 4), 0,
+                               (hipStream_t)stream, t, vo, lv->d_boxn, lv->d_dstart, lv->nboxes, omega, F0, F1, F2);
+        else
+            hipLaunchKernelGGL((k_level_direct<L, false, kDirectNC>), dim3(gx, gy), dim3(64, 4), 0,
+                               (hipStream_t)stream, t, vo, lv->d_boxn, lv->d_dstart, lv->nboxes, omega, F0, F1, F2);
         MFA_HIP_CHECK(hipGetLastError());
         return (int)MFA_OK;
     });

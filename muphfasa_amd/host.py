@@ -16,7 +16,8 @@ EXE_PATH = os.path.join(HERE, "lib", "muphfasa")
 EXPORTS = ["mfa_sim_error", "mfa_sim_create", "mfa_sim_destroy", "mfa_sim_initialize",
            "mfa_sim_advance", "mfa_sim_update_macroscopic", "mfa_sim_write_output", "mfa_sim_sync",
            "mfa_sim_info", "mfa_sim_copy_f", "mfa_sim_copy_macro", "mfa_sim_health", "mfa_inputs_units",
-           "mfa_run_main", "mfa_checkpoint_read"]
+           "mfa_run_main", "mfa_checkpoint_read", "mfa_sim_stage_times"]
+STAGES = ["advance", "updateMacroscopic", "writePlotfile", "writeCheckpoint", "initialize"]
 _lib = None
 
 
@@ -44,6 +45,7 @@ def lib():
         L.mfa_inputs_units.argtypes = [cp, ctypes.c_int, ctypes.POINTER(cp), dp]
         L.mfa_run_main.argtypes = [ctypes.c_int, ctypes.POINTER(cp)]
         L.mfa_checkpoint_read.argtypes = [cp, ctypes.c_int, ip, ctypes.c_int, dp, ip, dp]
+        L.mfa_sim_stage_times.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_longlong)]
         _lib = L
     return _lib
 
@@ -138,6 +140,13 @@ class Simulation:
         dp = ctypes.POINTER(ctypes.c_double)
         _check(lib().mfa_sim_copy_macro(self._h, r.ctypes.data_as(dp), v.ctypes.data_as(dp)))
         return r, v
+
+    def stage_times(self):
+        """{stage: (total ms, calls)} since creation (mfa_sim_stage_times)."""
+        ms = (ctypes.c_double * 5)()
+        calls = (ctypes.c_longlong * 5)()
+        _check(lib().mfa_sim_stage_times(self._h, ms, calls))
+        return {name: (ms[i], calls[i]) for i, name in enumerate(STAGES)}
 
     def health(self):
         """(cells with a non-finite population, cells with finite density <= 0)."""

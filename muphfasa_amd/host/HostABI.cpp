@@ -62,6 +62,14 @@ int mfa_sim_update_macroscopic(mfa_sim* s, int step)
 
 int mfa_sim_write_output(mfa_sim* s, int step) { return guarded([&] { s->sim->writeOutput(step); }); }
 
+int mfa_sim_stage_times(mfa_sim* s, double ms[5], long long calls[5])
+{
+    return guarded([&] {
+        if (!ms || !calls) throw std::runtime_error("mfa_sim_stage_times: null argument");
+        s->sim->stageTimes(ms, calls);
+    });
+}
+
 int mfa_sim_health(mfa_sim* s, double out[2])
 {
     return guarded([&] {
@@ -185,6 +193,7 @@ int mfa_run_main(int argc, const char* const* argv)
             step += k;
         }
         hip_check(hipStreamSynchronize(sim->stream()), "sync");
+        sim->reportStageTiming();
     });
 }
 

@@ -3,12 +3,16 @@
 // src/Simulation/SimulationSetup.cpp:32-299 of the reference.
 #include "Simulation.hpp"
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <random>
 #include <stdexcept>
 
 namespace muphfasa {
+
+const char* const kStageNames[ISimulation::kStages] = {"advance", "updateMacroscopic", "writePlotfile",
+                                                      "writeCheckpoint", "initialize"};
 
 BoxGeom Geometry::box() const
 {
@@ -115,6 +119,13 @@ public:
     {
         for (auto& g : m_graph)
             if (g) (void)hipGraphExecDestroy(g);
+        if (m_stream) (void)hipStreamSynchronize(m_stream);
+        for (auto& p : m_pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : m_event_pool) (void)hipEventDestroy(e);
+        if (m_scratch) (void)hipFree(m_scratch);
         if (m_stream) (void)hipStreamDestroy(m_stream);
     }
 
@@ -144,8 +155,23 @@ public:
     hipStream_t stream() const noexcept override { return m_stream; }
     double totalMass() override;
     std::array<double, 2> health() override;
+    void stageTimes(double ms[kStages], long long calls[kStages]) override;
+    void reportStageTiming() override;
 
 private:
+    enum Stage { kAdvance = 0, kMacro = 1, kPlot = 2, kCheckpoint = 3, kInit = 4 };
+    // device stages: a pair of events recorded on the stream around the
+    // stage's launches, summed once the second has completed
+    struct Pending {
+        int stage;
+        hipEvent_t a, b;
+    };
+    hipEvent_t takeEvent();
+    Pending deviceStageBegin(int stage);
+    void deviceStageEnd(Pending p, long long calls);
+    void harvest(bool wait);
+    void stepOnce();
+
     void setInitialConditions();
     void restartFromCheckpoint(const std::string& dir);
     void buildBCManager();
@@ -169,7 +195,81 @@ private:
     // graphs of kGraphSteps steps starting from m_fa (0) and from m_fb (1)
     static constexpr int kGraphSteps = 64;
     hipGraphExec_t m_graph[2] = {nullptr, nullptr};
+    double m_stage_ms[kStages] = {0, 0, 0, 0, 0};
+    long long m_stage_calls[kStages] = {0, 0, 0, 0, 0};
+    std::vector<Pending> m_pending;
+    std::vector<hipEvent_t> m_event_pool;
 };
+
+hipEvent_t SimulationImpl::takeEvent()
+{
+    if (m_event_pool.empty()) {
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+    hipEvent_t e = m_event_pool.back();
+    m_event_pool.pop_back();
+    return e;
+}
+
+SimulationImpl::Pending SimulationImpl::deviceStageBegin(int stage)
+{
+    Pending p{stage, takeEvent(), takeEvent()};
+    hip_check(hipEventRecord(p.a, m_stream), "hipEventRecord");
+    return p;
+}
+
+void SimulationImpl::deviceStageEnd(Pending p, long long calls)
+{
+    hip_check(hipEventRecord(p.b, m_stream), "hipEventRecord");
+    m_stage_calls[p.stage] += calls;
+    m_pending.push_back(p);
+    // keep the list short on long runs of single steps: fold in what has
+    // completed, and wait for the oldest pairs past a bound
+    if (m_pending.size() > 256) harvest(m_pending.size() > 1024);
+}
+
+void SimulationImpl::harvest(bool wait)
+{
+    size_t keep = 0;
+    for (size_t i = 0; i < m_pending.size(); ++i) {
+        Pending& p = m_pending[i];
+        hipError_t q = wait ? hipEventSynchronize(p.b) : hipEventQuery(p.b);
+        if (q == hipErrorNotReady) {
+            m_pending[keep++] = p;
+            continue;
+        }
+        hip_check(q, "hipEventQuery");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+        m_stage_ms[p.stage] += ms;
+        m_event_pool.push_back(p.a);
+        m_event_pool.push_back(p.b);
+    }
+    m_pending.resize(keep);
+}
+
+void SimulationImpl::stageTimes(double ms[kStages], long long calls[kStages])
+{
+    harvest(true);
+    for (int i = 0; i < kStages; ++i) {
+        ms[i] = m_stage_ms[i];
+        calls[i] = m_stage_calls[i];
+    }
+}
+
+void SimulationImpl::reportStageTiming()
+{
+    double ms[kStages];
+    long long calls[kStages];
+    stageTimes(ms, calls);
+    std::printf("Stage timing (device time for advance and updateMacroscopic, host time otherwise)\n");
+    std::printf("  %-20s %14s %12s %14s\n", "stage", "total ms", "calls", "ms per call");
+    for (int i = 0; i < kStages; ++i)
+        std::printf("  %-20s %14.3f %12lld %14.6f\n", kStageNames[i], ms[i], calls[i],
+                    calls[i] ? ms[i] / (double)calls[i] : 0.0);
+}
 
 SimulationImpl::SimulationImpl(const InputParser& params) : m_params(params)
 {
@@ -252,9 +352,14 @@ void SimulationImpl::buildBCManager()
 
 void SimulationImpl::initialize()
 {
+    const auto t0 = std::chrono::steady_clock::now();
     const std::string restart = m_params.restartFile();
     if (restart.empty()) setInitialConditions();
     else restartFromCheckpoint(restart);
+    hip_check(hipStreamSynchronize(m_stream), "sync");
+    m_stage_ms[kInit] +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ++m_stage_calls[kInit];
 }
 
 void SimulationImpl::setInitialConditions()
@@ -282,7 +387,7 @@ void SimulationImpl::restartFromCheckpoint(const std::string& dir)
     m_start_step = step + 1;
 }
 
-void SimulationImpl::advance()
+void SimulationImpl::stepOnce()
 {
     // src/Simulation/SimulationSetup.cpp:119-139: fused step (pull, collide,
     // periodic wrap and wall ghost values inline), swap, time += 1.
@@ -293,8 +398,18 @@ void SimulationImpl::advance()
     m_time += 1.0;
 }
 
+void SimulationImpl::advance()
+{
+    const Pending p = deviceStageBegin(kAdvance);
+    stepOnce();
+    deviceStageEnd(p, 1);
+}
+
 void SimulationImpl::advanceSteps(int n)
 {
+    if (n <= 0) return;
+    const Pending p = deviceStageBegin(kAdvance);
+    const int total = n;
     // Whole graphs first: kGraphSteps (even) steps leave the state in the
     // buffer they started from, so m_fin / m_fout keep their roles.
     while (n >= kGraphSteps) {
@@ -305,14 +420,11 @@ void SimulationImpl::advanceSteps(int n)
             DeviceField* a = m_fin;
             DeviceField* b = m_fout;
             const double* F = m_has_forcing ? m_force.data() : nullptr;
-            // 2-D boxes and fully periodic 3-D boxes take the steps in pairs
-            // (mfa_step2, bit-identical to two mfa_step calls)
-            const bool pairs = m_lattice == MFA_D2Q9 ||
-                               (m_geom.periodic[0] && m_geom.periodic[1] && m_geom.periodic[2]);
+            // the steps go in pairs (mfa_step2, bit-identical to two mfa_step
+            // calls; FP64 boxes of every lattice, periodic or walled)
             int rc = MFA_OK;
-            for (int s = 0; s < kGraphSteps && rc == MFA_OK; s += pairs ? 2 : 1) {
-                rc = pairs ? mfa_step2(a->view(), b->view(), &m_bc, m_omega, F, m_stream)
-                           : mfa_step(a->view(), b->view(), &m_bc, m_omega, F, m_stream);
+            for (int s = 0; s < kGraphSteps && rc == MFA_OK; s += 2) {
+                rc = mfa_step2(a->view(), b->view(), &m_bc, m_omega, F, m_stream);
                 std::swap(a, b);
             }
             const hipError_t ce = hipStreamEndCapture(m_stream, &graph);
@@ -329,15 +441,18 @@ void SimulationImpl::advanceSteps(int n)
         m_time += kGraphSteps;
         n -= kGraphSteps;
     }
-    while (n-- > 0) advance();
+    while (n-- > 0) stepOnce();
+    deviceStageEnd(p, total);
 }
 
 void SimulationImpl::updateMacroscopic(int step)
 {
     if (step > m_macro_update_step) {
+        const Pending p = deviceStageBegin(kMacro);
         mfa_check(mfa_moments(m_fin->view(), m_rho.view(), m_vel.view(), &m_bc,
                               m_has_forcing ? m_force.data() : nullptr, 1, m_stream),
                   "updateMacroscopic");
+        deviceStageEnd(p, 1);
         m_macro_update_step = step;
     }
 }
@@ -363,8 +478,12 @@ std::array<double, 2> SimulationImpl::health()
 void SimulationImpl::writeOutput(int step)
 {
     // src/Simulation/SimulationSetup.cpp:168-198
+    using clock = std::chrono::steady_clock;
     if (m_params.plotInterval() > 0 && step % m_params.plotInterval() == 0) {
         updateMacroscopic(step);
+        // the stage starts once the device work queued before it is done
+        hip_check(hipStreamSynchronize(m_stream), "sync");
+        const auto t0 = clock::now();
         const int D = m_geom.dim;
         const long long nc = m_rho.cells();
         std::vector<double> data((size_t)(1 + D) * nc);
@@ -379,14 +498,20 @@ void SimulationImpl::writeOutput(int step)
         const std::string fname = plotfileName(m_params.plotfilePrefix(), step);
         writeSingleLevelPlotfile(fname, names, data.data(), m_geom.box(),
                                  m_units->toPhysicalTime(m_time), step);
+        m_stage_ms[kPlot] += std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+        ++m_stage_calls[kPlot];
         std::printf("Plotfile: wrote %s  step=%d  time=%.6g\n", fname.c_str(), step,
                     m_units->toPhysicalTime(m_time));
     }
     if (m_params.checkpointInterval() > 0 && step % m_params.checkpointInterval() == 0) {
+        hip_check(hipStreamSynchronize(m_stream), "sync");
+        const auto t0 = clock::now();
         std::vector<double> f((size_t)m_Q * m_fin->cells());
         m_fin->copyValidToHost(f.data(), m_stream);
         const std::string cname = checkpointName(m_params.checkpointPrefix(), step);
         writeCheckpoint(cname, f.data(), m_Q, m_geom.box(), step, m_time);
+        m_stage_ms[kCheckpoint] += std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+        ++m_stage_calls[kCheckpoint];
         std::printf("Checkpoint: wrote %s  step=%d  time=%.17g\n", cname.c_str(), step, m_time);
     }
 }

@@ -68,7 +68,20 @@ public:
     // Health of the current populations (host values): {cells with a
     // non-finite f_q, cells whose density sum_q f_q is finite and <= 0}
     virtual std::array<double, 2> health() = 0;
+    // Time spent per stage since creation, in kStageNames order: advance
+    // (device time of the steps; calls = steps), updateMacroscopic (device
+    // time of the moment passes that ran), writePlotfile and writeCheckpoint
+    // (host time, device-to-host copies included), initialize (host time).
+    // Synchronises the stream.  The reference scopes the same stages with
+    // BL_PROFILE and declares reportStageTiming
+    // (src/Simulation/SimulationSetup.H:110).
+    static constexpr int kStages = 5;
+    virtual void stageTimes(double ms[kStages], long long calls[kStages]) = 0;
+    // Prints the stage table to stdout.
+    virtual void reportStageTiming() = 0;
 };
+
+extern const char* const kStageNames[ISimulation::kStages];
 
 std::unique_ptr<ISimulation> makeSimulation(const InputParser& params);
 

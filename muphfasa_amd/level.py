@@ -8,8 +8,9 @@ each dimension is cut into ceil(N / max_grid_size) pieces whose sizes differ
 by at most one cell.  A step of the level is one ghost exchange
 (``mfa_level_fill_boundary``), the physical boundary fill (nothing on a
 periodic box) and the step of every box through its ghosts in one launch
-(``mfa_level_step``); the sequence of two steps is captured once in a HIP
-graph and replayed.
+(``mfa_level_step``), or the direct step that reads neighbouring boxes' cells
+in place (``mfa_level_step_direct``, periodic grids of boxes); the sequence of
+two steps is captured once in a HIP graph and replayed.
 
 ``python -m muphfasa_amd.level --n 256 --max-grid 64 --steps 200`` prints the
 MLUPS of the level and of the single box, and whether their final states are
@@ -85,8 +86,13 @@ class BoxLevel:
             full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1], l[0]:l[0] + m[0]] = \
                 self.cv[slot][b][:, 1:-1, 1:-1, 1:-1]
 
-    def step(self, src: int, omega: float, force=None):
-        """One step from slot src into the other slot."""
+    def step(self, src: int, omega: float, force=None, direct=False):
+        """One step from slot src into the other slot: the exchange, the
+        physical boundary fill and the step of every box, or (direct=True)
+        the step that reads neighbouring boxes' cells in place."""
+        if direct:
+            self.level.step_direct(src, 1 - src, omega, force)
+            return
         self.level.fill_boundary(src)
         self.level.apply_bc(src, self.g)
         self.level.step(src, 1 - src, omega, force)
@@ -144,30 +150,35 @@ def run(n=256, max_grid=64, steps=200, tau=0.8, force=(1e-6, 0.0, 0.0), dev="cud
                       "mlups": round(n ** 3 * steps / (ms * 1e-3) / 1e6, 1)}
     single = av.contiguous()
     del a, b, av, gr
-    # the level
+    # the level, with the exchange and with the direct step
     lv = BoxLevel(lat, dom, max_grid, g, dev, layout)
-    lv.scatter(full, 0)
-    with torch.cuda.stream(s):
-        lv.step(0, omega, F)
-        lv.step(1, omega, F)
-    torch.cuda.synchronize()
-    lv.scatter(full, 0)
-    gl = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(gl):
-        lv.step(0, omega, F)
-        lv.step(1, omega, F)
-    e0.record()
-    for _ in range(steps // 2):
-        gl.replay()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    out["level"] = {"boxes": len(lv.n), "ghost_cells_per_exchange": lv.level.ghost_cells,
-                    "ms_per_step": round(ms / steps, 4),
-                    "mlups": round(n ** 3 * steps / (ms * 1e-3) / 1e6, 1)}
     got = torch.empty_like(full)
-    lv.gather(got, 0)
-    out["bit_identical"] = bool(torch.equal(got.view(torch.int64), single.view(torch.int64)))
+    for direct in ((False, True) if lv.level.direct_available else (False,)):
+        lv.scatter(full, 0)
+        with torch.cuda.stream(s):
+            lv.step(0, omega, F, direct)
+            lv.step(1, omega, F, direct)
+        torch.cuda.synchronize()
+        lv.scatter(full, 0)
+        gl = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gl):
+            lv.step(0, omega, F, direct)
+            lv.step(1, omega, F, direct)
+        e0.record()
+        for _ in range(steps // 2):
+            gl.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        key = "level_direct" if direct else "level"
+        out[key] = {"boxes": len(lv.n), "ms_per_step": round(ms / steps, 4),
+                    "mlups": round(n ** 3 * steps / (ms * 1e-3) / 1e6, 1)}
+        if not direct:
+            out[key]["ghost_cells_per_exchange"] = lv.level.ghost_cells
+        lv.gather(got, 0)
+        out[key]["bit_identical"] = bool(torch.equal(got.view(torch.int64), single.view(torch.int64)))
+        del gl
+    out["bit_identical"] = all(out[k]["bit_identical"] for k in ("level", "level_direct") if k in out)
     lv.close()
     return out
 

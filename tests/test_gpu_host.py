@@ -94,6 +94,31 @@ def test_uniform_force_every_step_through_executable(tmp_path):
         assert np.max(np.abs(data[0] - 1.0)) < 1e-12
         for d in range(2):
             assert np.max(np.abs(data[1 + d] / scale - k * F[d])) < 1e-12
+    # the run ends with the stage table: 30 steps, 30 moment passes and plotfiles
+    table = r.stdout[r.stdout.index("Stage timing"):].splitlines()[2:]
+    rows = {l.split()[0]: (float(l.split()[1]), int(l.split()[2])) for l in table if l.strip()}
+    assert rows["advance"][1] == 30 and rows["advance"][0] > 0
+    assert rows["updateMacroscopic"][1] == 30 and rows["writePlotfile"][1] == 30
+    assert rows["writeCheckpoint"][1] == 0 and rows["initialize"][1] == 1
+
+
+def test_stage_times_count_every_stage(tmp_path):
+    """Per-stage timing: device time of the steps (single and graph-replayed
+    alike), of the moment passes that ran, host time of the writes."""
+    sim = host.Simulation(_read("uniform_force_2d"),
+                          ["amr.plot_int=10", f"amr.plot_file={tmp_path}/plt", "amr.chk_int=0"])
+    sim.initialize()
+    sim.advance(1)
+    sim.advance(130)
+    sim.update_macroscopic(130)
+    sim.update_macroscopic(130)           # already up to date: no pass
+    sim.write_output(130)                 # a plotfile (its moments are current)
+    t = sim.stage_times()
+    print(t)
+    assert t["advance"][1] == 131 and t["advance"][0] > 0
+    assert t["updateMacroscopic"][1] == 1 and t["updateMacroscopic"][0] > 0
+    assert t["writePlotfile"][1] == 1 and t["writeCheckpoint"][1] == 0
+    assert t["initialize"][1] == 1
 
 
 def test_restart_is_bit_identical(tmp_path):

@@ -213,3 +213,75 @@ def test_level_run_bit_identical_in_both_population_orders(dev, layout):
     print(f"case level.run layout={layout}", flush=True)
     out = level.run(n=40, max_grid=16, steps=6, dev=dev, layout=layout)
     assert out["level"]["boxes"] == 27 and out["bit_identical"]
+    assert out["level_direct"]["bit_identical"]
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q15", "D3Q19", "D3Q27"])
+@pytest.mark.parametrize("forced", [False, True])
+def test_direct_level_steps_equal_one_box_oracle(dev, lat, forced):
+    """mfa_level_step_direct on the uneven periodic grids of boxes (one of
+    them a cell thick), ghosts left NaN and never filled: three steps equal
+    the oracle on the whole domain as one box, bit for bit."""
+    import torch
+    L = O.Lattice(lat)
+    dom, lo, n = _boxes(L.D)
+    print(f"case direct level lattice={lat} forced={forced} boxes={len(lo)}", flush=True)
+    g, ref, per = bc_tables(L, "periodic")
+    F = ([2e-5, -1e-5, 3e-6][:L.D]) if forced else None
+    omega = 1.0 / 0.7
+    steps = 3
+    f0 = random_state(L, dom[:L.D], seed=11)
+    f = f0.copy()
+    for _ in range(steps):
+        O.fill_periodic(L, f, dom[:L.D], per)
+        out = O.new_fab(L, dom[:L.D])
+        O.step(L, f, out, dom[:L.D], omega, F)
+        f = out
+    expect = O.valid(L, f)
+    full0 = O.valid(L, f0)
+    ta = [to_dev(_box_fab(L, full0, lo[b], n[b]), dev) for b in range(len(lo))]
+    tb = [torch.full_like(t, float("nan")) for t in ta]
+    fa = [abi.field(t, n[b][:L.D], 1, L.D) for b, t in enumerate(ta)]
+    fb = [abi.field(t, n[b][:L.D], 1, L.D) for b, t in enumerate(tb)]
+    lv = abi.Level(abi.LATTICE[lat], dom, per, lo, n)
+    try:
+        assert lv.direct_available
+        lv.bind(0, fa)
+        lv.bind(1, fb)
+        cur = 0
+        for _ in range(steps):
+            lv.step_direct(cur, 1 - cur, omega, F)
+            cur = 1 - cur
+        torch.cuda.synchronize()
+    finally:
+        lv.close()
+    final = ta if cur == 0 else tb
+    got = _gather(L, dom, lo, n, [to_host(t) for t in final])
+    assert np.array_equal(got, expect)
+    # the ghosts were never written or needed
+    for t in final:
+        h = to_host(t)
+        assert np.isnan(h).sum() == h.size - np.isfinite(O.valid(L, h)).sum()
+
+
+def test_direct_level_refuses_other_layouts(dev):
+    """Boxes that are not a tensor-product grid, or a domain with a wall,
+    have no direct step."""
+    lv = abi.Level(abi.LATTICE["D2Q9"], [20, 10], [1, 1], [[0, 0], [10, 0], [0, 5], [7, 5]],
+                   [[10, 5], [10, 5], [7, 5], [13, 5]])
+    try:
+        assert not lv.direct_available
+        with pytest.raises(abi.MfaError, match="tensor-product grid"):
+            lv.step_direct(0, 1, 1.0)
+    finally:
+        lv.close()
+    lv = abi.Level(abi.LATTICE["D2Q9"], [20, 10], [1, 0], [[0, 0], [10, 0]], [[10, 10], [10, 10]])
+    try:
+        assert not lv.direct_available
+    finally:
+        lv.close()
+    lv = abi.Level(abi.LATTICE["D2Q9"], [20, 10], [1, 1], [[0, 0], [10, 0]], [[10, 10], [10, 10]])
+    try:
+        assert lv.direct_available
+    finally:
+        lv.close()
