@@ -77,6 +77,11 @@ int mfa_level_step(const mfa_level* lv, int slot_in, int slot_out, double omega,
 int mfa_level_step_direct(const mfa_level* lv, int slot_in, int slot_out, double omega,
                           const double* force, void* stream);
 int mfa_level_direct_available(const mfa_level* lv);
+/* Launch shape of the direct step: cells per thread along z (1, 2 or 4;
+ * default 2) and the load kind for a box's own cells (0 plain, default; 1
+ * non-temporal; 2 non-temporal for the c_x = 0 rows only).  Results do not
+ * depend on it.  Synchronises the device. */
+int mfa_level_set_direct_launch(mfa_level* lv, int cells_per_thread, int nontemporal);
 
 /* PhysBCFunct on every box: each ghost cell outside the periodic-grown
  * domain gets, for every population whose source g + c_q is a cell of the

@@ -27,7 +27,7 @@ EXPORTS = ["mfa_version", "mfa_last_error", "mfa_resolve_bc", "mfa_step", "mfa_s
            "mfa_convert", "mfa_level_create", "mfa_level_destroy", "mfa_level_bind",
            "mfa_level_fill_boundary", "mfa_level_apply_bc", "mfa_level_ghost_cells",
            "mfa_level_step", "mfa_reduce_noalloc", "mfa_level_step_direct",
-           "mfa_level_direct_available"]
+           "mfa_level_direct_available", "mfa_level_set_direct_launch"]
 
 
 class MfaError(RuntimeError):
@@ -114,6 +114,7 @@ def lib():
         L.mfa_level_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, vp]
         L.mfa_level_step_direct.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, vp]
         L.mfa_level_direct_available.argtypes = [vp]
+        L.mfa_level_set_direct_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         for name in EXPORTS[2:]:
             getattr(L, name).restype = ctypes.c_int
         L.mfa_level_ghost_cells.restype = ctypes.c_longlong
@@ -279,6 +280,9 @@ class Level:
         """Whether step_direct serves this layout (periodic domain, boxes on a
         tensor-product grid)."""
         return bool(lib().mfa_level_direct_available(self.h))
+
+    def set_direct_launch(self, cells_per_thread: int, nontemporal: int):
+        check(lib().mfa_level_set_direct_launch(self.h, cells_per_thread, nontemporal))
 
     def step_direct(self, slot_in: int, slot_out: int, omega: float, force=None, stream=None):
         """The step reading other boxes' valid cells directly (no exchange)."""

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels_common.hpp"
@@ -129,12 +130,13 @@ struct DirectEntry {
 // The step of every box reading what it pulls from outside itself straight
 // from the owning box's valid cells (no ghost exchange, ghost cells never
 // read).  NC cells along z per thread, all their loads issued before the
-// first collision, as in the one-box step.  Blocks of 64 x 4 columns; box b
+// first collision, as in the one-box step; NT = 1: non-temporal loads of the
+// box's own cells, 2: of its own c_x = 0 rows only.  Blocks of 64 x 4 columns; box b
 // owns blocks [bstart[b], bstart[b+1]); the 27 entries of the block's box
 // are staged in LDS.  The values pulled are those the exchange would have
 // put in the ghosts, and the collision is collide_store's, so the result
 // equals mfa_level_fill_boundary + mfa_level_step bit for bit.
-template <class L, bool FORCED, int NC>
+template <class L, bool FORCED, int NC, int NT>
 __global__ __launch_bounds__(256) void k_level_direct(const DirectEntry* __restrict__ table,
                                                       const BoxView* __restrict__ vout,
                                                       const int* __restrict__ boxn,
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(256) void k_level_direct(const DirectEntry* __restr
             const int code = (sx < 0 ? 0 : (sx >= nx ? 2 : 1)) + 3 * (sy < 0 ? 0 : (sy >= ny ? 2 : 1)) +
                              9 * (sz < 0 ? 0 : (sz >= nz ? 2 : 1));
             if (__builtin_expect(code == 13, 1)) {
-                f[kk][q] = own.org[q * own.sq + sx + sy * own.sy + sz * own.sz];
+                f[kk][q] = ld<NT == 1 || (NT == 2 && cx == 0)>(&own.org[q * own.sq + sx + sy * own.sy + sz * own.sz]);
             } else {
                 const DirectEntry e = T[code];
                 f[kk][q] = e.org[q * e.sq + sx + sy * e.sy + sz * e.sz];
@@ -280,10 +282,19 @@ struct mfa_level {
     std::vector<int> gpos, at;              // nboxes x 3; gn[0] gn[1] gn[2]
     long long* d_dstart = nullptr;          // direct-step blocks per box, prefix sums
     long long direct_blocks = 0;
+    int direct_nc = 2, direct_nt = 0;       // cells per thread along z, load kind
     DirectEntry* d_direct = nullptr;        // kLevelSlots x nboxes x 27
 };
 
-constexpr int kDirectNC = 2;
+// blocks of the direct step per box (64 x 4 columns of nc cells), prefix sums
+static std::vector<long long> direct_starts(const mfa_level* lv, int nc)
+{
+    std::vector<long long> d(lv->nboxes + 1, 0);
+    for (int b = 0; b < lv->nboxes; ++b)
+        d[b + 1] = d[b] + (long long)((lv->n[3 * b] + 63) / 64) * ((lv->n[3 * b + 1] + 3) / 4) *
+                              ((lv->n[3 * b + 2] + nc - 1) / nc);
+    return d;
+}
 
 // Do two sets of byte ranges, each sorted by start and disjoint within
 // itself, share a byte?
@@ -462,10 +473,7 @@ int mfa_level_create(int lattice, const int domain_n[3], const int periodic[3], 
         }
     }
     lv->direct = grid;
-    std::vector<long long> dstart(nboxes + 1, 0);
-    for (int b = 0; b < nboxes; ++b)
-        dstart[b + 1] = dstart[b] + (long long)((lv->n[3 * b] + 63) / 64) * ((lv->n[3 * b + 1] + 3) / 4) *
-                                        ((lv->n[3 * b + 2] + kDirectNC - 1) / kDirectNC);
+    const std::vector<long long> dstart = direct_starts(lv, lv->direct_nc);
     lv->direct_blocks = dstart[nboxes];
     hipError_t e = hipGetDevice(&lv->device);
     if (e == hipSuccess && lv->ntags) e = hipMalloc((void**)&lv->d_tags, tags.size() * sizeof(CopyTag));
@@ -615,6 +623,27 @@ int mfa_level_step(const mfa_level* lv, int slot_in, int slot_out, double omega,
 
 int mfa_level_direct_available(const mfa_level* lv) { return lv && lv->direct ? 1 : 0; }
 
+int mfa_level_set_direct_launch(mfa_level* lv, int cells_per_thread, int nontemporal)
+{
+    if (!lv) return fail(MFA_ERR_INVALID, "mfa_level_set_direct_launch: null level");
+    if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4)
+        return fail(MFA_ERR_INVALID, "mfa_level_set_direct_launch: cells per thread must be 1, 2 or 4");
+    if (nontemporal < 0 || nontemporal > 2)
+        return fail(MFA_ERR_INVALID, "mfa_level_set_direct_launch: nontemporal must be 0, 1 or 2");
+    if (!lv->direct) return fail(MFA_ERR_UNSUPPORTED, "mfa_level_set_direct_launch: no direct step for this level");
+    const std::vector<long long> d = direct_starts(lv, cells_per_thread);
+    // the block table is read by launches already queued: wait for them
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = hipMemcpy(lv->d_dstart, d.data(), d.size() * sizeof(long long), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return fail(MFA_ERR_HIP, std::string("mfa_level_set_direct_launch: ") + hipGetErrorString(e));
+    lv->direct_nc = cells_per_thread;
+    lv->direct_nt = nontemporal;
+    lv->direct_blocks = d[lv->nboxes];
+    return MFA_OK;
+}
+
 int mfa_level_step_direct(const mfa_level* lv, int slot_in, int slot_out, double omega, const double* force,
                           void* stream)
 {
@@ -638,12 +667,26 @@ int mfa_level_step_direct(const mfa_level* lv, int slot_in, int slot_out, double
         const unsigned gy = (unsigned)((nb + gx - 1) / gx);
         const DirectEntry* t = lv->d_direct + (size_t)slot_in * lv->nboxes * 27;
         const BoxView* vo = lv->d_views + (size_t)slot_out * lv->nboxes;
-        if (force)
-            hipLaunchKernelGGL((k_level_direct<L, true, kDirectNC>), dim3(gx, gy), dim3(64, 4), 0,
-                               (hipStream_t)stream, t, vo, lv->d_boxn, lv->d_dstart, lv->nboxes, omega, F0, F1, F2);
-        else
-            hipLaunchKernelGGL((k_level_direct<L, false, kDirectNC>), dim3(gx, gy), dim3(64, 4), 0,
-                               (hipStream_t)stream, t, vo, lv->d_boxn, lv->d_dstart, lv->nboxes, omega, F0, F1, F2);
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(gx, gy), dim3(64, 4), 0, (hipStream_t)stream, t, vo, lv->d_boxn,
+                               lv->d_dstart, lv->nboxes, omega, F0, F1, F2);
+        };
+        auto with_nt = [&](auto nc) {
+            constexpr int NC = decltype(nc)::value;
+            const int nt = lv->direct_nt;
+            if (force) {
+                if (nt == 0) launch(k_level_direct<L, true, NC, 0>);
+                else if (nt == 1) launch(k_level_direct<L, true, NC, 1>);
+                else launch(k_level_direct<L, true, NC, 2>);
+            } else {
+                if (nt == 0) launch(k_level_direct<L, false, NC, 0>);
+                else if (nt == 1) launch(k_level_direct<L, false, NC, 1>);
+                else launch(k_level_direct<L, false, NC, 2>);
+            }
+        };
+        if (lv->direct_nc == 1) with_nt(std::integral_constant<int, 1>{});
+        else if (lv->direct_nc == 2) with_nt(std::integral_constant<int, 2>{});
+        else with_nt(std::integral_constant<int, 4>{});
         MFA_HIP_CHECK(hipGetLastError());
         return (int)MFA_OK;
     });

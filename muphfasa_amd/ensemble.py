@@ -306,6 +306,61 @@ def tune_tiles(sizes, omegas, dtype="f64", steps=200, candidates=None, masks=Non
     return out
 
 
+def compare_storage(ens, r, sizes, omegas, a, tile, grid, Re, log):
+    """Runs the same cavities with storage a.compare, each for exactly the
+    steps its --dtype run took, and prints the distance of its fields from
+    the --dtype run's: max over cells of |u - u_ref| / U per cavity (median,
+    90th percentile and max over the cavities, by Re band) and the relative
+    L2 distance of u."""
+    torch = ens.torch
+    ens.set_active(np.ones(ens.nlat, np.int32))
+    ens.moments()
+    ref_vel = ens.vel.clone()
+    cmp = Ensemble(sizes, omegas, dtype=a.compare, tile=tile, grid=grid)
+    cmp.init()
+    done = 0
+    steps = r["steps"]
+    for target in sorted(set(int(x) for x in steps)):
+        if target > done:
+            cmp.run(target - done, graph=not a.no_graph)
+            done = target
+        act = cmp.active.copy()
+        act[steps == target] = 0
+        # retire the cavities whose reference run stopped here
+        if (act != cmp.active).any():
+            cmp.set_active(act)
+        log(done, int(cmp.active.sum()), float("nan"))
+    cmp.set_active(np.ones(cmp.nlat, np.int32))
+    cmp.moments()
+    torch.cuda.synchronize()
+    cells = [x * y for x, y in sizes]
+    tc = sum(cells)
+    d = (cmp.vel - ref_vel).abs() / 0.1
+    dmax = np.array([float(v.max()) for v in torch.maximum(d[:tc], d[tc:]).split(cells)])
+    num = [(float((x * x).sum()) + float((y * y).sum())) for x, y in
+           zip((cmp.vel - ref_vel)[:tc].split(cells), (cmp.vel - ref_vel)[tc:].split(cells))]
+    den = [(float((x * x).sum()) + float((y * y).sum())) for x, y in
+           zip(ref_vel[:tc].split(cells), ref_vel[tc:].split(cells))]
+    l2 = np.sqrt(np.array(num) / np.maximum(np.array(den), 1e-300))
+    ok = r["status"] == "converged"
+    bands = []
+    for lo, hi in ((50, 100), (100, 200), (200, 400), (400, 1000.01)):
+        sel = (Re >= lo) & (Re < hi) & np.isfinite(dmax)
+        if sel.any():
+            bands.append({"Re": [lo, min(hi, 1000)], "cavities": int(sel.sum()),
+                          "max_du_over_U_median": float(np.median(dmax[sel])),
+                          "max_du_over_U_max": float(dmax[sel].max())})
+    fin = np.isfinite(dmax)
+    print(json.dumps({"compare": a.compare, "against": a.dtype, "cavities": int(fin.sum()),
+                      "max_du_over_U": {"median": float(np.median(dmax[fin])),
+                                        "p90": float(np.percentile(dmax[fin], 90)),
+                                        "max": float(dmax[fin].max())},
+                      "converged_reference_max_du_over_U_median":
+                          float(np.median(dmax[ok & fin])) if (ok & fin).any() else None,
+                      "rel_l2_u": {"median": float(np.median(l2[fin])), "max": float(l2[fin].max())},
+                      "bands": bands}), flush=True)
+
+
 def main():
     import argparse
     import torch
@@ -320,6 +375,9 @@ def main():
     p.add_argument("--grid", type=int, default=0)
     p.add_argument("--no-graph", action="store_true")
     p.add_argument("--out", default="")
+    p.add_argument("--compare", default="",
+                   help="also run this storage type (e.g. f32) on the same cavities and "
+                        "report its steady fields against those of --dtype")
     a = p.parse_args()
     N, Re, tau = cavity_ensemble_params(a.n)
     sizes = [(int(x), int(x)) for x in N]
@@ -341,6 +399,8 @@ def main():
             print(json.dumps({"step": s, "active": nact, "max_res": rmax,
                               "t": round(time.time() - t0, 1)}), flush=True)
     r = run_to_steady(ens, a.check, a.tol, a.max_steps, graph=not a.no_graph, log=log)
+    if a.compare:
+        compare_storage(ens, r, sizes, omegas, a, tile, grid, Re, log)
     summary = {"config": "batched D2Q9 lid cavities", "n_lattices": a.n, "dtype": a.dtype,
                "tile": list(tile), "grid": grid, "tol": a.tol, "check": a.check,
                "max_steps": a.max_steps,

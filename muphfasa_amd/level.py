@@ -183,13 +183,66 @@ def run(n=256, max_grid=64, steps=200, tau=0.8, force=(1e-6, 0.0, 0.0), dev="cud
     return out
 
 
+def tune_direct(n=256, max_grid=64, steps=100, tau=0.8, force=(1e-6, 0.0, 0.0), dev="cuda:0",
+                layout="component"):
+    """Times the direct step for every launch shape (cells per thread along z
+    1, 2, 4; plain, non-temporal, or non-temporal c_x = 0 loads) on the
+    chopped periodic D3Q19 domain; every shape's state must equal the
+    default's bit for bit.  Returns the rows sorted by time."""
+    import torch
+    dom = [n, n, n]
+    g = abi.resolve_bc(abi.LATTICE["D3Q19"], [abi.BC["Periodic"]] * 6)
+    omega = 1.0 / tau
+    F = list(force)
+    torch.manual_seed(2024)
+    rho = torch.ones((1, n, n, n), dtype=torch.float64, device=dev)
+    vel = (torch.rand((3, n, n, n), dtype=torch.float64, device=dev) - 0.5) * 2e-3
+    full = torch.empty((19, n, n, n), dtype=torch.float64, device=dev)
+    abi.init_equilibrium(abi.field(rho, dom, 0, 3), abi.field(vel, dom, 0, 3), abi.field(full, dom, 0, 3),
+                         abi.LATTICE["D3Q19"], F)
+    del rho, vel
+    lv = BoxLevel("D3Q19", dom, max_grid, g, dev, layout)
+    rows, ref = [], None
+    got = torch.empty_like(full)
+    for nc in (1, 2, 4):
+        for nt in (0, 1, 2):
+            lv.level.set_direct_launch(nc, nt)
+            lv.scatter(full, 0)
+            for _ in range(2):
+                lv.step(0, omega, F, True)
+                lv.step(1, omega, F, True)
+            torch.cuda.synchronize()
+            lv.gather(got, 0)
+            same = True if ref is None else bool(torch.equal(got.view(torch.int64), ref.view(torch.int64)))
+            if ref is None:
+                ref = got.clone()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(steps // 2):
+                lv.step(0, omega, F, True)
+                lv.step(1, omega, F, True)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / (steps - steps % 2)
+            rows.append({"cells_per_thread": nc, "nontemporal": nt, "ms_per_step": round(ms, 4), "same": same})
+            print(json.dumps(rows[-1]), flush=True)
+    lv.level.set_direct_launch(2, 0)
+    lv.close()
+    return sorted(rows, key=lambda r: r["ms_per_step"])
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--max-grid", type=int, default=64)
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--layout", default="component", choices=["component", "row"])
+    p.add_argument("--tune-direct", action="store_true",
+                   help="time every launch shape of the direct step instead")
     a = p.parse_args()
+    if a.tune_direct:
+        print(json.dumps({"best": tune_direct(a.n, a.max_grid, a.steps, layout=a.layout)[0]}), flush=True)
+        return
     print(json.dumps(run(a.n, a.max_grid, a.steps, layout=a.layout)), flush=True)
 
 

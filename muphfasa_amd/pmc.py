@@ -36,12 +36,18 @@ def _pick(d, needle):
 
 
 def summarize(fetch_csv, write_csv, n, Q=19, elem=8, step="k_step", copy="k_copy",
-              copy_bytes=None, commands=(), steps_per_launch=1, workload=None, commit=None):
+              copy_bytes=None, commands=(), steps_per_launch=1, workload=None, commit=None,
+              cells=None):
+    """cells: lattice cells one launch updates (default n^3).  The algorithmic
+    bytes count one read and one write of every population per step; a
+    launch of two steps that keeps the intermediate state on chip needs at
+    least one read and one write per launch (minimum_bytes_per_launch)."""
     fe = per_kernel(fetch_csv, "FETCH_SIZE")
     wr = per_kernel(write_csv, "WRITE_SIZE")
     ks, kc = _pick(fe, step), _pick(fe, copy)
-    algo = steps_per_launch * 2 * Q * elem * n ** 3
-    cb = copy_bytes if copy_bytes is not None else Q * elem * n ** 3
+    cells = n ** 3 if cells is None else cells
+    algo = steps_per_launch * 2 * Q * elem * cells
+    cb = copy_bytes if copy_bytes is not None else Q * elem * cells
     mean = lambda v: sum(v) / len(v)
     # factor = known bytes / counter bytes for the copy kernel
     ff = cb / (mean(fe[kc]) * 1024)
@@ -59,6 +65,7 @@ def summarize(fetch_csv, write_csv, n, Q=19, elem=8, step="k_step", copy="k_copy
                         "fetch_factor": ff, "write_factor": wf},
         "hbm_read_bytes_per_launch": rd, "hbm_write_bytes_per_launch": wt,
         "hbm_bytes_per_launch": rd + wt, "algorithmic_bytes_per_launch": algo,
+        "minimum_bytes_per_launch": 2 * Q * elem * cells,
         "traffic_over_algorithmic": (rd + wt) / algo, "commands": list(commands),
         "commit": commit,
     }
@@ -93,12 +100,19 @@ def main():
     p.add_argument("--steps-per-launch", type=int, default=1)
     p.add_argument("--workload", default=None)
     p.add_argument("--commit", default=None, help="source commit the counters were measured at")
+    p.add_argument("--cells", type=int, default=None, help="cells per launch (default n^3)")
+    p.add_argument("--q", type=int, default=19)
+    p.add_argument("--elem", type=int, default=8, help="bytes per stored population")
+    p.add_argument("--copy", default="k_copy", help="name fragment of the calibration copy kernel")
+    p.add_argument("--copy-bytes", type=int, default=None,
+                   help="bytes the calibration copy reads (and writes) per launch")
     a = p.parse_args()
     if a.stats:
         print(stats_table(a.stats, a.top))
         return
-    r = summarize(a.fetch_csv, a.write_csv, a.n, step=a.step, commands=a.command,
-                  steps_per_launch=a.steps_per_launch, workload=a.workload, commit=a.commit)
+    r = summarize(a.fetch_csv, a.write_csv, a.n, Q=a.q, elem=a.elem, step=a.step, copy=a.copy,
+                  copy_bytes=a.copy_bytes, commands=a.command, steps_per_launch=a.steps_per_launch,
+                  workload=a.workload, commit=a.commit, cells=a.cells)
     s = json.dumps(r, indent=1)
     if a.out:
         with open(a.out, "w") as fh:

@@ -337,9 +337,53 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
     return summary, per_case
 
 
+def target_error(n, seed, gen_tol=2e-6, gen_max_steps=200000, ref_tol=1e-8, N=256):
+    """How far the training targets are from the steady state: n cavities
+    (the training distribution, another seed) generated as for training (FP32
+    storage to gen_tol) and, as the reference, in FP64 to ref_tol; per case
+    max over fluid cells of |u - u_ref| / U and the relative L2 distance of
+    u, over the cases whose reference converged, by Re band."""
+    import torch
+    cases = S.make_cases(n, seed, N)
+    fields = {}
+    runs = {}
+    for dt, tol in (("f64", ref_tol), ("f32", gen_tol)):
+        masks, Re, ens, r, stats = generate(n, seed, dtype=dt, tol=tol, max_steps=gen_max_steps, N=N,
+                                            cases=cases)
+        _log(stage="target_error_run", dtype=dt, **stats)
+        fields[dt] = ens.vel.clone()
+        runs[dt] = r
+        del ens
+        torch.cuda.empty_cache()
+    masks, Re, tau = cases
+    tc = n * N * N
+    d = (fields["f32"] - fields["f64"]).abs() / 0.1
+    dmax = torch.maximum(d[:tc], d[tc:]).view(n, -1).max(dim=1).values.cpu().numpy()
+    dv = (fields["f32"] - fields["f64"]).view(2, n, -1)
+    uv = fields["f64"].view(2, n, -1)
+    l2 = torch.sqrt((dv * dv).sum(dim=(0, 2)) / (uv * uv).sum(dim=(0, 2)).clamp_min(1e-300)).cpu().numpy()
+    ok = (runs["f64"]["status"] == "converged") & np.isfinite(dmax)
+    bands = []
+    for lo, hi in zip(RE_BANDS[:-1], RE_BANDS[1:]):
+        sel = ok & (Re >= lo) & (Re < hi)
+        if sel.any():
+            bands.append({"Re": [lo, hi], "cases": int(sel.sum()),
+                          "max_du_over_U_median": float(np.median(dmax[sel])),
+                          "rel_l2_median": float(np.median(l2[sel]))})
+    out = {"cases": n, "reference_converged": int(ok.sum()),
+           "f32_converged": int((runs["f32"]["status"] == "converged").sum()),
+           "max_du_over_U": {"median": float(np.median(dmax[ok])), "p90": float(np.percentile(dmax[ok], 90)),
+                             "max": float(dmax[ok].max())} if ok.any() else None,
+           "rel_l2_u": {"median": float(np.median(l2[ok])), "max": float(l2[ok].max())} if ok.any() else None,
+           "bands": bands}
+    _log(stage="target_error_done", **out)
+    return out
+
+
 def main():
     p = argparse.ArgumentParser()
-    p.add_argument("--stage", default="all", choices=["train", "eval", "all"])
+    p.add_argument("--stage", default="all", choices=["train", "eval", "all", "target-error"])
+    p.add_argument("--n-target", type=int, default=64, help="cases of --stage target-error")
     p.add_argument("--n-train", type=int, default=4096)
     p.add_argument("--gen-tol", type=float, default=1e-6)
     p.add_argument("--gen-max-steps", type=int, default=60000)
@@ -361,6 +405,9 @@ def main():
                    help="starts to evaluate (comma list of " + ", ".join(MODES) + ")")
     a = p.parse_args()
     import torch
+    if a.stage == "target-error":
+        target_error(a.n_target, a.seed + 7, gen_tol=a.gen_tol, gen_max_steps=a.gen_max_steps, ref_tol=a.tol)
+        return
     net = S.WarmStartNet(seed=a.seed)
     _log(stage="init", params=net.nparams)
     if a.stage in ("train", "all"):

@@ -218,14 +218,16 @@ def test_level_run_bit_identical_in_both_population_orders(dev, layout):
 
 @pytest.mark.parametrize("lat", ["D2Q9", "D3Q15", "D3Q19", "D3Q27"])
 @pytest.mark.parametrize("forced", [False, True])
-def test_direct_level_steps_equal_one_box_oracle(dev, lat, forced):
+@pytest.mark.parametrize("launch", [(2, 0), (1, 2), (4, 1)])
+def test_direct_level_steps_equal_one_box_oracle(dev, lat, forced, launch):
     """mfa_level_step_direct on the uneven periodic grids of boxes (one of
-    them a cell thick), ghosts left NaN and never filled: three steps equal
-    the oracle on the whole domain as one box, bit for bit."""
+    them a cell thick), ghosts left NaN and never filled, with each launch
+    shape (cells per thread along z, load kind): three steps equal the
+    oracle on the whole domain as one box, bit for bit."""
     import torch
     L = O.Lattice(lat)
     dom, lo, n = _boxes(L.D)
-    print(f"case direct level lattice={lat} forced={forced} boxes={len(lo)}", flush=True)
+    print(f"case direct level lattice={lat} forced={forced} launch={launch} boxes={len(lo)}", flush=True)
     g, ref, per = bc_tables(L, "periodic")
     F = ([2e-5, -1e-5, 3e-6][:L.D]) if forced else None
     omega = 1.0 / 0.7
@@ -246,6 +248,7 @@ def test_direct_level_steps_equal_one_box_oracle(dev, lat, forced):
     lv = abi.Level(abi.LATTICE[lat], dom, per, lo, n)
     try:
         assert lv.direct_available
+        lv.set_direct_launch(*launch)
         lv.bind(0, fa)
         lv.bind(1, fb)
         cur = 0
@@ -283,5 +286,8 @@ def test_direct_level_refuses_other_layouts(dev):
     lv = abi.Level(abi.LATTICE["D2Q9"], [20, 10], [1, 1], [[0, 0], [10, 0]], [[10, 10], [10, 10]])
     try:
         assert lv.direct_available
+        for bad in ((3, 0), (2, 3)):
+            with pytest.raises(abi.MfaError):
+                lv.set_direct_launch(*bad)
     finally:
         lv.close()
