@@ -45,8 +45,10 @@ int mfa_level_destroy(mfa_level* lv);
 /* Number of ghost cells the level's fill_boundary writes (all boxes). */
 long long mfa_level_ghost_cells(const mfa_level* lv);
 
-/* Bind the fields of all boxes (boxes[i]: box i, FP64, ngrow >= 1, extents
- * equal to the box's) to slot 0..3.  Copies their views to device memory
+/* Bind the fields of all boxes (boxes[i]: box i, FP64, extents equal to the
+ * box's) to slot 0..3.  The exchange, the boundary fill and mfa_level_step
+ * need ngrow >= 1; a slot whose boxes have no ghost layer (ngrow 0) serves
+ * mfa_level_step_direct only.  Copies their views to device memory
  * with a synchronous copy, so rebind a slot (after reallocating) only while
  * no enqueued level call that uses that slot is still to run. */
 int mfa_level_bind(mfa_level* lv, int slot, const mfa_field* boxes);

@@ -271,6 +271,7 @@ struct mfa_level {
     long long step_blocks = 0;
     BoxView* d_views = nullptr;             // kLevelSlots x nboxes
     bool bound[kLevelSlots] = {false, false, false, false};
+    bool ghosts[kLevelSlots] = {false, false, false, false};   // every box of the slot has ngrow >= 1
     std::vector<BoxView> views[kLevelSlots];
     // byte ranges of the bound boxes, sorted by start
     std::vector<std::pair<long long, long long>> ranges[kLevelSlots];
@@ -518,10 +519,12 @@ int mfa_level_bind(mfa_level* lv, int slot, const mfa_field* boxes)
     if (!lv || !boxes) return fail(MFA_ERR_INVALID, "mfa_level_bind: null argument");
     if (slot < 0 || slot >= kLevelSlots) return fail(MFA_ERR_INVALID, "mfa_level_bind: slot out of range");
     std::vector<BoxView> v(lv->nboxes);
+    bool ghosts = true;
     for (int b = 0; b < lv->nboxes; ++b) {
         const mfa_field* f = &boxes[b];
-        int rc = field_check(f, lv->Q, 1, "mfa_level_bind box");
+        int rc = field_check(f, lv->Q, 0, "mfa_level_bind box");
         if (rc) return rc;
+        ghosts = ghosts && f->ngrow >= 1;
         for (int d = 0; d < 3; ++d)
             if (f->n[d] != lv->n[3 * b + d])
                 return fail(MFA_ERR_INVALID, "mfa_level_bind: box " + std::to_string(b) + " has other extents");
@@ -568,6 +571,7 @@ int mfa_level_bind(mfa_level* lv, int slot, const mfa_field* boxes)
     lv->views[slot] = v;
     lv->ranges[slot] = r;
     lv->bound[slot] = true;
+    lv->ghosts[slot] = ghosts;
     return MFA_OK;
 }
 
@@ -576,6 +580,9 @@ int mfa_level_fill_boundary(const mfa_level* lv, int slot, void* stream)
     if (!lv) return fail(MFA_ERR_INVALID, "mfa_level_fill_boundary: null level");
     if (slot < 0 || slot >= kLevelSlots || !lv->bound[slot])
         return fail(MFA_ERR_INVALID, "mfa_level_fill_boundary: slot not bound");
+    if (!lv->ghosts[slot])
+        return fail(MFA_ERR_INVALID, "mfa_level_fill_boundary: the slot's boxes have no ghost layer "
+                                     "(such a slot serves mfa_level_step_direct only)");
     if (!lv->ntags) return MFA_OK;
     return with_lattice_lv(lv->lattice, [&](auto Lc) {
         using L = decltype(Lc);
@@ -597,6 +604,9 @@ int mfa_level_step(const mfa_level* lv, int slot_in, int slot_out, double omega,
         if (s < 0 || s >= kLevelSlots || !lv->bound[s])
             return fail(MFA_ERR_INVALID, "mfa_level_step: slot not bound");
     if (slot_in == slot_out) return fail(MFA_ERR_INVALID, "mfa_level_step: slot_in == slot_out");
+    if (!lv->ghosts[slot_in])
+        return fail(MFA_ERR_INVALID, "mfa_level_step: the input slot's boxes have no ghost layer "
+                                     "(such a slot serves mfa_level_step_direct only)");
     if (ranges_meet(lv->ranges[slot_in], lv->ranges[slot_out]))
         return fail(MFA_ERR_INVALID, "mfa_level_step: an input box and an output box overlap in memory");
     return with_lattice_lv(lv->lattice, [&](auto Lc) {
@@ -697,6 +707,9 @@ int mfa_level_apply_bc(const mfa_level* lv, int slot, const mfa_geom_bc* g, void
     if (!lv || !g) return fail(MFA_ERR_INVALID, "mfa_level_apply_bc: null argument");
     if (slot < 0 || slot >= kLevelSlots || !lv->bound[slot])
         return fail(MFA_ERR_INVALID, "mfa_level_apply_bc: slot not bound");
+    if (!lv->ghosts[slot])
+        return fail(MFA_ERR_INVALID, "mfa_level_apply_bc: the slot's boxes have no ghost layer "
+                                     "(such a slot serves mfa_level_step_direct only)");
     if (g->lattice != lv->lattice) return fail(MFA_ERR_INVALID, "mfa_level_apply_bc: lattice differs from the level's");
     for (int d = 0; d < lv->D; ++d)
         if ((g->periodic[d] != 0) != (lv->per[d] != 0))

@@ -52,7 +52,7 @@ class BoxLevel:
     0 and 1.  layout "component": one plane per population (the MultiFab's
     order); "row": the Q rows of one (y, z) side by side."""
 
-    def __init__(self, lattice: str, domain, max_grid: int, g, dev="cuda:0", layout="component"):
+    def __init__(self, lattice: str, domain, max_grid: int, g, dev="cuda:0", layout="component", ngrow=1):
         import torch
         self.L = abi.LATTICE[lattice]
         self.Q = {"D3Q15": 15, "D3Q19": 19, "D3Q27": 27}[lattice]
@@ -61,30 +61,34 @@ class BoxLevel:
         self.g = g
         per = [int(g.periodic[d]) for d in range(3)]
         self.level = abi.Level(self.L, self.domain, per, self.lo, self.n)
+        self.g0 = ngrow
+        e = 2 * ngrow
         if layout == "component":
-            shape = lambda m: (self.Q, m[2] + 2, m[1] + 2, m[0] + 2)
+            shape = lambda m: (self.Q, m[2] + e, m[1] + e, m[0] + e)
             self.t = [[torch.full(shape(m), float("nan"), dtype=torch.float64, device=dev) for m in self.n]
                       for _ in range(2)]
             self.cv = self.t
         else:
-            shape = lambda m: (m[2] + 2, m[1] + 2, self.Q, m[0] + 2)
+            shape = lambda m: (m[2] + e, m[1] + e, self.Q, m[0] + e)
             self.t = [[torch.full(shape(m), float("nan"), dtype=torch.float64, device=dev) for m in self.n]
                       for _ in range(2)]
             self.cv = [[t.permute(2, 0, 1, 3) for t in ts] for ts in self.t]   # (Q, z, y, x) views
-        self.f = [[abi.field(t, self.n[b], 1, 3, layout=layout) for b, t in enumerate(ts)] for ts in self.t]
+        self.f = [[abi.field(t, self.n[b], ngrow, 3, layout=layout) for b, t in enumerate(ts)] for ts in self.t]
         self.level.bind(0, self.f[0])
         self.level.bind(1, self.f[1])
+
+    def _valid(self, slot, b):
+        g, m = self.g0, self.n[b]
+        return self.cv[slot][b][:, g:g + m[2], g:g + m[1], g:g + m[0]]
 
     def scatter(self, full, slot=0):
         """Copy a (Q, nz, ny, nx) tensor of the whole domain into the boxes."""
         for b, (l, m) in enumerate(zip(self.lo, self.n)):
-            self.cv[slot][b][:, 1:-1, 1:-1, 1:-1] = full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1],
-                                                        l[0]:l[0] + m[0]]
+            self._valid(slot, b)[...] = full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1], l[0]:l[0] + m[0]]
 
     def gather(self, full, slot=0):
         for b, (l, m) in enumerate(zip(self.lo, self.n)):
-            full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1], l[0]:l[0] + m[0]] = \
-                self.cv[slot][b][:, 1:-1, 1:-1, 1:-1]
+            full[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1], l[0]:l[0] + m[0]] = self._valid(slot, b)
 
     def step(self, src: int, omega: float, force=None, direct=False):
         """One step from slot src into the other slot: the exchange, the
@@ -178,8 +182,36 @@ def run(n=256, max_grid=64, steps=200, tau=0.8, force=(1e-6, 0.0, 0.0), dev="cud
         lv.gather(got, 0)
         out[key]["bit_identical"] = bool(torch.equal(got.view(torch.int64), single.view(torch.int64)))
         del gl
-    out["bit_identical"] = all(out[k]["bit_identical"] for k in ("level", "level_direct") if k in out)
+    direct_ok = lv.level.direct_available
     lv.close()
+    if direct_ok:
+        # the direct step on boxes stored without a ghost layer
+        lv = BoxLevel(lat, dom, max_grid, g, dev, layout, ngrow=0)
+        lv.scatter(full, 0)
+        with torch.cuda.stream(s):
+            lv.step(0, omega, F, True)
+            lv.step(1, omega, F, True)
+        torch.cuda.synchronize()
+        lv.scatter(full, 0)
+        gl = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gl):
+            lv.step(0, omega, F, True)
+            lv.step(1, omega, F, True)
+        e0.record()
+        for _ in range(steps // 2):
+            gl.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        lv.gather(got, 0)
+        out["level_direct_no_ghosts"] = {
+            "boxes": len(lv.n), "ms_per_step": round(ms / steps, 4),
+            "mlups": round(n ** 3 * steps / (ms * 1e-3) / 1e6, 1),
+            "bit_identical": bool(torch.equal(got.view(torch.int64), single.view(torch.int64)))}
+        del gl
+        lv.close()
+    out["bit_identical"] = all(out[k]["bit_identical"] for k in ("level", "level_direct", "level_direct_no_ghosts")
+                               if k in out)
     return out
 
 

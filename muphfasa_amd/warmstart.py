@@ -350,7 +350,7 @@ def target_error(n, seed, gen_tol=2e-6, gen_max_steps=200000, ref_tol=1e-8, N=25
     for dt, tol in (("f64", ref_tol), ("f32", gen_tol)):
         masks, Re, ens, r, stats = generate(n, seed, dtype=dt, tol=tol, max_steps=gen_max_steps, N=N,
                                             cases=cases)
-        _log(stage="target_error_run", dtype=dt, **stats)
+        _log(stage="target_error_run", **stats)
         fields[dt] = ens.vel.clone()
         runs[dt] = r
         del ens

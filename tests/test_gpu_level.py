@@ -267,6 +267,60 @@ def test_direct_level_steps_equal_one_box_oracle(dev, lat, forced, launch):
         assert np.isnan(h).sum() == h.size - np.isfinite(O.valid(L, h)).sum()
 
 
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19"])
+def test_direct_level_on_boxes_without_ghosts(dev, lat):
+    """Boxes stored without a ghost layer (ngrow 0) serve the direct step,
+    which equals the oracle on the whole domain bit for bit; the exchange,
+    the boundary fill and the step through ghosts refuse such a slot."""
+    import torch
+    L = O.Lattice(lat)
+    dom, lo, n = _boxes(L.D)
+    print(f"case direct level no ghosts lattice={lat}", flush=True)
+    g, ref, per = bc_tables(L, "periodic")
+    F = [2e-5, -1e-5, 3e-6][:L.D]
+    omega = 1.0 / 0.7
+    f0 = random_state(L, dom[:L.D], seed=13)
+    f = f0.copy()
+    for _ in range(3):
+        O.fill_periodic(L, f, dom[:L.D], per)
+        out = O.new_fab(L, dom[:L.D])
+        O.step(L, f, out, dom[:L.D], omega, F)
+        f = out
+    expect = O.valid(L, f)
+    full0 = O.valid(L, f0)
+    def box(b):
+        l, m = lo[b], n[b]
+        if L.D == 3:
+            return full0[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1], l[0]:l[0] + m[0]]
+        return full0[:, :, l[1]:l[1] + m[1], l[0]:l[0] + m[0]]
+    ta = [to_dev(np.ascontiguousarray(box(b)), dev) for b in range(len(lo))]
+    tb = [torch.full_like(t, float("nan")) for t in ta]
+    lv = abi.Level(abi.LATTICE[lat], dom, per, lo, n)
+    try:
+        lv.bind(0, [abi.field(t, n[b][:L.D], 0, L.D) for b, t in enumerate(ta)])
+        lv.bind(1, [abi.field(t, n[b][:L.D], 0, L.D) for b, t in enumerate(tb)])
+        with pytest.raises(abi.MfaError, match="ghost layer"):
+            lv.fill_boundary(0)
+        with pytest.raises(abi.MfaError, match="ghost layer"):
+            lv.apply_bc(0, g)
+        with pytest.raises(abi.MfaError, match="ghost layer"):
+            lv.step(0, 1, omega, F)
+        cur = 0
+        for _ in range(3):
+            lv.step_direct(cur, 1 - cur, omega, F)
+            cur = 1 - cur
+        torch.cuda.synchronize()
+    finally:
+        lv.close()
+    final = ta if cur == 0 else tb
+    got = np.full((L.Q, dom[2], dom[1], dom[0]), np.nan)
+    for b, t in enumerate(final):
+        l, m = lo[b], n[b]
+        h = to_host(t).reshape(L.Q, m[2], m[1], m[0])
+        got[:, l[2]:l[2] + m[2], l[1]:l[1] + m[1], l[0]:l[0] + m[0]] = h
+    assert np.array_equal(got, expect.reshape(got.shape))
+
+
 def test_direct_level_refuses_other_layouts(dev):
     """Boxes that are not a tensor-product grid, or a domain with a wall,
     have no direct step."""
