@@ -324,3 +324,70 @@ def fields_from_prediction(pred, u_lid=0.1):
     rho = 1.0 + p[..., 0] * (RHO_SCALE * u_lid ** 2)
     u = np.stack([p[..., 1], p[..., 2]], axis=1) * u_lid
     return rho, u
+
+
+# ---- timing -----------------------------------------------------------------
+def conv_flops(net, N):
+    """FLOPs (2 per multiply-add) of one training iteration on a batch of N,
+    per kernel family: forward of every layer, data gradient of every layer
+    but the first (its input needs none), weight gradient of every layer."""
+    sh = net.shapes(N)
+    per = {name: 2 * 9 * cin * cout * sh[name][0] * sh[name][1] * sh[name][2]
+           for name, cin, cout, stride, relu, src in LAYERS}
+    return {"conv_fwd": sum(per.values()), "conv_bwd_data": sum(v for k, v in per.items() if k != "e1"),
+            "conv_bwd_weight": sum(per.values())}
+
+
+def bench(batch=16, iters=20, warmup=3, H=256, seed=0, dev="cuda:0"):
+    """Training iterations (forward, masked MSE, both backward passes, Adam)
+    of the warm-start network on random cavities: ms per iteration and
+    aggregate TFLOP/s of the convolutions."""
+    import json
+    import time
+    import torch
+    net = WarmStartNet(H=H, W=H, seed=seed, dev=dev)
+    rng = np.random.default_rng(seed)
+    masks = torch.from_numpy(np.stack([random_obstacles(rng, H) for _ in range(batch)])).to(net.dev)
+    re = torch.from_numpy(re_feature(np.exp(rng.uniform(math.log(100), math.log(3200), batch)))).to(net.dev)
+    tgt = torch.from_numpy(rng.standard_normal((batch, H, H, 3)).astype(np.float32)).to(net.dev)
+    x0 = torch.empty((batch, H, H, 8), dtype=torch.bfloat16, device=net.dev)
+    g = torch.empty((batch, H, H, 8), dtype=torch.bfloat16, device=net.dev)
+    part = torch.zeros((batch * H * H + 255) // 256, dtype=torch.float32, device=net.dev)
+    _ck(lib().mfa_nn_make_input(_p(masks), _p(re), _p(x0), batch, H, H, _s()))
+    count = float(batch * H * H * 3)
+
+    def it():
+        out = net.forward(x0)
+        _ck(lib().mfa_nn_mse(_p(out), _p(tgt), _p(masks), _p(g), _p(part), batch * H * H, 8, 3,
+                             1.0 / count, _s()))
+        net.zero_grad()
+        net.backward(g)
+        net.adam_step(1e-4)
+    for _ in range(warmup):
+        it()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(iters):
+        it()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    fl = conv_flops(net, batch)
+    out = {"batch": batch, "H": H, "iters": iters, "ms_per_iteration": round(ms, 4),
+           "conv_gflop_per_iteration": {k: round(v / 1e9, 2) for k, v in fl.items()},
+           "conv_tflops_aggregate": round(sum(fl.values()) / (ms * 1e-3) / 1e12, 2),
+           "wall_s": round(time.perf_counter() - t0, 3)}
+    print(json.dumps(out), flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    p = argparse.ArgumentParser(description="time training iterations of the warm-start network")
+    p.add_argument("--batch", type=int, default=16)
+    p.add_argument("--iters", type=int, default=20)
+    p.add_argument("--size", type=int, default=256)
+    a = p.parse_args()
+    bench(a.batch, a.iters, H=a.size)
