@@ -236,10 +236,37 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
     else inA = inA && gx >= 0 && gx < nx;
     if (py) gy = (gy % ny + ny) % ny;
     else inA = inA && gy >= 0 && gy < ny;
+    // The in-plane pull sources of the grown cell are fixed for the whole
+    // march: byte offsets (x' + y' sy) * 8 for each (c_x, c_y), with x' and y'
+    // wrapped (periodic) or clamped (walls; the clamped value is replaced by
+    // the wall value), 32-bit because the host has checked that a plane's
+    // offsets fit.  The source planes z - c_z are wave-uniform bases, so a
+    // load is one scalar base plus one vector offset.
+    unsigned srcA[3][3];
+    {
+        int xs[3], ys[3];
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci) {
+            int sx = gx - (ci - 1), sy = gy - (ci - 1);
+            if (sx < 0) sx = px ? nx - 1 : 0;
+            else if (sx >= nx) sx = px ? 0 : nx - 1;
+            if (sy < 0) sy = py ? ny - 1 : 0;
+            else if (sy >= ny) sy = py ? 0 : ny - 1;
+            xs[ci] = sx;
+            ys[ci] = sy;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                srcA[a][b] = (unsigned)(xs[a] + (long long)ys[b] * vi.sy) * (unsigned)sizeof(double);
+    }
     // phase B: one output cell
     const int ox = tid % T::TX, oy = tid / T::TX;
     const bool inB = tid < T::TC && X0 + ox < nx && Y0 + oy < ny;
     const int own = ox + 1 + T::SX * oy;    // slot of the output cell
+    const unsigned dstB =
+        inB ? (unsigned)((X0 + ox) + (long long)(Y0 + oy) * vo.sy) * (unsigned)sizeof(double) : 0u;
 
     auto has_plane = [&](int p) { return pz || (p >= 0 && p < nz); };
     // edge flags of a cell of the domain at (x, y, z)
@@ -257,17 +284,28 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
     auto loadA = [&](double (&P)[L::Q], int p) {
         if (!inA || !has_plane(p)) return;
         const int pw = (p % nz + nz) % nz;
-        pull<L, 0, false, 0>(P, fin, gx, gy, pw, gr, vi);
+        const double* plane[3];
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci) {
+            int sz = pw - (ci - 1);
+            if (sz < 0) sz = pz ? nz - 1 : 0;
+            else if (sz >= nz) sz = pz ? 0 : nz - 1;
+            plane[ci] = fin + (long long)sz * vi.sz;
+        }
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
+            P[q] = *at_bytes(plane[cz + 1] + q * vi.sq, srcA[cx + 1][cy + 1]);
+        });
         if constexpr (WALLS) {
             // pulls from behind a wall take the first step's ghost values,
             // from the populations of this cell
             int lo[3], hi[3];
             if (edges(gx, gy, pw, lo, hi)) {
-                const long long self = gx + gy * vi.sy + pw * vi.sz;
                 double s[L::Q];
                 for_q<L>([&](auto mc) {
                     constexpr int m = decltype(mc)::value;
-                    s[m] = fin[m * vi.sq + self];
+                    s[m] = *at_bytes(plane[1] + m * vi.sq, srcA[1][1]);
                 });
                 wall_fill<L>(s, gr, lo, hi, any_outlet,
                              [&](auto qc, double v) { P[decltype(qc)::value] = v; });
@@ -311,8 +349,11 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
             constexpr int q = decltype(qc)::value;
             f[q] = lds.template at<q>(pr, own);
         });
-        collide_store<L, FORCED>(f, fout, (X0 + ox) + (long long)(Y0 + oy) * vo.sy + (long long)z * vo.sz,
-                                 vo, omega, F0, F1, F2);
+        double* const orow = fout + (long long)z * vo.sz;
+        collide<L, FORCED>(f, omega, F0, F1, F2, [&](auto qc, double out) {
+            constexpr int q = decltype(qc)::value;
+            __builtin_nontemporal_store(out, at_bytes(orow + q * vo.sq, dstB));
+        });
     };
 
     // iterations Z0-2 and Z0-1 compute the two planes below the block's range

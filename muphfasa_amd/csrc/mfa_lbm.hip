@@ -933,6 +933,10 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
         const int TZ = geo.n[2] < kStep2Planes ? geo.n[2] : kStep2Planes;
         return step2_dev(fin, fout, geo, lat, omega, force, TZ, (hipStream_t)stream);
     }
+    // in-plane byte offsets are 32-bit in k_step2p
+    for (const mfa_field* f : {fin, (const mfa_field*)fout})
+        if (((long long)(f->n[0] - 1) + (long long)(f->n[1] - 1) * f->sy + 1) * 8 > 0xffffffffLL)
+            return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: a z-plane spans more than 4 GiB (use mfa_step)");
     const View vi = view_of(*fin), vo = view_of(*fout);
     const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0, F2 = force ? force[2] : 0.0;
     int ncu = 0;

@@ -195,6 +195,8 @@ private:
     // graphs of kGraphSteps steps starting from m_fa (0) and from m_fb (1)
     static constexpr int kGraphSteps = 64;
     hipGraphExec_t m_graph[2] = {nullptr, nullptr};
+    // steps in pairs through mfa_step2 until it refuses the box
+    bool m_pairs = true;
     double m_stage_ms[kStages] = {0, 0, 0, 0, 0};
     long long m_stage_calls[kStages] = {0, 0, 0, 0, 0};
     std::vector<Pending> m_pending;
@@ -421,10 +423,23 @@ void SimulationImpl::advanceSteps(int n)
             DeviceField* b = m_fout;
             const double* F = m_has_forcing ? m_force.data() : nullptr;
             // the steps go in pairs (mfa_step2, bit-identical to two mfa_step
-            // calls; FP64 boxes of every lattice, periodic or walled)
+            // calls; FP64 boxes of every lattice, periodic or walled), or one
+            // at a time where mfa_step2 refuses the box (a z-plane beyond
+            // 4 GiB); it refuses before it launches anything
             int rc = MFA_OK;
-            for (int s = 0; s < kGraphSteps && rc == MFA_OK; s += 2) {
-                rc = mfa_step2(a->view(), b->view(), &m_bc, m_omega, F, m_stream);
+            for (int s = 0; s < kGraphSteps && rc == MFA_OK;) {
+                if (m_pairs) {
+                    rc = mfa_step2(a->view(), b->view(), &m_bc, m_omega, F, m_stream);
+                    if (rc == MFA_ERR_UNSUPPORTED && s == 0) {
+                        m_pairs = false;
+                        rc = MFA_OK;
+                        continue;
+                    }
+                    s += 2;
+                } else {
+                    rc = mfa_step(a->view(), b->view(), &m_bc, m_omega, F, m_stream);
+                    s += 1;
+                }
                 std::swap(a, b);
             }
             const hipError_t ce = hipStreamEndCapture(m_stream, &graph);

@@ -269,3 +269,35 @@ def test_pull_collide_refuses_an_output_in_the_z_ghost_of_a_one_plane_box():
     rc = abi.lib().mfa_pull_collide(ctypes.byref(fin), ctypes.byref(fout), abi.LATTICE["D3Q19"], 1.0,
                                     None, None)
     assert rc == -1 and b"overlap" in abi.lib().mfa_last_error()
+
+
+def test_step2_refuses_planes_beyond_32_bit_offsets_before_launch():
+    """The 3-D pair kernels address a z-plane through 32-bit byte offsets from
+    a wave-uniform plane base, so mfa_step2 refuses (MFA_ERR_UNSUPPORTED, no
+    device work; the pointers are never dereferenced) a box whose plane spans
+    more than 4 GiB, in FP64 and FP32 storage alike.  ny_fit is the tallest
+    plane whose last in-plane byte lies below 2^32 (FP64: 512 + 55188 * 9728
+    elements = 4 294 959 104 bytes); one row more crosses it."""
+    g = abi.resolve_bc(abi.LATTICE["D3Q19"], [abi.BC["Periodic"]] * 6)
+    L = abi.lib()
+
+    def fab(addr, nx, ny, es, dtype):
+        f = abi.Field()
+        f.data = addr
+        f.n[0], f.n[1], f.n[2] = nx, ny, 2
+        f.ngrow = 0
+        f.sy, f.sq = 19 * nx, nx             # row-interleaved order
+        f.sz, f.ncomp, f.dtype = 19 * nx * ny, 19, dtype
+        return f
+
+    for es, dtype in ((8, abi.DTYPE["f64"]), (4, abi.DTYPE["f32"])):
+        nx = 512
+        # in-plane elements of the last cell + 1 = nx + (ny - 1) * 19 nx
+        ny_fit = (2 ** 32 // es - nx) // (19 * nx) + 1
+        assert (nx + (ny_fit - 1) * 19 * nx) * es <= 2 ** 32 - 1
+        assert (nx + ny_fit * 19 * nx) * es > 2 ** 32 - 1
+        a = fab(1 << 40, nx, ny_fit + 1, es, dtype)
+        b = fab(1 << 44, nx, ny_fit + 1, es, dtype)
+        rc = L.mfa_step2(ctypes.byref(a), ctypes.byref(b), ctypes.byref(g), 1.25, None, None)
+        msg = L.mfa_last_error().decode()
+        assert rc == -2 and "4 GiB" in msg, (es, rc, msg)
