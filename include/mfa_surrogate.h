@@ -44,6 +44,23 @@ int mfa_nn_mse(const float* pred, const float* target, const unsigned char* soli
                float* partial, long long npix, int C, int nt, float inv_count, void* stream);
 int mfa_nn_adam(float* w, const float* g, float* m, float* v, long long n, float lr, float beta1,
                 float beta2, float eps, int t, float weight_decay, void* stream);
+/* One tensor of a flat parameter buffer: elements [offset, offset + n).  A
+ * weight segment (w_bf16 not NULL) holds cout * 9 * cin elements [cout][9][cin]
+ * and gets its bf16 copy and, if wt_bf16 is not NULL, the transpose
+ * [9][cin][cout] written after the update; a bias segment has w_bf16 NULL. */
+typedef struct {
+    long long offset;
+    long long n;
+    int cout, cin;
+    float weight_decay;
+    void* w_bf16;
+    void* wt_bf16;
+} mfa_nn_segment;
+/* Adam (FP32, bias-corrected, step t >= 1) over every segment of the flat
+ * buffers w, g, m, v in one launch; segments follow each other from offset 0
+ * (at most 64) */
+int mfa_nn_adam_multi(float* w, const float* g, float* m, float* v, const mfa_nn_segment* seg,
+                      int nseg, float lr, float beta1, float beta2, float eps, int t, void* stream);
 /* per pixel [solid, re_feat[n], (x+.5)/W, (y+.5)/H, 1, 0, 0, 0] */
 int mfa_nn_make_input(const unsigned char* solid, const float* re_feat, void* out, int N, int H,
                       int W, void* stream);

@@ -8,16 +8,17 @@
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
 //   forward        Y[p][co]  = sum_k  X(p,k)  W[co][k],  k = tap*Cin + ci
 //   data grad      dX[p][ci] = sum_k dY(p,k) WT[k][ci],  k = tap*Cout + co
-//   weight grad    dW[co][k] = sum_p dY[p][co] X(p,k)   (split over pixels)
+//   weight grad    dW[co][k] = sum_p dY[p][co] X(p,k)   (split over pixels;
+//                  operands transposed in registers by a selector MFMA)
 // MFMA operand layout (wave64): lane l holds A[row l%16][k 8*(l/16)..+7] and
 // B[k 8*(l/16)..+7][col l%16]; the 16x16 FP32 result lands as
 // C[4*(l/16)+i][l%16], i = 0..3.  A wave computes a 32x32 tile with 2x2
 // MFMAs per 32-deep slice of k.  Channel counts are multiples of 8 so each
-// lane's 8 consecutive k of the forward / data-gradient operands are one
-// 16-byte load; the weight gradient stages both operands through LDS
-// transposed so its pixel-major reduction reads 16 bytes per lane as well.
+// lane's 8 consecutive k of every operand load is one 16-byte load.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <initializer_list>
 #include <string>
 
 #include "../../include/mfa_surrogate.h"
@@ -60,25 +61,89 @@ struct ConvShape {
 };
 
 // ---------------------------------------------------------------- forward --
-// Block: 4 waves.  WC waves along output channels (1 or 2), 4/WC along pixels;
-// a wave computes 16*TI pixels x 16*TJ channels (TI x TJ MFMA tiles per
-// 32-deep slice of k).  TI = 4, TJ = 1 serves layers with 16 or fewer output
-// channels without idle MFMA columns.
-template <int WC, int TI, int TJ>
+// Block: 4 waves.  With KS = 1, WC waves along output channels (1 or 2) and
+// 4/WC along pixels; a wave computes 16*TI pixels x 16*TJ channels (TI x TJ
+// MFMA tiles per 32-deep slice of k).  TI = 4, TJ = 1 serves layers with 16
+// or fewer output channels without idle MFMA columns.  With KS = 4 (WC = 1)
+// the four waves share one 16*TI x 16*TJ tile and each takes every fourth
+// slice of k; their partial sums meet in LDS.  That serves the deep layers
+// on small maps, whose few output tiles would otherwise leave most CUs idle
+// while each wave walks all of k = 9 * Cin (up to 1080).  The next slice's
+// operands are loaded before the current slice's MFMAs.
+template <int TI, int TJ>
+struct ConvAcc {
+    f32x4 v[TI][TJ];
+};
+
+// Sum of the KS waves' accumulators into wave 0's (KS = 4 only).
+template <int KS, int TI, int TJ>
+__device__ __forceinline__ void reduce_waves(f32x4 (&acc)[TI][TJ], int wave, int lane)
+{
+    if constexpr (KS > 1) {
+        __shared__ ConvAcc<TI, TJ> red[KS - 1][64];
+        if (wave > 0) {
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) red[wave - 1][lane].v[i][j] = acc[i][j];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int w = 0; w < KS - 1; ++w)
+#pragma unroll
+                for (int i = 0; i < TI; ++i)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j) acc[i][j] += red[w][lane].v[i][j];
+        }
+    }
+}
+
+// The k loop of one wave: slices 32 wk, 32 (wk + KS), ...  With KS > 1
+// (few waves per CU, each walking a long k) the next slice's operands are
+// loaded before the current slice's MFMAs; with KS = 1 the many resident
+// waves hide the loads and the second set of operand registers would cost
+// occupancy (measured: 36 -> 41 us for a 16-channel 256^2 layer).
+template <int KS, int TI, int TJ, class Load, class Mma>
+__device__ __forceinline__ void k_loop(int K, int wk, Load&& load, Mma&& mma)
+{
+    constexpr int STEP = 32 * KS;
+    if constexpr (KS == 1) {
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            bf16x8 a[TI], b[TJ];
+            load(k0, a, b);
+            mma(a, b);
+        }
+    } else {
+        bf16x8 a0[TI], b0[TJ], a1[TI], b1[TJ];
+        int k0 = 32 * wk;
+        if (k0 < K) load(k0, a0, b0);
+        for (; k0 < K; k0 += 2 * STEP) {
+            if (k0 + STEP < K) load(k0 + STEP, a1, b1);
+            mma(a0, b0);
+            if (k0 + STEP >= K) break;
+            if (k0 + 2 * STEP < K) load(k0 + 2 * STEP, a0, b0);
+            mma(a1, b1);
+        }
+    }
+}
+
+template <int WC, int TI, int TJ, int KS>
 __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, const bf16* __restrict__ w,
                                                   const float* __restrict__ bias, bf16* __restrict__ y,
                                                   float* __restrict__ yf, const ConvShape s, const int relu)
 {
-    constexpr int WP = 4 / WC;
+    static_assert(KS == 1 || WC == 1, "split k: the waves share one tile");
+    constexpr int WP = KS > 1 ? 1 : 4 / WC;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wp = wave % WP, wc = wave / WP;
+    const int wp = KS > 1 ? 0 : wave % WP, wc = KS > 1 ? 0 : wave / WP, wk = KS > 1 ? wave : 0;
     const long long P = (long long)s.N * s.Ho * s.Wo;
     const long long p0 = (long long)blockIdx.x * (16 * TI * WP) + 16 * TI * wp;
     const int c0 = blockIdx.y * (16 * TJ * WC) + 16 * TJ * wc;
     const int K = 9 * s.Cin;
     const int r = lane & 15, kg = lane >> 4;
 
-    // pixel coordinates of the two A rows this lane feeds
+    // pixel coordinates of the A rows this lane feeds
     int pn[TI], py[TI], px[TI];
     bool pv[TI];
 #pragma unroll
@@ -97,9 +162,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, co
 #pragma unroll
         for (int j = 0; j < TJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < K; k0 += 32) {
+    auto load = [&](int k0, bf16x8 (&a)[TI], bf16x8 (&b)[TJ]) {
         const int k = k0 + 8 * kg;
-        bf16x8 a[TI], b[TJ];
         const bool kv = k < K;
         const int tap = kv ? k / s.Cin : 0, ci = kv ? k % s.Cin : 0;
         const int ky = tap / 3, kx = tap % 3;
@@ -114,11 +178,16 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, co
             const int co = c0 + 16 * j + r;
             b[j] = (kv && co < s.Cout) ? load8(w + (long long)co * K + k) : zero8();
         }
+    };
+    auto mma = [&](const bf16x8 (&a)[TI], const bf16x8 (&b)[TJ]) {
 #pragma unroll
         for (int i = 0; i < TI; ++i)
 #pragma unroll
             for (int j = 0; j < TJ; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-    }
+    };
+    k_loop<KS, TI, TJ>(K, wk, load, mma);
+    reduce_waves<KS>(acc, wave, lane);
+    if (KS > 1 && wave != 0) return;
 #pragma unroll
     for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -142,15 +211,17 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, co
 // dX[p_in][ci] = sum over (tap, co) dY[p_out][co] * WT[tap][ci][co], with
 // p_out = ((y_in + 1 - ky) / stride, (x_in + 1 - kx) / stride) when exact.
 // Epilogue: dX = (accumulate ? dX : 0) + value * (mask ? (mask > 0) : 1).
-// Wave tile as in the forward kernel: 16*TI pixels x 16*TJ input channels.
-template <int WC, int TI, int TJ>
+// Wave tiles, split k and prefetch as in the forward kernel (pixels x input
+// channels).
+template <int WC, int TI, int TJ, int KS>
 __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ dy, const bf16* __restrict__ wt,
                                                        bf16* __restrict__ dx, const bf16* __restrict__ mask,
                                                        const ConvShape s, const int accumulate)
 {
-    constexpr int WP = 4 / WC;
+    static_assert(KS == 1 || WC == 1, "split k: the waves share one tile");
+    constexpr int WP = KS > 1 ? 1 : 4 / WC;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wp = wave % WP, wc = wave / WP;
+    const int wp = KS > 1 ? 0 : wave % WP, wc = KS > 1 ? 0 : wave / WP, wk = KS > 1 ? wave : 0;
     const long long P = (long long)s.N * s.H * s.W;
     const long long p0 = (long long)blockIdx.x * (16 * TI * WP) + 16 * TI * wp;
     const int c0 = blockIdx.y * (16 * TJ * WC) + 16 * TJ * wc;
@@ -173,12 +244,11 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ 
     for (int i = 0; i < TI; ++i)
 #pragma unroll
         for (int j = 0; j < TJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
+    auto load = [&](int k0, bf16x8 (&a)[TI], bf16x8 (&b)[TJ]) {
         const int k = k0 + 8 * kg;
         const bool kv = k < K;
         const int tap = kv ? k / s.Cout : 0, co = kv ? k % s.Cout : 0;
         const int ky = tap / 3, kx = tap % 3;
-        bf16x8 a[TI], b[TJ];
 #pragma unroll
         for (int i = 0; i < TI; ++i) {
             const int ty = py[i] + 1 - ky, tx = px[i] + 1 - kx;
@@ -196,11 +266,16 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ 
             const int ci = c0 + 16 * j + r;
             b[j] = (kv && ci < s.Cin) ? load8(wt + ((long long)tap * s.Cin + ci) * s.Cout + co) : zero8();
         }
+    };
+    auto mma = [&](const bf16x8 (&a)[TI], const bf16x8 (&b)[TJ]) {
 #pragma unroll
         for (int i = 0; i < TI; ++i)
 #pragma unroll
             for (int j = 0; j < TJ; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-    }
+    };
+    k_loop<KS, TI, TJ>(K, wk, load, mma);
+    reduce_waves<KS>(acc, wave, lane);
+    if (KS > 1 && wave != 0) return;
 #pragma unroll
     for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -221,106 +296,187 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ 
 }
 
 // -------------------------------------------------------- weight gradient --
-// Block tile: 64 output channels x 64 columns of k; each block reduces one
-// contiguous range of output pixels (split over blockIdx.z) and adds its
-// FP32 partial into dW (and dbias) atomically.  Per stage of kWgStage
-// pixels the dY and im2col(X) tiles are staged in LDS transposed
-// ([row][pixel]) so each lane's 8 pixels are one 16-byte LDS read; with 128
-// pixels a stage each wave issues 16 MFMAs between two barriers.
-constexpr int kWgStage = 128;
+// dW[co][k] = sum_p dY[p][co] X(p,k) and dbias[co] = sum_p dY[p][co]: the
+// MFMA's reduction runs over pixels, so each lane must hold 8 pixels of one
+// channel, while NHWC storage gives 8 channels of one pixel per 16-byte load.
+// The operands are transposed in registers by an MFMA against a selector
+// matrix: with lane l holding D[pixel l%16][channel 8(l/16)..+7] (16 pixels
+// x 32 channels, one natural load), D x S_h, S_h[k][n] = (k == n + 16h),
+// leaves lane l holding channel 16h + l%16 of pixels 4(l/16)..+3 in FP32.
+// Each result is one bf16 value times 1.0 plus exact zeros, so it is the
+// stored value itself.  Two such results (pixels 0-15 and 16-31 of a chunk)
+// give a lane the 8 pixels of one channel that the reduction MFMA takes; the
+// k index of that MFMA names pixel 4(l/16)+j (j < 4) or 16+4(l/16)+j-4, the
+// same order for both operands, so the sum covers each pixel once.
+//
+// Block: 4 waves over one tile set (16*MB output channels x 32*NB columns of
+// k) and one range of 32-pixel chunks (blockIdx.z), the waves taking every
+// fourth chunk; the waves' FP32 partials are summed in LDS and added to dW
+// (and dbias) atomically, once per block.
+__device__ __forceinline__ bf16x8 selector(int lane, int h)
+{
+    bf16x8 sel;
+    const int g = lane >> 4, n = lane & 15;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sel[j] = (bf16)((8 * g + j == n + 16 * h) ? 1.0f : 0.0f);
+    return sel;
+}
 
+__device__ __forceinline__ bf16x8 pack8(const f32x4 a, const f32x4 b)
+{
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        v[e] = (bf16)a[e];
+        v[4 + e] = (bf16)b[e];
+    }
+    return v;
+}
+
+// Blocks per launch the pixel split aims at, and whether a wave loads its
+// next chunk before computing the current one.
+constexpr int kWgBlocks = 512;
+constexpr bool kWgPrefetch = false;
+
+template <int MB, int NB>
 __global__ __launch_bounds__(256) void k_conv_bwd_weight(const bf16* __restrict__ x,
                                                          const bf16* __restrict__ dy,
                                                          float* __restrict__ dw, float* __restrict__ db,
-                                                         const ConvShape s, const int pix_per_block)
+                                                         const ConvShape s, const int chunks_per_block)
 {
-    constexpr int PS = kWgStage;
-    __shared__ __attribute__((aligned(16))) bf16 sA[64][PS + 8];  // [co][p]  (+ pad)
-    __shared__ __attribute__((aligned(16))) bf16 sB[64][PS + 8];  // [k][p]
+    constexpr int MJ = (MB + 1) / 2;          // 32-channel loads of dY per half chunk
+    constexpr int NT = 2 * NB;                // 16-column tiles of k
+    __shared__ f32x4 red[3][MB][NT][64];
+    __shared__ float bred[4][MB][16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave & 1, wcl = wave >> 1;
-    const int co0 = blockIdx.x * 64, kk0 = blockIdx.y * 64;
+    const int g = lane >> 4, r = lane & 15;
+    const int co0 = blockIdx.x * 16 * MB, kc0 = blockIdx.y * 32 * NB;
     const int K = 9 * s.Cin;
-    const long long P = (long long)s.N * s.Ho * s.Wo;
-    const long long pbeg = (long long)blockIdx.z * pix_per_block;
-    const long long pend = (pbeg + pix_per_block < P) ? pbeg + pix_per_block : P;
-    const int r = lane & 15, kg = lane >> 4;
-    f32x4 acc[2][2];
+    const unsigned P = (unsigned)s.N * (unsigned)s.Ho * (unsigned)s.Wo;
+    const unsigned nchunks = (P + 31) / 32;
+    const unsigned cbeg = blockIdx.z * (unsigned)chunks_per_block;
+    const unsigned cend = cbeg + chunks_per_block < nchunks ? cbeg + chunks_per_block : nchunks;
+    // this lane's 8 columns of each 32-column block: tap and first channel
+    int ky[NB], kx[NB], kci[NB];
+    bool kv[NB];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    float bsum = 0.f;
-    for (long long ps = pbeg; ps < pend; ps += PS) {
-        // staging: PS pixels x 8 groups of 8 rows per operand, PS*8/256 items per thread
-#pragma unroll
-        for (int m = 0; m < PS * 8 / 256; ++m) {
-            const int item = threadIdx.x + 256 * m;
-            const int sp = item >> 3;           // pixel within the stage
-            const int sg = item & 7;            // group of 8 rows
-            const long long p = ps + sp;
-            const bool pv = p < pend;
-            int n = 0, yo = 0, xo = 0;
-            if (pv) {
-                xo = (int)(p % s.Wo);
-                const long long t = p / s.Wo;
-                yo = (int)(t % s.Ho);
-                n = (int)(t / s.Ho);
-            }
-            // dY slice -> sA[co][p]
-            {
-                const int co = co0 + 8 * sg;
-                const bf16x8 v = (pv && co < s.Cout) ? load8(dy + p * s.Cout + co) : zero8();
-#pragma unroll
-                for (int e = 0; e < 8; ++e) sA[8 * sg + e][sp] = v[e];
-            }
-            // im2col(X) slice -> sB[k][p]
-            {
-                const int k = kk0 + 8 * sg;
-                bf16x8 v = zero8();
-                if (pv && k < K) {
-                    const int tap = k / s.Cin, ci = k % s.Cin;
-                    const int yy = yo * s.stride + tap / 3 - 1, xx = xo * s.stride + tap % 3 - 1;
-                    if (yy >= 0 && yy < s.H && xx >= 0 && xx < s.W)
-                        v = load8(x + (((long long)n * s.H + yy) * s.W + xx) * s.Cin + ci);
-                }
-#pragma unroll
-                for (int e = 0; e < 8; ++e) sB[8 * sg + e][sp] = v[e];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < PS / 32; ++kk) {
-            bf16x8 a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = *(const bf16x8*)&sA[32 * wr + 16 * i + r][32 * kk + 8 * kg];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = *(const bf16x8*)&sB[32 * wcl + 16 * j + r][32 * kk + 8 * kg];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-        }
-        if (db && blockIdx.y == 0 && threadIdx.x < 64) {
-#pragma unroll 8
-            for (int q = 0; q < PS; ++q) bsum += (float)sA[threadIdx.x][q];
-        }
-        __syncthreads();
+    for (int b = 0; b < NB; ++b) {
+        const int kk = kc0 + 32 * b + 8 * g;
+        kv[b] = kk < K;
+        const int tap = kv[b] ? kk / s.Cin : 0;
+        kci[b] = kv[b] ? kk % s.Cin : 0;
+        ky[b] = tap / 3;
+        kx[b] = tap % 3;
     }
+    const bf16x8 sel0 = selector(lane, 0), sel1 = selector(lane, 1);
+    const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc[MB][NT];
+    float bsum[MB];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int m = 0; m < MB; ++m) {
+        bsum[m] = 0.f;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int k = kk0 + 32 * wcl + 16 * j + r;
-            if (k >= K) continue;
+        for (int t = 0; t < NT; ++t) acc[m][t] = z4;
+    }
+    // operands of one chunk: dY (32 channels per load) and X columns, for
+    // pixels 0-15 (half 0) and 16-31 (half 1) of the chunk
+    auto load = [&](unsigned c, bf16x8 (&yv)[MJ][2], bf16x8 (&xv)[NB][2]) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int co = co0 + 32 * wr + 16 * i + 4 * kg + e;
-                if (co < s.Cout) atomicAdd(&dw[(long long)co * K + k], acc[i][j][e]);
+        for (int half = 0; half < 2; ++half) {
+            const unsigned p = 32 * c + 16 * half + r;
+            const bool pv = p < P;
+            const unsigned pp = pv ? p : 0u;
+            const unsigned xo = pp % (unsigned)s.Wo, t = pp / (unsigned)s.Wo;
+            const unsigned yo = t % (unsigned)s.Ho, n = t / (unsigned)s.Ho;
+#pragma unroll
+            for (int j = 0; j < MJ; ++j) {
+                const int co = co0 + 32 * j + 8 * g;
+                yv[j][half] = (pv && co < s.Cout) ? load8(dy + (long long)pp * s.Cout + co) : zero8();
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int yy = (int)yo * s.stride + ky[b] - 1, xx = (int)xo * s.stride + kx[b] - 1;
+                const bool ok = pv && kv[b] && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W;
+                xv[b][half] = ok ? load8(x + (((long long)n * s.H + yy) * s.W + xx) * s.Cin + kci[b]) : zero8();
             }
         }
-    if (db && blockIdx.y == 0 && threadIdx.x < 64 && co0 + (int)threadIdx.x < s.Cout)
-        atomicAdd(&db[co0 + threadIdx.x], bsum);
+    };
+    auto compute = [&](const bf16x8 (&yv)[MJ][2], const bf16x8 (&xv)[NB][2]) {
+        bf16x8 opy[MB], opx[NT];
+#pragma unroll
+        for (int m = 0; m < MB; ++m) {
+            const bf16x8 sl = (m & 1) ? sel1 : sel0;
+            const f32x4 t0 = mfma(yv[m / 2][0], sl, z4), t1 = mfma(yv[m / 2][1], sl, z4);
+            opy[m] = pack8(t0, t1);
+            bsum[m] += ((t0[0] + t0[1]) + (t0[2] + t0[3])) + ((t1[0] + t1[1]) + (t1[2] + t1[3]));
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bf16x8 sl = (t & 1) ? sel1 : sel0;
+            opx[t] = pack8(mfma(xv[t / 2][0], sl, z4), mfma(xv[t / 2][1], sl, z4));
+        }
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[m][t] = mfma(opy[m], opx[t], acc[m][t]);
+    };
+    if constexpr (kWgPrefetch) {
+        // the next chunk's loads are issued before the current chunk's MFMAs
+        bf16x8 y0[MJ][2], x0[NB][2], y1[MJ][2], x1[NB][2];
+        unsigned c = cbeg + wave;
+        if (c < cend) load(c, y0, x0);
+        for (; c < cend; c += 8) {
+            if (c + 4 < cend) load(c + 4, y1, x1);
+            compute(y0, x0);
+            if (c + 4 >= cend) break;
+            if (c + 8 < cend) load(c + 8, y0, x0);
+            compute(y1, x1);
+        }
+    } else {
+        for (unsigned c = cbeg + wave; c < cend; c += 4) {
+            bf16x8 y0[MJ][2], x0[NB][2];
+            load(c, y0, x0);
+            compute(y0, x0);
+        }
+    }
+    // bias partials: lane l holds channel l%16 of each tile; sum the four
+    // lane groups
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+        float v = bsum[m];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (g == 0) bred[wave][m][r] = v;
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) red[wave - 1][m][t][lane] = acc[m][t];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const f32x4 a = acc[m][t], b1 = red[0][m][t][lane], b2 = red[1][m][t][lane],
+                            b3 = red[2][m][t][lane];
+                const int k = kc0 + 16 * t + r;
+                if (k >= K) continue;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int co = co0 + 16 * m + 4 * g + e;
+                    if (co < s.Cout) atomicAdd(&dw[(long long)co * K + k], (a[e] + b1[e]) + (b2[e] + b3[e]));
+                }
+            }
+    }
+    if (db && blockIdx.y == 0 && threadIdx.x < 16 * MB) {
+        const int m = threadIdx.x / 16, ch = threadIdx.x % 16;
+        const int co = co0 + threadIdx.x;
+        if (co < s.Cout)
+            atomicAdd(&db[co], (bred[0][m][ch] + bred[1][m][ch]) + (bred[2][m][ch] + bred[3][m][ch]));
+    }
 }
 
 // ------------------------------------------------------------ elementwise --
@@ -388,6 +544,92 @@ __global__ void k_upsample_bwd(const bf16* __restrict__ dy, bf16* __restrict__ d
     dx[i] = (bf16)v;
 }
 
+// The same three element-wise kernels over 8 channels per thread (16-byte
+// loads and stores), for C % 8 == 0 and 16-byte aligned tensors: eight
+// times fewer memory instructions for the same bytes.
+__device__ __forceinline__ void store8(bf16* p, const bf16x8 v) { *(bf16x8*)p = v; }
+
+__global__ void k_upsample_add8(const bf16* __restrict__ x, const bf16* __restrict__ skip, bf16* __restrict__ y,
+                                int N, int H, int W, int C)
+{
+    const int C8 = C / 8;
+    const long long n = (long long)N * 4 * H * W * C8;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C8) * 8;
+    long long t = i / C8;
+    const int xo = (int)(t % (2 * W));
+    t /= 2 * W;
+    const int yo = (int)(t % (2 * H));
+    const int nn = (int)(t / (2 * H));
+    const bf16x8 a = load8(x + (((long long)nn * H + yo / 2) * W + xo / 2) * C + c);
+    bf16x8 v = a;
+    if (skip) {
+        const bf16x8 b = load8(skip + i * 8);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (bf16)((float)a[e] + (float)b[e]);
+    }
+    store8(y + i * 8, v);
+}
+
+__global__ void k_upsample_bwd8(const bf16* __restrict__ dy, bf16* __restrict__ dx, const bf16* __restrict__ mask,
+                                int N, int H, int W, int C, int accumulate)
+{
+    const int C8 = C / 8;
+    const long long n = (long long)N * H * W * C8;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C8) * 8;
+    long long t = i / C8;
+    const int xx = (int)(t % W);
+    t /= W;
+    const int yy = (int)(t % H);
+    const int nn = (int)(t / H);
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = 0.f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const bf16x8 d = load8(dy + (((long long)nn * 2 * H + 2 * yy + a) * 2 * W + 2 * xx + b) * C + c);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] += (float)d[e];
+        }
+    bf16x8 m = zero8(), o = zero8();
+    if (mask) m = load8(mask + i * 8);
+    if (accumulate) o = load8(dx + i * 8);
+    bf16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        float u = v[e];
+        if (mask && !((float)m[e] > 0.f)) u = 0.f;
+        if (accumulate) u += (float)o[e];
+        r[e] = (bf16)u;
+    }
+    store8(dx + i * 8, r);
+}
+
+__global__ void k_masked_add8(const bf16* __restrict__ src, bf16* __restrict__ dst, const bf16* __restrict__ mask,
+                              long long n8, int accumulate)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const bf16x8 a = load8(src + i * 8);
+    bf16x8 m = zero8(), o = zero8();
+    if (mask) m = load8(mask + i * 8);
+    if (accumulate) o = load8(dst + i * 8);
+    bf16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        float v = (float)a[e];
+        if (mask && !((float)m[e] > 0.f)) v = 0.f;
+        if (accumulate) v += (float)o[e];
+        r[e] = (bf16)v;
+    }
+    store8(dst + i * 8, r);
+}
+
 // dst = (accumulate ? dst : 0) + src * (mask > 0)
 __global__ void k_masked_add(const bf16* __restrict__ src, bf16* __restrict__ dst, const bf16* __restrict__ mask,
                              long long n, int accumulate)
@@ -445,6 +687,53 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
     w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// Adam over every parameter tensor in one launch: the tensors lie in flat
+// FP32 buffers (parameters, gradients, both moments) as segments; a thread
+// finds its segment by binary search over the table, updates its element and,
+// for a weight segment, writes the bf16 working copy [Cout][9][Cin] and its
+// transpose [9][Cin][Cout] the convolutions read.
+constexpr int kMaxSegments = 64;
+struct SegTable {
+    long long off[kMaxSegments + 1];   // first element of each segment, then the total
+    int cout[kMaxSegments], cin[kMaxSegments];
+    float wd[kMaxSegments];
+    bf16* w[kMaxSegments];
+    bf16* wt[kMaxSegments];
+    int n;
+};
+
+__global__ void k_adam_multi(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                             float* __restrict__ v, const SegTable tab, float lr, float b1, float b2,
+                             float eps, float c1, float c2)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tab.off[tab.n]) return;
+    int lo = 0, hi = tab.n - 1;            // last segment whose start is <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab.off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const float wi = w[i];
+    const float gi = g[i] + tab.wd[lo] * wi;
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wn = wi - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wn;
+    if (tab.w[lo]) {
+        const long long e = i - tab.off[lo];
+        const int Cin = tab.cin[lo], Cout = tab.cout[lo];
+        const int ci = (int)(e % Cin);
+        const long long t = e / Cin;
+        const int tap = (int)(t % 9), co = (int)(t / 9);
+        const bf16 b = (bf16)wn;
+        tab.w[lo][e] = b;
+        if (tab.wt[lo]) tab.wt[lo][((long long)tap * Cin + ci) * Cout + co] = b;
+    }
+}
+
 // Network input: per pixel [solid, log-Re feature, x, y, 1, 0, 0, 0] (bf16)
 __global__ void k_make_input(const unsigned char* __restrict__ solid, const float* __restrict__ re_feat,
                              bf16* __restrict__ out, int N, int H, int W)
@@ -483,16 +772,24 @@ int mfa_nn_conv3x3(const void* x, const void* w, const float* bias, void* y, flo
         return nn_fail(-1, "mfa_nn_conv3x3: channels must be multiples of 8, stride 1 or 2");
     ConvShape s{N, H, W, Cin, Cout, stride, (H + stride - 1) / stride, (W + stride - 1) / stride};
     const long long P = (long long)N * s.Ho * s.Wo;
-    // block tiles (pixels x channels): 256 x 16, 128 x 32, 64 x 64
+    // block tiles (pixels x channels): 256 x 16, 128 x 32, 64 x 64; layers
+    // whose 64 x 64 tiles number fewer than 512 (the deep ones on small
+    // maps) take 32 x 32 tiles with the four waves splitting k
+    hipStream_t st = (hipStream_t)stream;
+    const bf16* xb = (const bf16*)x;
+    const bf16* wb = (const bf16*)w;
     if (Cout <= 16) {
         dim3 grid((unsigned)((P + 255) / 256), (Cout + 15) / 16);
-        hipLaunchKernelGGL((k_conv_fwd<1, 4, 1>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)w, bias, (bf16*)y, y_f32, s, relu);
+        hipLaunchKernelGGL((k_conv_fwd<1, 4, 1, 1>), grid, dim3(256), 0, st, xb, wb, bias, (bf16*)y, y_f32, s, relu);
     } else if (Cout <= 32) {
         dim3 grid((unsigned)((P + 127) / 128), (Cout + 31) / 32);
-        hipLaunchKernelGGL((k_conv_fwd<1, 2, 2>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)w, bias, (bf16*)y, y_f32, s, relu);
-    } else {
+        hipLaunchKernelGGL((k_conv_fwd<1, 2, 2, 1>), grid, dim3(256), 0, st, xb, wb, bias, (bf16*)y, y_f32, s, relu);
+    } else if (((P + 63) / 64) * ((Cout + 63) / 64) >= 512) {
         dim3 grid((unsigned)((P + 63) / 64), (Cout + 63) / 64);
-        hipLaunchKernelGGL((k_conv_fwd<2, 2, 2>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)w, bias, (bf16*)y, y_f32, s, relu);
+        hipLaunchKernelGGL((k_conv_fwd<2, 2, 2, 1>), grid, dim3(256), 0, st, xb, wb, bias, (bf16*)y, y_f32, s, relu);
+    } else {
+        dim3 grid((unsigned)((P + 31) / 32), (Cout + 31) / 32);
+        hipLaunchKernelGGL((k_conv_fwd<1, 2, 2, 4>), grid, dim3(256), 0, st, xb, wb, bias, (bf16*)y, y_f32, s, relu);
     }
     NN_HIP(hipGetLastError());
     return 0;
@@ -506,15 +803,22 @@ int mfa_nn_conv3x3_bwd_data(const void* dy, const void* wt, void* dx, const void
         return nn_fail(-1, "mfa_nn_conv3x3_bwd_data: channels must be multiples of 8, stride 1 or 2");
     ConvShape s{N, H, W, Cin, Cout, stride, (H + stride - 1) / stride, (W + stride - 1) / stride};
     const long long P = (long long)N * H * W;
+    hipStream_t st = (hipStream_t)stream;
+    const bf16* dyb = (const bf16*)dy;
+    const bf16* wtb = (const bf16*)wt;
+    const bf16* mb = (const bf16*)mask;
     if (Cin <= 16) {
         dim3 grid((unsigned)((P + 255) / 256), (Cin + 15) / 16);
-        hipLaunchKernelGGL((k_conv_bwd_data<1, 4, 1>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (const bf16*)wt, (bf16*)dx, (const bf16*)mask, s, accumulate);
+        hipLaunchKernelGGL((k_conv_bwd_data<1, 4, 1, 1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
     } else if (Cin <= 32) {
         dim3 grid((unsigned)((P + 127) / 128), (Cin + 31) / 32);
-        hipLaunchKernelGGL((k_conv_bwd_data<1, 2, 2>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (const bf16*)wt, (bf16*)dx, (const bf16*)mask, s, accumulate);
-    } else {
+        hipLaunchKernelGGL((k_conv_bwd_data<1, 2, 2, 1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
+    } else if (((P + 63) / 64) * ((Cin + 63) / 64) >= 512) {
         dim3 grid((unsigned)((P + 63) / 64), (Cin + 63) / 64);
-        hipLaunchKernelGGL((k_conv_bwd_data<2, 2, 2>), grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (const bf16*)wt, (bf16*)dx, (const bf16*)mask, s, accumulate);
+        hipLaunchKernelGGL((k_conv_bwd_data<2, 2, 2, 1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
+    } else {
+        dim3 grid((unsigned)((P + 31) / 32), (Cin + 31) / 32);
+        hipLaunchKernelGGL((k_conv_bwd_data<1, 2, 2, 4>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
     }
     NN_HIP(hipGetLastError());
     return 0;
@@ -528,14 +832,31 @@ int mfa_nn_conv3x3_bwd_weight(const void* x, const void* dy, float* dw, float* d
         return nn_fail(-1, "mfa_nn_conv3x3_bwd_weight: channels must be multiples of 8, stride 1 or 2");
     ConvShape s{N, H, W, Cin, Cout, stride, (H + stride - 1) / stride, (W + stride - 1) / stride};
     const long long P = (long long)N * s.Ho * s.Wo;
-    const int tiles = ((Cout + 63) / 64) * ((9 * Cin + 63) / 64);
-    // enough pixel slices to fill the chip (~2048 blocks), whole LDS stages
-    long long per = (P * tiles + 2047) / 2048;
-    per = ((per + kWgStage - 1) / kWgStage) * kWgStage;
-    if (per < 2 * kWgStage) per = 2 * kWgStage;
-    const int nz = (int)((P + per - 1) / per);
-    dim3 grid((Cout + 63) / 64, (9 * Cin + 63) / 64, nz);
-    hipLaunchKernelGGL(k_conv_bwd_weight, grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
+    if (P > 0x7fffffffLL) return nn_fail(-1, "mfa_nn_conv3x3_bwd_weight: more than 2^31 output pixels");
+    const int K = 9 * Cin;
+    // tiles: 16 or 32 output channels (MB 1 or 2) x 32 NB columns of k.
+    // Thin layers (K <= 160) take all their columns in one block; wider ones
+    // 96-column blocks (K = 288, 576, 864 split evenly).
+    const int MB = Cout <= 16 ? 1 : 2;
+    const int NB = K <= 96 ? 3 : (K <= 160 ? 5 : 3);
+    const int gx = (Cout + 16 * MB - 1) / (16 * MB), gy = (K + 32 * NB - 1) / (32 * NB);
+    // about kWgBlocks blocks over the pixel chunks
+    const long long nchunks = (P + 31) / 32;
+    long long nz = kWgBlocks / (gx * gy);
+    if (nz < 1) nz = 1;
+    if (nz > nchunks) nz = nchunks;
+    long long per = (nchunks + nz - 1) / nz;
+    nz = (nchunks + per - 1) / per;
+    dim3 grid(gx, gy, (unsigned)nz);
+    hipStream_t st = (hipStream_t)stream;
+    if (MB == 1 && NB == 3)
+        hipLaunchKernelGGL((k_conv_bwd_weight<1, 3>), grid, dim3(256), 0, st, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
+    else if (MB == 1 && NB == 5)
+        hipLaunchKernelGGL((k_conv_bwd_weight<1, 5>), grid, dim3(256), 0, st, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
+    else if (MB == 2 && NB == 3)
+        hipLaunchKernelGGL((k_conv_bwd_weight<2, 3>), grid, dim3(256), 0, st, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
+    else
+        hipLaunchKernelGGL((k_conv_bwd_weight<2, 5>), grid, dim3(256), 0, st, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
     NN_HIP(hipGetLastError());
     return 0;
 }
@@ -557,10 +878,21 @@ int mfa_nn_prep_weights(const float* w32, void* w, void* wt, int Cout, int Cin, 
     return 0;
 }
 
+static bool vec8(int C, std::initializer_list<const void*> ptrs)
+{
+    if (C % 8) return false;
+    for (const void* p : ptrs)
+        if ((uintptr_t)p % 16) return false;
+    return true;
+}
+
 int mfa_nn_upsample_add(const void* x, const void* skip, void* y, int N, int H, int W, int C, void* stream)
 {
     if (!x || !y) return nn_fail(-1, "mfa_nn_upsample_add: null pointer");
-    hipLaunchKernelGGL(k_upsample_add, g1((long long)N * 4 * H * W * C), dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)skip, (bf16*)y, N, H, W, C);
+    if (vec8(C, {x, skip, y}))
+        hipLaunchKernelGGL(k_upsample_add8, g1((long long)N * 4 * H * W * (C / 8)), dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)skip, (bf16*)y, N, H, W, C);
+    else
+        hipLaunchKernelGGL(k_upsample_add, g1((long long)N * 4 * H * W * C), dim3(256), 0, (hipStream_t)stream, (const bf16*)x, (const bf16*)skip, (bf16*)y, N, H, W, C);
     NN_HIP(hipGetLastError());
     return 0;
 }
@@ -569,7 +901,10 @@ int mfa_nn_upsample_bwd(const void* dy, void* dx, const void* mask, int N, int H
                         int accumulate, void* stream)
 {
     if (!dy || !dx) return nn_fail(-1, "mfa_nn_upsample_bwd: null pointer");
-    hipLaunchKernelGGL(k_upsample_bwd, g1((long long)N * H * W * C), dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (bf16*)dx, (const bf16*)mask, N, H, W, C, accumulate);
+    if (vec8(C, {dy, dx, mask}))
+        hipLaunchKernelGGL(k_upsample_bwd8, g1((long long)N * H * W * (C / 8)), dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (bf16*)dx, (const bf16*)mask, N, H, W, C, accumulate);
+    else
+        hipLaunchKernelGGL(k_upsample_bwd, g1((long long)N * H * W * C), dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, (bf16*)dx, (const bf16*)mask, N, H, W, C, accumulate);
     NN_HIP(hipGetLastError());
     return 0;
 }
@@ -578,7 +913,10 @@ int mfa_nn_masked_add(const void* src, void* dst, const void* mask, long long n,
                       void* stream)
 {
     if (!src || !dst) return nn_fail(-1, "mfa_nn_masked_add: null pointer");
-    hipLaunchKernelGGL(k_masked_add, g1(n), dim3(256), 0, (hipStream_t)stream, (const bf16*)src, (bf16*)dst, (const bf16*)mask, n, accumulate);
+    if (n % 8 == 0 && vec8(8, {src, dst, mask}))
+        hipLaunchKernelGGL(k_masked_add8, g1(n / 8), dim3(256), 0, (hipStream_t)stream, (const bf16*)src, (bf16*)dst, (const bf16*)mask, n / 8, accumulate);
+    else
+        hipLaunchKernelGGL(k_masked_add, g1(n), dim3(256), 0, (hipStream_t)stream, (const bf16*)src, (bf16*)dst, (const bf16*)mask, n, accumulate);
     NN_HIP(hipGetLastError());
     return 0;
 }
@@ -598,6 +936,37 @@ int mfa_nn_adam(float* w, const float* g, float* m, float* v, long long n, float
     if (!w || !g || !m || !v || t < 1) return nn_fail(-1, "mfa_nn_adam: bad argument");
     const float c1 = 1.f - powf(beta1, (float)t), c2 = 1.f - powf(beta2, (float)t);
     hipLaunchKernelGGL(k_adam, g1(n), dim3(256), 0, (hipStream_t)stream, w, g, m, v, n, lr, beta1, beta2, eps, c1, c2, weight_decay);
+    NN_HIP(hipGetLastError());
+    return 0;
+}
+
+int mfa_nn_adam_multi(float* w, const float* g, float* m, float* v, const mfa_nn_segment* seg,
+                      int nseg, float lr, float beta1, float beta2, float eps, int t, void* stream)
+{
+    if (!w || !g || !m || !v || !seg || t < 1) return nn_fail(-1, "mfa_nn_adam_multi: bad argument");
+    if (nseg < 1 || nseg > kMaxSegments)
+        return nn_fail(-1, "mfa_nn_adam_multi: 1 to " + std::to_string(kMaxSegments) + " segments");
+    SegTable tab;
+    tab.n = nseg;
+    long long end = 0;
+    for (int k = 0; k < nseg; ++k) {
+        const mfa_nn_segment& sg = seg[k];
+        if (sg.offset != end || sg.n < 1)
+            return nn_fail(-1, "mfa_nn_adam_multi: segments must be non-empty and follow each other");
+        if (sg.w_bf16 && (sg.cout < 1 || sg.cin < 1 || (long long)sg.cout * 9 * sg.cin != sg.n))
+            return nn_fail(-1, "mfa_nn_adam_multi: a weight segment holds cout * 9 * cin elements");
+        tab.off[k] = sg.offset;
+        tab.cout[k] = sg.cout;
+        tab.cin[k] = sg.cin;
+        tab.wd[k] = sg.weight_decay;
+        tab.w[k] = (bf16*)sg.w_bf16;
+        tab.wt[k] = sg.w_bf16 ? (bf16*)sg.wt_bf16 : nullptr;
+        end += sg.n;
+    }
+    tab.off[nseg] = end;
+    const float c1 = 1.f - powf(beta1, (float)t), c2 = 1.f - powf(beta2, (float)t);
+    hipLaunchKernelGGL(k_adam_multi, g1(end), dim3(256), 0, (hipStream_t)stream, w, g, m, v, tab, lr, beta1,
+                       beta2, eps, c1, c2);
     NN_HIP(hipGetLastError());
     return 0;
 }

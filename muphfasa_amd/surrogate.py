@@ -59,6 +59,13 @@ RELU_OUT = {name for name, _, _, _, relu, _ in LAYERS if relu}
 _lib = None
 
 
+class Segment(ctypes.Structure):
+    """mfa_nn_segment: one tensor of the flat parameter buffers."""
+    _fields_ = [("offset", ctypes.c_longlong), ("n", ctypes.c_longlong), ("cout", ctypes.c_int),
+                ("cin", ctypes.c_int), ("weight_decay", ctypes.c_float), ("w_bf16", ctypes.c_void_p),
+                ("wt_bf16", ctypes.c_void_p)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -79,10 +86,11 @@ def lib():
         L.mfa_nn_mse.argtypes = [vp, vp, vp, vp, vp, ll, i, i, f, vp]
         L.mfa_nn_adam.argtypes = [vp, vp, vp, vp, ll, f, f, f, f, i, f, vp]
         L.mfa_nn_make_input.argtypes = [vp, vp, vp, i, i, i, vp]
+        L.mfa_nn_adam_multi.argtypes = [vp, vp, vp, vp, ctypes.POINTER(Segment), i, f, f, f, f, i, vp]
         for n in ("mfa_nn_conv3x3", "mfa_nn_conv3x3_bwd_data", "mfa_nn_conv3x3_bwd_weight",
                   "mfa_nn_fill", "mfa_nn_prep_weights", "mfa_nn_upsample_add",
                   "mfa_nn_upsample_bwd", "mfa_nn_masked_add", "mfa_nn_mse", "mfa_nn_adam",
-                  "mfa_nn_make_input"):
+                  "mfa_nn_make_input", "mfa_nn_adam_multi"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -145,7 +153,12 @@ def masked_add(src, dst, mask, accumulate=False):
 
 class WarmStartNet:
     """Parameters (FP32 master + BF16 working copies) and the hand-scheduled
-    forward / backward passes over the MFMA kernels."""
+    forward / backward passes over the MFMA kernels.
+
+    The FP32 parameters, their gradients and Adam's two moments each live in
+    one flat buffer, every layer's weight and bias a view into it, so zeroing
+    the gradients is one launch and the optimiser step (which also writes the
+    BF16 working copies) is another."""
 
     def __init__(self, H=256, W=256, seed=0, dev="cuda:0"):
         import torch
@@ -153,21 +166,32 @@ class WarmStartNet:
         self.dev = torch.device(dev)
         self.H, self.W = H, W
         rng = np.random.default_rng(seed)
+        total = sum(cout * 9 * cin + cout for _, cin, cout, _, _, _ in LAYERS)
+        self.flat_w = torch.zeros(total, dtype=torch.float32, device=self.dev)
+        self.flat_g = torch.zeros(total, dtype=torch.float32, device=self.dev)
         self.layers = {}
+        off = 0
         for name, cin, cout, stride, relu, src in LAYERS:
             std = math.sqrt(2.0 / (9 * cin)) if relu else math.sqrt(1.0 / (9 * cin))
             w = rng.standard_normal((cout, 9 * cin)) * std
             if name == "e1":
                 w.reshape(cout, 9, cin)[:, :, 5:] = 0.0     # padded input channels
-            self.layers[name] = {
+            nw = cout * 9 * cin
+            l = self.layers[name] = {
                 "cin": cin, "cout": cout, "stride": stride, "relu": relu, "src": src,
-                "w32": torch.tensor(w, dtype=torch.float32, device=self.dev),
-                "b32": torch.zeros(cout, dtype=torch.float32, device=self.dev),
+                "off": off,
+                "w32": self.flat_w[off:off + nw].view(cout, 9 * cin),
+                "gw": self.flat_g[off:off + nw].view(cout, 9 * cin),
+                "b32": self.flat_w[off + nw:off + nw + cout],
+                "gb": self.flat_g[off + nw:off + nw + cout],
                 "w": torch.empty((cout, 3, 3, cin), dtype=torch.bfloat16, device=self.dev),
                 "wt": torch.empty((3, 3, cin, cout), dtype=torch.bfloat16, device=self.dev),
             }
-        self.nparams = sum(l["w32"].numel() + l["b32"].numel() for l in self.layers.values())
+            l["w32"].copy_(torch.tensor(w, dtype=torch.float32))
+            off += nw + cout
+        self.nparams = total
         self._adam = None
+        self._segs = {}
         self.t = 0
         self.prep()
 
@@ -233,13 +257,7 @@ class WarmStartNet:
 
     # -- backward ----------------------------------------------------------
     def zero_grad(self):
-        torch = self.torch
-        for l in self.layers.values():
-            if "gw" not in l:
-                l["gw"] = torch.zeros_like(l["w32"])
-                l["gb"] = torch.zeros_like(l["b32"])
-            fill(l["gw"])
-            fill(l["gb"])
+        fill(self.flat_g)
 
     def backward(self, g_out):
         """g_out: (N,H,W,8) bf16 gradient of the loss w.r.t. the head output."""
@@ -269,20 +287,31 @@ class WarmStartNet:
                     grad[b] = torch.empty_like(act[b])
                 masked_add(gu, grad[b], act[b], accumulate=accb)
 
+    def _segments(self, wd):
+        """The flat buffers' segment table for mfa_nn_adam_multi: per layer
+        its weight (decayed by wd; BF16 copy and transpose written) and its
+        bias (not decayed)."""
+        if wd not in self._segs:
+            segs = []
+            for name, cin, cout, _, _, _ in LAYERS:
+                l = self.layers[name]
+                nw = cout * 9 * cin
+                segs.append(Segment(l["off"], nw, cout, cin, wd, l["w"].data_ptr(), l["wt"].data_ptr()))
+                segs.append(Segment(l["off"] + nw, cout, 0, 0, 0.0, None, None))
+            self._segs[wd] = (Segment * len(segs))(*segs)
+        return self._segs[wd]
+
     def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
+        """One Adam step of every parameter (one launch), which also writes
+        the BF16 working copies the convolutions read."""
         torch = self.torch
         if self._adam is None:
-            self._adam = {k: (torch.zeros_like(l["w32"]), torch.zeros_like(l["w32"]),
-                              torch.zeros_like(l["b32"]), torch.zeros_like(l["b32"]))
-                          for k, l in self.layers.items()}
+            self._adam = (torch.zeros_like(self.flat_w), torch.zeros_like(self.flat_w))
         self.t += 1
-        for k, l in self.layers.items():
-            mw, vw, mb, vb = self._adam[k]
-            _ck(lib().mfa_nn_adam(_p(l["w32"]), _p(l["gw"]), _p(mw), _p(vw), l["w32"].numel(), lr,
-                                  beta1, beta2, eps, self.t, wd, _s()))
-            _ck(lib().mfa_nn_adam(_p(l["b32"]), _p(l["gb"]), _p(mb), _p(vb), l["b32"].numel(), lr,
-                                  beta1, beta2, eps, self.t, 0.0, _s()))
-        self.prep()
+        segs = self._segments(float(wd))
+        m, v = self._adam
+        _ck(lib().mfa_nn_adam_multi(_p(self.flat_w), _p(self.flat_g), _p(m), _p(v), segs, len(segs),
+                                    lr, beta1, beta2, eps, self.t, _s()))
 
 
 # ---- data -------------------------------------------------------------------

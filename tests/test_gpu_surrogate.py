@@ -31,7 +31,13 @@ SHAPES = [(2, 9, 7, 8, 32, 1), (1, 16, 16, 32, 64, 2), (2, 8, 12, 64, 32, 1),
           (1, 10, 10, 128, 128, 2), (1, 6, 6, 32, 8, 1), (3, 5, 11, 16, 24, 1),
           # the warm-start network's deep layers on small maps
           (2, 4, 4, 96, 120, 2), (2, 2, 2, 120, 120, 2), (2, 1, 1, 120, 120, 1),
-          (2, 2, 2, 120, 96, 1), (1, 3, 5, 120, 120, 2), (2, 16, 16, 16, 8, 1)]
+          (2, 2, 2, 120, 96, 1), (1, 3, 5, 120, 120, 2), (2, 16, 16, 16, 8, 1),
+          # thin layers over many pixels (the weight gradient's long reductions)
+          # and sizes that leave a partial 32-pixel chunk
+          (1, 64, 64, 16, 16, 1), (2, 33, 31, 8, 16, 1), (1, 37, 29, 16, 32, 2),
+          # enough 64 x 64 output tiles for the forward and data gradient to
+          # run without splitting k
+          (8, 64, 64, 32, 64, 1)]
 
 
 @pytest.mark.parametrize("N,H,W,Cin,Cout,stride", SHAPES)
@@ -165,3 +171,97 @@ def test_training_reduces_loss(dev):
         losses.append(part.sum().item() / count)
     print(losses[0], losses[-1])
     assert losses[-1] < 0.1 * losses[0]
+
+
+def test_fused_adam_equals_per_tensor_adam_and_weight_prep(dev):
+    """mfa_nn_adam_multi over the network's flat buffers gives, bit for bit,
+    what mfa_nn_adam on every weight and bias followed by
+    mfa_nn_prep_weights gives: the FP32 parameters, both moments and the
+    BF16 weights and their transpose, over three steps with weight decay."""
+    import torch
+    net = S.WarmStartNet(H=16, W=16, seed=3, dev=dev)
+    rng = np.random.default_rng(5)
+    ref = {k: {"w": l["w32"].clone(), "b": l["b32"].clone(),
+               "m": [torch.zeros_like(l["w32"]), torch.zeros_like(l["b32"])],
+               "v": [torch.zeros_like(l["w32"]), torch.zeros_like(l["b32"])],
+               "wb": torch.empty_like(l["w"]), "wtb": torch.empty_like(l["wt"])}
+           for k, l in net.layers.items()}
+    L = S.lib()
+    for t in range(1, 4):
+        g = torch.from_numpy(rng.standard_normal(net.flat_g.numel()).astype(np.float32)).to(dev)
+        net.flat_g.copy_(g)
+        net.adam_step(1e-3, wd=1e-4)
+        for k, l in net.layers.items():
+            r = ref[k]
+            S._ck(L.mfa_nn_adam(S._p(r["w"]), S._p(l["gw"]), S._p(r["m"][0]), S._p(r["v"][0]),
+                                r["w"].numel(), 1e-3, 0.9, 0.999, 1e-8, t, 1e-4, S._s()))
+            S._ck(L.mfa_nn_adam(S._p(r["b"]), S._p(l["gb"]), S._p(r["m"][1]), S._p(r["v"][1]),
+                                r["b"].numel(), 1e-3, 0.9, 0.999, 1e-8, t, 0.0, S._s()))
+            S._ck(L.mfa_nn_prep_weights(S._p(r["w"]), S._p(r["wb"]), S._p(r["wtb"]), l["cout"],
+                                        l["cin"], S._s()))
+        torch.cuda.synchronize()
+        m, v = net._adam
+        for k, l in net.layers.items():
+            r = ref[k]
+            nw = l["w32"].numel()
+            assert torch.equal(l["w32"], r["w"]) and torch.equal(l["b32"], r["b"]), (t, k)
+            assert torch.equal(m[l["off"]:l["off"] + nw].view_as(r["m"][0]), r["m"][0]), (t, k)
+            assert torch.equal(v[l["off"] + nw:l["off"] + nw + l["cout"]], r["v"][1]), (t, k)
+            assert torch.equal(l["w"], r["wb"]) and torch.equal(l["wt"], r["wtb"]), (t, k)
+
+
+@pytest.mark.parametrize("C", [8, 16, 24])
+def test_elementwise_kernels_vector_and_scalar_paths(dev, C):
+    """upsample_add, upsample_bwd and masked_add take 8 channels per thread
+    on 16-byte aligned tensors and one element per thread otherwise; both
+    paths give the same bits, and match numpy (sums of bf16 values in FP32,
+    rounded to bf16)."""
+    import torch
+    rng = np.random.default_rng(C)
+    N, H, W = 2, 5, 7
+
+    def pair(shape):
+        """an aligned tensor and a misaligned view holding the same values"""
+        a = _bf16(torch, rng.standard_normal(shape), dev)
+        buf = torch.empty(a.numel() + 1, dtype=torch.bfloat16, device=dev)
+        b = buf[1:].view(shape)
+        b.copy_(a)
+        return a, b
+
+    def f(t):
+        return t.float().cpu().numpy()
+
+    x, xm = pair((N, H, W, C))
+    sk, skm = pair((N, 2 * H, 2 * W, C))
+    y = torch.empty((N, 2 * H, 2 * W, C), dtype=torch.bfloat16, device=dev)
+    ym = torch.empty(y.numel() + 1, dtype=torch.bfloat16, device=dev)[1:].view(y.shape)
+    S.upsample_add(x, sk, y)
+    S.upsample_add(xm, skm, ym)
+    torch.cuda.synchronize()
+    up = np.repeat(np.repeat(f(x), 2, axis=1), 2, axis=2)
+    ref = torch.from_numpy((up + f(sk)).astype(np.float32)).to(torch.bfloat16).float().numpy()
+    assert torch.equal(y, ym) and np.array_equal(f(y), ref)
+
+    dy, dym = pair((N, 2 * H, 2 * W, C))
+    mask, maskm = pair((N, H, W, C))
+    dx0, dx0m = pair((N, H, W, C))
+    for acc in (0, 1):
+        dx, dxm = dx0.clone(), dx0m.clone()
+        S.upsample_bwd(dy, dx, mask, accumulate=bool(acc))
+        S.upsample_bwd(dym, dxm, maskm, accumulate=bool(acc))
+        torch.cuda.synchronize()
+        d = f(dy).reshape(N, H, 2, W, 2, C)
+        s4 = (((d[:, :, 0, :, 0] + d[:, :, 0, :, 1]) + d[:, :, 1, :, 0]) + d[:, :, 1, :, 1])
+        s4 = np.where(f(mask) > 0, s4, 0.0) + (f(dx0) if acc else 0.0)
+        ref = torch.from_numpy(s4.astype(np.float32)).to(torch.bfloat16).float().numpy()
+        assert torch.equal(dx, dxm.contiguous()) and np.array_equal(f(dx), ref), acc
+
+    src, srcm = pair((N, H, W, C))
+    for acc in (0, 1):
+        dst, dstm = dx0.clone(), dx0m.clone()
+        S.masked_add(src, dst, mask, accumulate=bool(acc))
+        S.masked_add(srcm, dstm, maskm, accumulate=bool(acc))
+        torch.cuda.synchronize()
+        v = np.where(f(mask) > 0, f(src), 0.0) + (f(dx0) if acc else 0.0)
+        ref = torch.from_numpy(v.astype(np.float32)).to(torch.bfloat16).float().numpy()
+        assert torch.equal(dst, dstm.contiguous()) and np.array_equal(f(dst), ref), acc
