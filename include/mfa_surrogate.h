@@ -7,7 +7,8 @@
  * Tensors are NHWC; activations and gradients bfloat16 unless named f32;
  * channel counts are multiples of 8; weights [Cout][3][3][Cin] (bf16 copy of
  * an FP32 master) and their transpose [3][3][Cin][Cout] for the data
- * gradient.  Padding 1; stride 1 or 2 (output (H+1)/2 x (W+1)/2).
+ * gradient.  Padding 1; stride 1 or 2 (output (H+1)/2 x (W+1)/2).  The
+ * convolutions' tensors must be 16-byte aligned (refused otherwise).
  * Functions return 0 or a negative code (message: mfa_nn_error()).
  */
 #ifndef MFA_SURROGATE_H

@@ -30,6 +30,11 @@ HIP_FLAGS = [
 
 SOURCES = ["mfa_lbm.hip", "mfa_lowprec.hip", "mfa_ensemble.hip", "mfa_surrogate.hip", "mfa_level.hip"]
 
+# The network kernels keep MFMA results in ordinary VGPRs: with AGPRs the
+# compiler copies every result out again before the next VALU use (88 extra
+# instructions per chunk of the weight gradient's loop).
+EXTRA_FLAGS = {"mfa_surrogate.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
 
 def build(verbose: bool = False) -> str:
     """Compile each source to an object in parallel, then link the library."""
@@ -42,7 +47,7 @@ def build(verbose: bool = False) -> str:
 
     def compile_one(src):
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *cflags, "-c", "-o", obj, os.path.join(CSRC, src)]
+        cmd = [HIPCC, *cflags, *EXTRA_FLAGS.get(src, []), "-c", "-o", obj, os.path.join(CSRC, src)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <initializer_list>
 #include <string>
+#include <type_traits>
 
 #include "../../include/mfa_surrogate.h"
 
@@ -35,6 +36,14 @@ static int nn_fail(int code, const std::string& m)
     g_nn_err = m;
     return code;
 }
+// Tensors the kernels read or write 16 bytes at a time.
+static bool aligned16(std::initializer_list<const void*> ptrs)
+{
+    for (const void* p : ptrs)
+        if ((uintptr_t)p % 16) return false;
+    return true;
+}
+
 #define NN_HIP(expr)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \
@@ -50,6 +59,12 @@ __device__ __forceinline__ bf16x8 zero8()
 }
 
 __device__ __forceinline__ bf16x8 load8(const bf16* p) { return *(const bf16x8*)p; }
+
+// 16 bytes at element offset off (32-bit) from a wave-uniform base.
+__device__ __forceinline__ bf16x8 load8_at(const bf16* base, unsigned off)
+{
+    return *(const bf16x8*)((const char*)base + off * 2u);
+}
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
 {
@@ -128,6 +143,55 @@ __device__ __forceinline__ void k_loop(int K, int wk, Load&& load, Mma&& mma)
     }
 }
 
+// Epilogue of one wave's tile (16 TI pixels x 16 TJ channels, FP32, MFMA
+// layout: lane l holds rows 4(l/16)+e, column l%16 of each 16 x 16 tile).
+// The tile goes through the wave's own LDS slab [pixel][channel] and comes
+// back as 8 consecutive channels of one pixel per lane and vector, so the
+// global accesses of the epilogue are 16 bytes per lane instead of 2.
+// emit(pixel in tile, first channel in tile, float[8]) runs once per vector.
+template <int TI, int TJ>
+struct EpiSlab {
+    static constexpr int NP = 16 * TI, NC = 16 * TJ, LD = NC + 4;
+    float v[NP * LD];
+};
+
+template <int TI, int TJ, class Emit>
+__device__ __forceinline__ void tile_out(const f32x4 (&acc)[TI][TJ], EpiSlab<TI, TJ>& slab, int lane,
+                                         Emit&& emit)
+{
+    using S = EpiSlab<TI, TJ>;
+    const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) slab.v[(16 * i + 4 * g + e) * S::LD + 16 * j + r] = acc[i][j][e];
+    // lanes of this wave read what others wrote
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    constexpr int CV = S::NC / 8, NV = S::NP * CV;
+#pragma unroll
+    for (int v = lane; v < NV; v += 64) {
+        const int px = v / CV, c8 = 8 * (v % CV);
+        const f32x4 lo = *(const f32x4*)&slab.v[px * S::LD + c8];
+        const f32x4 hi = *(const f32x4*)&slab.v[px * S::LD + c8 + 4];
+        const float o[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        emit(px, c8, o);
+    }
+}
+
+// Per 8-column group of k (k = tap * C + c): the tap, and the element
+// offset of the group's source from the pixel's own base offset in the
+// gathered tensor.  Built once per block in LDS, so the k loop needs no
+// division by the channel count.
+struct KEntry {
+    int delta;
+    short ky, kx;
+};
+constexpr int kMaxKGroups = 9 * 128 / 8;    // k = 9 * C, C <= 128
+
 template <int WC, int TI, int TJ, int KS>
 __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, const bf16* __restrict__ w,
                                                   const float* __restrict__ bias, bf16* __restrict__ y,
@@ -135,27 +199,43 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, co
 {
     static_assert(KS == 1 || WC == 1, "split k: the waves share one tile");
     constexpr int WP = KS > 1 ? 1 : 4 / WC;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ KEntry tbl[kMaxKGroups];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wp = KS > 1 ? 0 : wave % WP, wc = KS > 1 ? 0 : wave / WP, wk = KS > 1 ? wave : 0;
-    const long long P = (long long)s.N * s.Ho * s.Wo;
-    const long long p0 = (long long)blockIdx.x * (16 * TI * WP) + 16 * TI * wp;
+    const unsigned P = (unsigned)s.N * (unsigned)s.Ho * (unsigned)s.Wo;
+    const unsigned p0 = blockIdx.x * (16 * TI * WP) + 16 * TI * wp;
     const int c0 = blockIdx.y * (16 * TJ * WC) + 16 * TJ * wc;
     const int K = 9 * s.Cin;
     const int r = lane & 15, kg = lane >> 4;
-
-    // pixel coordinates of the A rows this lane feeds
-    int pn[TI], py[TI], px[TI];
+    for (int t = threadIdx.x; t < K / 8; t += 256) {
+        const int tap = (8 * t) / s.Cin, ci = (8 * t) % s.Cin;
+        const int ky = tap / 3, kx = tap % 3;
+        tbl[t] = KEntry{((ky - 1) * s.W + (kx - 1)) * s.Cin + ci, (short)ky, (short)kx};
+    }
+    // the A rows this lane feeds: source base offset and corner (y s - 1, x s - 1)
+    unsigned pb[TI];
+    int ys[TI], xs[TI];
     bool pv[TI];
 #pragma unroll
     for (int i = 0; i < TI; ++i) {
-        const long long p = p0 + 16 * i + r;
+        const unsigned p = p0 + 16 * i + r;
         pv[i] = p < P;
-        const long long pp = pv[i] ? p : 0;
-        px[i] = (int)(pp % s.Wo);
-        const long long t = pp / s.Wo;
-        py[i] = (int)(t % s.Ho);
-        pn[i] = (int)(t / s.Ho);
+        const unsigned pp = pv[i] ? p : 0u, t = pp / (unsigned)s.Wo;
+        const unsigned xo = pp - t * (unsigned)s.Wo, yo = t % (unsigned)s.Ho, n = t / (unsigned)s.Ho;
+        pb[i] = ((n * (unsigned)s.H + yo * s.stride) * (unsigned)s.W + xo * s.stride) * (unsigned)s.Cin;
+        ys[i] = (int)yo * s.stride - 1;
+        xs[i] = (int)xo * s.stride - 1;
     }
+    unsigned wrow[TJ];
+    bool cv[TJ];
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+        const int co = c0 + 16 * j + r;
+        cv[j] = co < s.Cout;
+        wrow[j] = cv[j] ? (unsigned)co * (unsigned)K : 0u;
+    }
+    __syncthreads();
     f32x4 acc[TI][TJ];
 #pragma unroll
     for (int i = 0; i < TI; ++i)
@@ -165,19 +245,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, co
     auto load = [&](int k0, bf16x8 (&a)[TI], bf16x8 (&b)[TJ]) {
         const int k = k0 + 8 * kg;
         const bool kv = k < K;
-        const int tap = kv ? k / s.Cin : 0, ci = kv ? k % s.Cin : 0;
-        const int ky = tap / 3, kx = tap % 3;
+        const KEntry e = tbl[kv ? k / 8 : 0];
 #pragma unroll
         for (int i = 0; i < TI; ++i) {
-            const int yy = py[i] * s.stride + ky - 1, xx = px[i] * s.stride + kx - 1;
-            const bool ok = kv && pv[i] && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W;
-            a[i] = ok ? load8(x + (((long long)pn[i] * s.H + yy) * s.W + xx) * s.Cin + ci) : zero8();
+            const bool ok = kv && pv[i] && (unsigned)(ys[i] + e.ky) < (unsigned)s.H &&
+                            (unsigned)(xs[i] + e.kx) < (unsigned)s.W;
+            a[i] = ok ? load8_at(x, pb[i] + e.delta) : zero8();
         }
 #pragma unroll
-        for (int j = 0; j < TJ; ++j) {
-            const int co = c0 + 16 * j + r;
-            b[j] = (kv && co < s.Cout) ? load8(w + (long long)co * K + k) : zero8();
-        }
+        for (int j = 0; j < TJ; ++j) b[j] = (kv && cv[j]) ? load8_at(w, wrow[j] + k) : zero8();
     };
     auto mma = [&](const bf16x8 (&a)[TI], const bf16x8 (&b)[TJ]) {
 #pragma unroll
@@ -188,57 +264,85 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, co
     k_loop<KS, TI, TJ>(K, wk, load, mma);
     reduce_waves<KS>(acc, wave, lane);
     if (KS > 1 && wave != 0) return;
+    __shared__ EpiSlab<TI, TJ> slab[KS > 1 ? 1 : 4];
+    tile_out(acc, slab[KS > 1 ? 0 : wave], lane, [&](int px, int c8, const float (&o)[8]) {
+        const unsigned p = p0 + px;
+        const int co = c0 + c8;
+        if (p >= P || co >= s.Cout) return;
+        const long long idx = (long long)p * s.Cout + co;
+        float v[8];
 #pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j) {
-            const int co = c0 + 16 * j + r;
-            if (co >= s.Cout) continue;
-            const float bv = bias ? bias[co] : 0.f;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const long long p = p0 + 16 * i + 4 * kg + e;
-                if (p >= P) continue;
-                float v = acc[i][j][e] + bv;
-                if (relu) v = v > 0.f ? v : 0.f;
-                if (y) y[p * s.Cout + co] = (bf16)v;
-                if (yf) yf[p * s.Cout + co] = v;
-            }
+        for (int e = 0; e < 8; ++e) {
+            v[e] = o[e] + (bias ? bias[co + e] : 0.f);
+            if (relu) v[e] = v[e] > 0.f ? v[e] : 0.f;
         }
+        if (y) {
+            bf16x8 b;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) b[e] = (bf16)v[e];
+            *(bf16x8*)(y + idx) = b;
+        }
+        if (yf) {
+            *(f32x4*)(yf + idx) = f32x4{v[0], v[1], v[2], v[3]};
+            *(f32x4*)(yf + idx + 4) = f32x4{v[4], v[5], v[6], v[7]};
+        }
+    });
 }
 
 // ---------------------------------------------------------- data gradient --
 // dX[p_in][ci] = sum over (tap, co) dY[p_out][co] * WT[tap][ci][co], with
 // p_out = ((y_in + 1 - ky) / stride, (x_in + 1 - kx) / stride) when exact.
 // Epilogue: dX = (accumulate ? dX : 0) + value * (mask ? (mask > 0) : 1).
-// Wave tiles, split k and prefetch as in the forward kernel (pixels x input
-// channels).
-template <int WC, int TI, int TJ, int KS>
+// Wave tiles, split k and the tap table as in the forward kernel (pixels x
+// input channels); with stride 1 the source is a fixed offset from the
+// pixel, with stride 2 it is found per pixel and tap.
+template <int WC, int TI, int TJ, int KS, bool S1>
 __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ dy, const bf16* __restrict__ wt,
                                                        bf16* __restrict__ dx, const bf16* __restrict__ mask,
                                                        const ConvShape s, const int accumulate)
 {
     static_assert(KS == 1 || WC == 1, "split k: the waves share one tile");
     constexpr int WP = KS > 1 ? 1 : 4 / WC;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ KEntry tbl[kMaxKGroups];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wp = KS > 1 ? 0 : wave % WP, wc = KS > 1 ? 0 : wave / WP, wk = KS > 1 ? wave : 0;
-    const long long P = (long long)s.N * s.H * s.W;
-    const long long p0 = (long long)blockIdx.x * (16 * TI * WP) + 16 * TI * wp;
+    const unsigned P = (unsigned)s.N * (unsigned)s.H * (unsigned)s.W;
+    const unsigned p0 = blockIdx.x * (16 * TI * WP) + 16 * TI * wp;
     const int c0 = blockIdx.y * (16 * TJ * WC) + 16 * TJ * wc;
     const int K = 9 * s.Cout;
     const int r = lane & 15, kg = lane >> 4;
-    int pn[TI], py[TI], px[TI];
+    for (int t = threadIdx.x; t < K / 8; t += 256) {
+        const int tap = (8 * t) / s.Cout, co = (8 * t) % s.Cout;
+        const int ky = tap / 3, kx = tap % 3;
+        // stride 1: source (y + 1 - ky, x + 1 - kx); stride 2: the channel only
+        const int d = S1 ? ((1 - ky) * s.Wo + (1 - kx)) * s.Cout + co : co;
+        tbl[t] = KEntry{d, (short)ky, (short)kx};
+    }
+    unsigned pb[TI];
+    int py[TI], px[TI];
     bool pv[TI];
 #pragma unroll
     for (int i = 0; i < TI; ++i) {
-        const long long p = p0 + 16 * i + r;
+        const unsigned p = p0 + 16 * i + r;
         pv[i] = p < P;
-        const long long pp = pv[i] ? p : 0;
-        px[i] = (int)(pp % s.W);
-        const long long t = pp / s.W;
-        py[i] = (int)(t % s.H);
-        pn[i] = (int)(t / s.H);
+        const unsigned pp = pv[i] ? p : 0u, t = pp / (unsigned)s.W;
+        const unsigned xi = pp - t * (unsigned)s.W, yi = t % (unsigned)s.H, n = t / (unsigned)s.H;
+        py[i] = (int)yi;
+        px[i] = (int)xi;
+        // stride 1: the pixel's own offset in dY; stride 2: its image's
+        pb[i] = S1 ? ((n * (unsigned)s.Ho + yi) * (unsigned)s.Wo + xi) * (unsigned)s.Cout
+                   : n * (unsigned)s.Ho * (unsigned)s.Wo * (unsigned)s.Cout;
     }
+    unsigned wcol[TJ];
+    bool cv[TJ];
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+        const int ci = c0 + 16 * j + r;
+        cv[j] = ci < s.Cin;
+        wcol[j] = cv[j] ? (unsigned)ci * (unsigned)s.Cout : 0u;
+    }
+    __syncthreads();
     f32x4 acc[TI][TJ];
 #pragma unroll
     for (int i = 0; i < TI; ++i)
@@ -247,25 +351,26 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ 
     auto load = [&](int k0, bf16x8 (&a)[TI], bf16x8 (&b)[TJ]) {
         const int k = k0 + 8 * kg;
         const bool kv = k < K;
-        const int tap = kv ? k / s.Cout : 0, co = kv ? k % s.Cout : 0;
-        const int ky = tap / 3, kx = tap % 3;
+        const KEntry e = tbl[kv ? k / 8 : 0];
 #pragma unroll
         for (int i = 0; i < TI; ++i) {
-            const int ty = py[i] + 1 - ky, tx = px[i] + 1 - kx;
-            bool ok = kv && pv[i] && ty >= 0 && tx >= 0;
-            int yo = 0, xo = 0;
-            if (ok) {
-                if (s.stride == 1) { yo = ty; xo = tx; }
-                else { ok = ((ty | tx) & 1) == 0; yo = ty >> 1; xo = tx >> 1; }
-                ok = ok && yo < s.Ho && xo < s.Wo;
+            const int ty = py[i] + 1 - e.ky, tx = px[i] + 1 - e.kx;
+            if constexpr (S1) {
+                const bool ok = kv && pv[i] && (unsigned)ty < (unsigned)s.Ho && (unsigned)tx < (unsigned)s.Wo;
+                a[i] = ok ? load8_at(dy, pb[i] + e.delta) : zero8();
+            } else {
+                const bool ok = kv && pv[i] && ((ty | tx) & 1) == 0 && (unsigned)(ty >> 1) < (unsigned)s.Ho &&
+                                (unsigned)(tx >> 1) < (unsigned)s.Wo;
+                const unsigned off = pb[i] + ((unsigned)(ty >> 1) * (unsigned)s.Wo + (unsigned)(tx >> 1)) *
+                                                 (unsigned)s.Cout + e.delta;
+                a[i] = ok ? load8_at(dy, off) : zero8();
             }
-            a[i] = ok ? load8(dy + (((long long)pn[i] * s.Ho + yo) * s.Wo + xo) * s.Cout + co) : zero8();
         }
+        // WT[tap][ci][co] at (tap * Cin + ci) * Cout + co = wcol + tap * Cin * Cout + co
+        const unsigned tb = (unsigned)(e.ky * 3 + e.kx) * (unsigned)s.Cin * (unsigned)s.Cout +
+                            (unsigned)(S1 ? (k - (e.ky * 3 + e.kx) * s.Cout) : e.delta);
 #pragma unroll
-        for (int j = 0; j < TJ; ++j) {
-            const int ci = c0 + 16 * j + r;
-            b[j] = (kv && ci < s.Cin) ? load8(wt + ((long long)tap * s.Cin + ci) * s.Cout + co) : zero8();
-        }
+        for (int j = 0; j < TJ; ++j) b[j] = (kv && cv[j]) ? load8_at(wt, wcol[j] + tb) : zero8();
     };
     auto mma = [&](const bf16x8 (&a)[TI], const bf16x8 (&b)[TJ]) {
 #pragma unroll
@@ -276,23 +381,24 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const bf16* __restrict__ 
     k_loop<KS, TI, TJ>(K, wk, load, mma);
     reduce_waves<KS>(acc, wave, lane);
     if (KS > 1 && wave != 0) return;
+    __shared__ EpiSlab<TI, TJ> slab[KS > 1 ? 1 : 4];
+    tile_out(acc, slab[KS > 1 ? 0 : wave], lane, [&](int pxl, int c8, const float (&o)[8]) {
+        const unsigned p = p0 + pxl;
+        const int ci = c0 + c8;
+        if (p >= P || ci >= s.Cin) return;
+        const long long idx = (long long)p * s.Cin + ci;
+        const bf16x8 m = mask ? load8(mask + idx) : zero8();
+        const bf16x8 a = accumulate ? load8(dx + idx) : zero8();
+        bf16x8 b;
 #pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j) {
-            const int ci = c0 + 16 * j + r;
-            if (ci >= s.Cin) continue;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const long long p = p0 + 16 * i + 4 * kg + e;
-                if (p >= P) continue;
-                const long long idx = p * s.Cin + ci;
-                float v = acc[i][j][e];
-                if (mask && !((float)mask[idx] > 0.f)) v = 0.f;
-                if (accumulate) v += (float)dx[idx];
-                dx[idx] = (bf16)v;
-            }
+        for (int e = 0; e < 8; ++e) {
+            float v = o[e];
+            if (mask && !((float)m[e] > 0.f)) v = 0.f;
+            if (accumulate) v += (float)a[e];
+            b[e] = (bf16)v;
         }
+        *(bf16x8*)(dx + idx) = b;
+    });
 }
 
 // -------------------------------------------------------- weight gradient --
@@ -338,7 +444,11 @@ __device__ __forceinline__ bf16x8 pack8(const f32x4 a, const f32x4 b)
 constexpr int kWgBlocks = 512;
 constexpr bool kWgPrefetch = false;
 
-template <int MB, int NB>
+// ROW: the output width is a multiple of 32, so a chunk's 32 pixels lie in
+// one row and its (image, row, first column) are wave-uniform, computed once
+// per chunk in scalar registers.  Offsets are 32-bit (the host checks that
+// the tensors hold fewer than 2^31 elements).
+template <int MB, int NB, bool ROW>
 __global__ __launch_bounds__(256) void k_conv_bwd_weight(const bf16* __restrict__ x,
                                                          const bf16* __restrict__ dy,
                                                          float* __restrict__ dw, float* __restrict__ db,
@@ -348,7 +458,8 @@ __global__ __launch_bounds__(256) void k_conv_bwd_weight(const bf16* __restrict_
     constexpr int NT = 2 * NB;                // 16-column tiles of k
     __shared__ f32x4 red[3][MB][NT][64];
     __shared__ float bred[4][MB][16];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = lane >> 4, r = lane & 15;
     const int co0 = blockIdx.x * 16 * MB, kc0 = blockIdx.y * 32 * NB;
     const int K = 9 * s.Cin;
@@ -356,17 +467,18 @@ __global__ __launch_bounds__(256) void k_conv_bwd_weight(const bf16* __restrict_
     const unsigned nchunks = (P + 31) / 32;
     const unsigned cbeg = blockIdx.z * (unsigned)chunks_per_block;
     const unsigned cend = cbeg + chunks_per_block < nchunks ? cbeg + chunks_per_block : nchunks;
-    // this lane's 8 columns of each 32-column block: tap and first channel
-    int ky[NB], kx[NB], kci[NB];
+    // this lane's 8 columns of each 32-column block: tap, and the element
+    // offset of the column's source from the pixel (y s, x s) of X
+    int ky[NB], kx[NB], dk[NB];
     bool kv[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int kk = kc0 + 32 * b + 8 * g;
         kv[b] = kk < K;
-        const int tap = kv[b] ? kk / s.Cin : 0;
-        kci[b] = kv[b] ? kk % s.Cin : 0;
+        const int tap = kv[b] ? kk / s.Cin : 0, ci = kv[b] ? kk % s.Cin : 0;
         ky[b] = tap / 3;
         kx[b] = tap % 3;
+        dk[b] = ((ky[b] - 1) * s.W + (kx[b] - 1)) * s.Cin + ci;
     }
     const bf16x8 sel0 = selector(lane, 0), sel1 = selector(lane, 1);
     const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
@@ -381,23 +493,43 @@ __global__ __launch_bounds__(256) void k_conv_bwd_weight(const bf16* __restrict_
     // operands of one chunk: dY (32 channels per load) and X columns, for
     // pixels 0-15 (half 0) and 16-31 (half 1) of the chunk
     auto load = [&](unsigned c, bf16x8 (&yv)[MJ][2], bf16x8 (&xv)[NB][2]) {
+        unsigned n0 = 0, yo0 = 0, xo0 = 0;
+        if constexpr (ROW) {
+            const unsigned cpr = (unsigned)s.Wo / 32, row = c / cpr;
+            xo0 = (c - row * cpr) * 32;
+            yo0 = row % (unsigned)s.Ho;
+            n0 = row / (unsigned)s.Ho;
+        }
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             const unsigned p = 32 * c + 16 * half + r;
-            const bool pv = p < P;
-            const unsigned pp = pv ? p : 0u;
-            const unsigned xo = pp % (unsigned)s.Wo, t = pp / (unsigned)s.Wo;
-            const unsigned yo = t % (unsigned)s.Ho, n = t / (unsigned)s.Ho;
+            bool pv;
+            unsigned n, yo, xo;
+            if constexpr (ROW) {
+                pv = true;
+                n = n0;
+                yo = yo0;
+                xo = xo0 + 16 * half + r;
+            } else {
+                pv = p < P;
+                const unsigned pp = pv ? p : 0u, t = pp / (unsigned)s.Wo;
+                xo = pp - t * (unsigned)s.Wo;
+                yo = t % (unsigned)s.Ho;
+                n = t / (unsigned)s.Ho;
+            }
 #pragma unroll
             for (int j = 0; j < MJ; ++j) {
                 const int co = co0 + 32 * j + 8 * g;
-                yv[j][half] = (pv && co < s.Cout) ? load8(dy + (long long)pp * s.Cout + co) : zero8();
+                yv[j][half] = (pv && co < s.Cout) ? load8_at(dy, p * (unsigned)s.Cout + co) : zero8();
             }
+            const int ys = (int)yo * s.stride - 1, xs = (int)xo * s.stride - 1;
+            const unsigned xb = ((n * (unsigned)s.H + yo * s.stride) * (unsigned)s.W + xo * s.stride) *
+                                (unsigned)s.Cin;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                const int yy = (int)yo * s.stride + ky[b] - 1, xx = (int)xo * s.stride + kx[b] - 1;
-                const bool ok = pv && kv[b] && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W;
-                xv[b][half] = ok ? load8(x + (((long long)n * s.H + yy) * s.W + xx) * s.Cin + kci[b]) : zero8();
+                const bool ok = pv && kv[b] && (unsigned)(ys + ky[b]) < (unsigned)s.H &&
+                                (unsigned)(xs + kx[b]) < (unsigned)s.W;
+                xv[b][half] = ok ? load8_at(x, xb + dk[b]) : zero8();
             }
         }
     };
@@ -770,6 +902,9 @@ int mfa_nn_conv3x3(const void* x, const void* w, const float* bias, void* y, flo
     if (!x || !w || (!y && !y_f32)) return nn_fail(-1, "mfa_nn_conv3x3: null pointer");
     if (Cin % 8 || Cout % 8 || (stride != 1 && stride != 2) || N < 1 || H < 1 || W < 1)
         return nn_fail(-1, "mfa_nn_conv3x3: channels must be multiples of 8, stride 1 or 2");
+    if (!aligned16({x, w, y, y_f32})) return nn_fail(-1, "mfa_nn_conv3x3: tensors must be 16-byte aligned");
+    if (Cin > 128 || (long long)N * H * W * Cin > 0x7fffffffLL || (long long)Cout * 9 * Cin > 0x7fffffffLL)
+        return nn_fail(-1, "mfa_nn_conv3x3: at most 128 input channels and 2^31 elements per tensor");
     ConvShape s{N, H, W, Cin, Cout, stride, (H + stride - 1) / stride, (W + stride - 1) / stride};
     const long long P = (long long)N * s.Ho * s.Wo;
     // block tiles (pixels x channels): 256 x 16, 128 x 32, 64 x 64; layers
@@ -801,25 +936,34 @@ int mfa_nn_conv3x3_bwd_data(const void* dy, const void* wt, void* dx, const void
     if (!dy || !wt || !dx) return nn_fail(-1, "mfa_nn_conv3x3_bwd_data: null pointer");
     if (Cin % 8 || Cout % 8 || (stride != 1 && stride != 2))
         return nn_fail(-1, "mfa_nn_conv3x3_bwd_data: channels must be multiples of 8, stride 1 or 2");
+    if (!aligned16({dy, wt, dx, mask}))
+        return nn_fail(-1, "mfa_nn_conv3x3_bwd_data: tensors must be 16-byte aligned");
+    if (Cout > 128 || (long long)N * H * W * Cin > 0x7fffffffLL || (long long)N * H * W * Cout > 0x7fffffffLL)
+        return nn_fail(-1, "mfa_nn_conv3x3_bwd_data: at most 128 output channels and 2^31 elements per tensor");
     ConvShape s{N, H, W, Cin, Cout, stride, (H + stride - 1) / stride, (W + stride - 1) / stride};
     const long long P = (long long)N * H * W;
     hipStream_t st = (hipStream_t)stream;
     const bf16* dyb = (const bf16*)dy;
     const bf16* wtb = (const bf16*)wt;
     const bf16* mb = (const bf16*)mask;
-    if (Cin <= 16) {
-        dim3 grid((unsigned)((P + 255) / 256), (Cin + 15) / 16);
-        hipLaunchKernelGGL((k_conv_bwd_data<1, 4, 1, 1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
-    } else if (Cin <= 32) {
-        dim3 grid((unsigned)((P + 127) / 128), (Cin + 31) / 32);
-        hipLaunchKernelGGL((k_conv_bwd_data<1, 2, 2, 1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
-    } else if (((P + 63) / 64) * ((Cin + 63) / 64) >= 512) {
-        dim3 grid((unsigned)((P + 63) / 64), (Cin + 63) / 64);
-        hipLaunchKernelGGL((k_conv_bwd_data<2, 2, 2, 1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
-    } else {
-        dim3 grid((unsigned)((P + 31) / 32), (Cin + 31) / 32);
-        hipLaunchKernelGGL((k_conv_bwd_data<1, 2, 2, 4>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
-    }
+    auto go = [&](auto s1) {
+        constexpr bool S1 = decltype(s1)::value;
+        if (Cin <= 16) {
+            dim3 grid((unsigned)((P + 255) / 256), (Cin + 15) / 16);
+            hipLaunchKernelGGL((k_conv_bwd_data<1, 4, 1, 1, S1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
+        } else if (Cin <= 32) {
+            dim3 grid((unsigned)((P + 127) / 128), (Cin + 31) / 32);
+            hipLaunchKernelGGL((k_conv_bwd_data<1, 2, 2, 1, S1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
+        } else if (((P + 63) / 64) * ((Cin + 63) / 64) >= 512) {
+            dim3 grid((unsigned)((P + 63) / 64), (Cin + 63) / 64);
+            hipLaunchKernelGGL((k_conv_bwd_data<2, 2, 2, 1, S1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
+        } else {
+            dim3 grid((unsigned)((P + 31) / 32), (Cin + 31) / 32);
+            hipLaunchKernelGGL((k_conv_bwd_data<1, 2, 2, 4, S1>), grid, dim3(256), 0, st, dyb, wtb, (bf16*)dx, mb, s, accumulate);
+        }
+    };
+    if (stride == 1) go(std::true_type{});
+    else go(std::false_type{});
     NN_HIP(hipGetLastError());
     return 0;
 }
@@ -830,9 +974,11 @@ int mfa_nn_conv3x3_bwd_weight(const void* x, const void* dy, float* dw, float* d
     if (!x || !dy || !dw) return nn_fail(-1, "mfa_nn_conv3x3_bwd_weight: null pointer");
     if (Cin % 8 || Cout % 8 || (stride != 1 && stride != 2))
         return nn_fail(-1, "mfa_nn_conv3x3_bwd_weight: channels must be multiples of 8, stride 1 or 2");
+    if (!aligned16({x, dy})) return nn_fail(-1, "mfa_nn_conv3x3_bwd_weight: tensors must be 16-byte aligned");
     ConvShape s{N, H, W, Cin, Cout, stride, (H + stride - 1) / stride, (W + stride - 1) / stride};
     const long long P = (long long)N * s.Ho * s.Wo;
-    if (P > 0x7fffffffLL) return nn_fail(-1, "mfa_nn_conv3x3_bwd_weight: more than 2^31 output pixels");
+    if (P * Cout > 0x7fffffffLL || (long long)N * H * W * Cin > 0x7fffffffLL)
+        return nn_fail(-1, "mfa_nn_conv3x3_bwd_weight: a tensor holds 2^31 elements or more");
     const int K = 9 * Cin;
     // tiles: 16 or 32 output channels (MB 1 or 2) x 32 NB columns of k.
     // Thin layers (K <= 160) take all their columns in one block; wider ones
@@ -849,14 +995,23 @@ int mfa_nn_conv3x3_bwd_weight(const void* x, const void* dy, float* dw, float* d
     nz = (nchunks + per - 1) / per;
     dim3 grid(gx, gy, (unsigned)nz);
     hipStream_t st = (hipStream_t)stream;
-    if (MB == 1 && NB == 3)
-        hipLaunchKernelGGL((k_conv_bwd_weight<1, 3>), grid, dim3(256), 0, st, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
-    else if (MB == 1 && NB == 5)
-        hipLaunchKernelGGL((k_conv_bwd_weight<1, 5>), grid, dim3(256), 0, st, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
-    else if (MB == 2 && NB == 3)
-        hipLaunchKernelGGL((k_conv_bwd_weight<2, 3>), grid, dim3(256), 0, st, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
-    else
-        hipLaunchKernelGGL((k_conv_bwd_weight<2, 5>), grid, dim3(256), 0, st, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
+    const bool row = s.Wo % 32 == 0;
+    auto go = [&](auto mb, auto nb, auto rw) {
+        hipLaunchKernelGGL((k_conv_bwd_weight<decltype(mb)::value, decltype(nb)::value, decltype(rw)::value>),
+                           grid, dim3(256), 0, st, (const bf16*)x, (const bf16*)dy, dw, dbias, s, (int)per);
+    };
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I5 = std::integral_constant<int, 5>;
+    auto by_row = [&](auto mb, auto nb) {
+        if (row) go(mb, nb, std::true_type{});
+        else go(mb, nb, std::false_type{});
+    };
+    if (MB == 1 && NB == 3) by_row(I1{}, I3{});
+    else if (MB == 1 && NB == 5) by_row(I1{}, I5{});
+    else if (MB == 2 && NB == 3) by_row(I2{}, I3{});
+    else by_row(I2{}, I5{});
     NN_HIP(hipGetLastError());
     return 0;
 }
@@ -878,13 +1033,7 @@ int mfa_nn_prep_weights(const float* w32, void* w, void* wt, int Cout, int Cin, 
     return 0;
 }
 
-static bool vec8(int C, std::initializer_list<const void*> ptrs)
-{
-    if (C % 8) return false;
-    for (const void* p : ptrs)
-        if ((uintptr_t)p % 16) return false;
-    return true;
-}
+static bool vec8(int C, std::initializer_list<const void*> ptrs) { return C % 8 == 0 && aligned16(ptrs); }
 
 int mfa_nn_upsample_add(const void* x, const void* skip, void* y, int N, int H, int W, int C, void* stream)
 {

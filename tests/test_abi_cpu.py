@@ -301,3 +301,19 @@ def test_step2_refuses_planes_beyond_32_bit_offsets_before_launch():
         rc = L.mfa_step2(ctypes.byref(a), ctypes.byref(b), ctypes.byref(g), 1.25, None, None)
         msg = L.mfa_last_error().decode()
         assert rc == -2 and "4 GiB" in msg, (es, rc, msg)
+
+
+def test_convolutions_refuse_misaligned_tensors_before_launch():
+    """The warm-start network's convolutions move 16 bytes per lane, so a
+    tensor that is not 16-byte aligned is refused (-1) on the host; the
+    pointers are never dereferenced."""
+    from muphfasa_amd import surrogate as S
+    L = S.lib()
+    ok, bad = 1 << 20, (1 << 20) + 2
+    shape = (1, 8, 8, 16, 16, 1)
+    assert L.mfa_nn_conv3x3(bad, ok, None, ok, None, *shape, 1, None) == -1
+    assert "aligned" in L.mfa_nn_error().decode()
+    assert L.mfa_nn_conv3x3_bwd_data(ok, ok, bad, None, *shape, 0, None) == -1
+    assert "aligned" in L.mfa_nn_error().decode()
+    assert L.mfa_nn_conv3x3_bwd_weight(ok, bad, ok, None, *shape, None) == -1
+    assert "aligned" in L.mfa_nn_error().decode()
