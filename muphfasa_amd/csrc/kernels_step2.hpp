@@ -368,4 +368,157 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
     }
 }
 
+// ------------------------------------------------------------ wall shell --
+// Two steps on a walled box in two launches.  A cell two or more cells from
+// every wall depends, over two steps, only on cells inside the domain: its
+// first-step neighbours are at least one cell from the walls, so none of
+// their pulls crosses one.  k_step2p run as if the box were periodic gives
+// those cells exactly (its wrap-around reaches only the shell), and
+// k_step2_faces then recomputes the shell: the cells within one cell of a
+// non-periodic face.
+//
+// A block takes a TA x TB tile of one face's in-plane dimensions (a, b) and
+// the face's two shell cells along its normal d.  Phase A computes the first
+// step, with k_step's per-cell arithmetic (pull with the wall values of the
+// input, collide), for the tile grown by one cell in a and b and for three
+// cells along d (the shell and the next cell inward), into LDS as
+// [cell][q].  Phase B pulls the second step of each shell cell from there;
+// a pull from behind a wall takes the wall value the reference's BCManager
+// writes there, computed from the cell's own first-step populations
+// (src/LBM/BoundaryConditions.H:124-189 and src/LBM/BCTypes/).  Cells on
+// two faces are computed by both, with the same bits.
+template <int TA_, int TB_>
+struct FaceTile {
+    static constexpr int TA = TA_, TB = TB_;
+    static constexpr int GA = TA + 2, GB = TB + 2, GC = GA * GB * 3;
+    static constexpr int OC = 2 * TA * TB;
+    static constexpr int THREADS = 256;
+};
+
+// The wall faces of a box: normal dimension, side, tiles per in-plane
+// dimension and the first block of each face (block ranges, face by face).
+struct FaceList {
+    int n;
+    int dim[6], hi[6], ta[6], tb[6];
+    int first[7];
+};
+
+template <class L, bool FORCED, class T>
+__global__ __launch_bounds__(T::THREADS) void k_step2_faces(const double* __restrict__ fin,
+                                                            double* __restrict__ fout, const Geo g,
+                                                            const View vi, const View vo,
+                                                            const double omega, const double F0,
+                                                            const double F1, const double F2,
+                                                            const FaceList fl)
+{
+    static_assert(L::D == 3, "three-dimensional lattices only");
+    __shared__ double s1[T::GC][L::Q];
+    __shared__ signed char s_rtype[27];
+    __shared__ double s_rrho[27];
+    __shared__ double s_ru[27][3];
+    const int tid = threadIdx.x;
+    if (tid < 27) {
+        s_rtype[tid] = g.rtype[tid];
+        s_rrho[tid] = g.rrho[tid];
+        s_ru[tid][0] = g.ru[tid][0];
+        s_ru[tid][1] = g.ru[tid][1];
+        s_ru[tid][2] = g.ru[tid][2];
+    }
+    GeoRef gr;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        gr.n[d] = g.n[d];
+        gr.per[d] = g.per[d];
+    }
+    gr.walls = g.walls;
+    gr.rtype = s_rtype;
+    gr.rrho = s_rrho;
+    gr.ru = s_ru;
+    // face and tile of this block (wave-uniform)
+    const int blk = blockIdx.x;
+    int f = 0;
+    while (f + 1 < fl.n && blk >= fl.first[f + 1]) ++f;
+    const int d = fl.dim[f], hi = fl.hi[f];
+    const int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;     // in-plane dimensions, a < b
+    const int t = blk - fl.first[f];
+    const int a0 = (t % fl.ta[f]) * T::TA, b0 = (t / fl.ta[f]) * T::TB;
+    const int na = g.n[da], nb = g.n[db], nd = g.n[d];
+    const int d0 = hi ? nd - 3 : 0;                         // first-step slab [d0, d0 + 3)
+    __syncthreads();
+
+    // phase A: first step of the grown slab.  Threads walk the slab with x
+    // fastest: along a for the y and z faces, along d for the x faces (the
+    // three cells of one row side by side in one cache line).
+    for (int c = tid; c < T::GC; c += T::THREADS) {
+        int ia, ib, id;
+        if (d == 0) {
+            id = c % 3;
+            ia = (c / 3) % T::GA;
+            ib = c / (3 * T::GA);
+        } else {
+            ia = c % T::GA;
+            ib = (c / T::GA) % T::GB;
+            id = c / (T::GA * T::GB);
+        }
+        int ca = a0 - 1 + ia, cb = b0 - 1 + ib;
+        if (g.per[da]) ca = (ca % na + na) % na;
+        else if (ca < 0 || ca >= na) continue;
+        if (g.per[db]) cb = (cb % nb + nb) % nb;
+        else if (cb < 0 || cb >= nb) continue;
+        int pos[3];
+        pos[d] = d0 + id;
+        pos[da] = ca;
+        pos[db] = cb;
+        double P[L::Q];
+        pull<L, 0, true, 0>(P, fin, pos[0], pos[1], pos[2], gr, vi);
+        const int sl = ia + T::GA * (ib + T::GB * id);
+        collide<L, FORCED>(P, omega, F0, F1, F2, [&](auto qc, double out) { s1[sl][decltype(qc)::value] = out; });
+    }
+    __syncthreads();
+
+    // phase B: second step of the shell cells
+    for (int o = tid; o < T::OC; o += T::THREADS) {
+        int ia, ib, k;
+        if (d == 0) {
+            k = o % 2;
+            ia = (o / 2) % T::TA;
+            ib = o / (2 * T::TA);
+        } else {
+            ia = o % T::TA;
+            ib = (o / T::TA) % T::TB;
+            k = o / (T::TA * T::TB);
+        }
+        const int ca = a0 + ia, cb = b0 + ib;
+        if (ca >= na || cb >= nb) continue;
+        const int cd = hi ? nd - 1 - k : k;
+        int pos[3];
+        pos[d] = cd;
+        pos[da] = ca;
+        pos[db] = cb;
+        int lo[3], hf[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            lo[e] = !g.per[e] && pos[e] == 0;
+            hf[e] = !g.per[e] && pos[e] == g.n[e] - 1;
+        }
+        // grown-slab coordinates of the cell
+        const int ga = ia + 1, gb = ib + 1, gd = cd - d0;
+        auto slot = [&](int xa, int xb, int xd) { return xa + T::GA * (xb + T::GB * xd); };
+        const int own = slot(ga, gb, gd);
+        double fq[L::Q];
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int c3[3] = {L::c(q, 0), L::c(q, 1), L::c(q, 2)};
+            // the pull source x - c_q in the grown slab; a source behind the
+            // normal face lies outside it and is replaced by the wall value
+            int sd = gd - c3[d];
+            sd = sd < 0 ? 0 : (sd > 2 ? 2 : sd);
+            fq[q] = s1[slot(ga - c3[da], gb - c3[db], sd)][q];
+            fix_wall_at<L, q>(fq[q], [&](int m) { return s1[own][m]; }, gr, lo, hf);
+        });
+        collide_store<L, FORCED>(fq, fout, pos[0] + pos[1] * vo.sy + (long long)pos[2] * vo.sz, vo, omega, F0,
+                                 F1, F2);
+    }
+}
+
 }  // namespace mfa
