@@ -387,12 +387,12 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
 // writes there, computed from the cell's own first-step populations
 // (src/LBM/BoundaryConditions.H:124-189 and src/LBM/BCTypes/).  Cells on
 // two faces are computed by both, with the same bits.
-template <int TA_, int TB_>
+template <int TA_, int TB_, int THREADS_ = 256>
 struct FaceTile {
     static constexpr int TA = TA_, TB = TB_;
     static constexpr int GA = TA + 2, GB = TB + 2, GC = GA * GB * 3;
     static constexpr int OC = 2 * TA * TB;
-    static constexpr int THREADS = 256;
+    static constexpr int THREADS = THREADS_;
 };
 
 // The wall faces of a box: normal dimension, side, tiles per in-plane

@@ -87,8 +87,10 @@ def main():
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--lattice", default="D3Q19", choices=["D3Q15", "D3Q19", "D3Q27"])
     p.add_argument("--reps", type=int, default=30)
+    p.add_argument("--sets", default="lid,noslip,xper_walls",
+                   help="comma-separated boundary sets (" + ", ".join(SETS) + ")")
     a = p.parse_args()
-    print(json.dumps(run(a.n, a.lattice, reps=a.reps)), flush=True)
+    print(json.dumps(run(a.n, a.lattice, sets=a.sets.split(","), reps=a.reps)), flush=True)
 
 
 if __name__ == "__main__":
