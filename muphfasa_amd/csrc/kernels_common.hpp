@@ -71,45 +71,6 @@ constexpr int cz_rank(int q)
     return r;
 }
 
-// Intermediate state of the two-step kernels, as words W of an EC-cell
-// grown column.  Phase B of plane z reads populations with c_z = -1 from
-// plane z+1, c_z = 0 from plane z and c_z = +1 from plane z-1.  Plane p is
-// written while plane p-1 is finished, so its c_z = -1 part is read in the
-// same iteration, its c_z = 0 part one iteration later and its c_z = +1 part
-// two iterations later: per class k a ring of k+1 planes holds exactly what
-// is still to be read (D3Q27: 54 population planes instead of the 81 of
-// three whole planes).  The writes of plane p reuse slots whose last reads
-// were in the previous iteration, so a barrier after each iteration's reads
-// is needed (ONE_BARRIER false).
-template <class L, class W, int EC>
-struct CzRings {
-    static constexpr bool ONE_BARRIER = false;
-    static constexpr int N0 = cz_count<L>(0), N1 = cz_count<L>(1), N2 = cz_count<L>(2);
-    W c0[1][N0][EC];
-    W c1[2][N1][EC];
-    W c2[3][N2][EC];
-    template <int q>
-    __device__ __forceinline__ W& at(int p, int Z0, int cell)
-    {
-        constexpr int k = L::c(q, 2) + 1, r = cz_rank<L>(q);
-        if constexpr (k == 0) return c0[0][r][cell];
-        else if constexpr (k == 1) return c1[(p - Z0 + 2) % 2][r][cell];
-        else return c2[(p - Z0 + 3) % 3][r][cell];
-    }
-};
-
-// Whole planes in a ring of RING.  With four, the plane phase A overwrites
-// was last read two iterations back, behind the one barrier every iteration
-// has (ONE_BARRIER true).
-template <class L, class W, int EC, int RING>
-struct PlaneRing {
-    static_assert(RING == 3 || RING == 4, "ring of 3 or 4 planes");
-    static constexpr bool ONE_BARRIER = RING == 4;
-    W r[RING][L::Q][EC];
-    template <int q>
-    __device__ __forceinline__ W& at(int p, int Z0, int cell) { return r[(p - Z0 + RING) % RING][q][cell]; }
-};
-
 // ------------------------------------------------------------- pull stage --
 // Coordinates of the pull source for each lattice velocity component value
 // (index c+1): source = x - c.  Fused mode wraps periodic dimensions and

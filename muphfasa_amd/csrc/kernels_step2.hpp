@@ -39,15 +39,16 @@ namespace mfa {
 // the slot of source plane z - c_z; for class 0 that is plane z+1, one
 // iteration before phase B of z reads it, while the slot of plane z is still
 // being read: with walls class 0 gets a second slot.
-template <class L, int TC, bool WALLS>
+// W: the LDS word (double, or the 32-bit word of FP32 storage).
+template <class L, int TC, bool WALLS, class W = double>
 struct PushRings {
     static_assert(L::D == 3, "three-dimensional lattices only");
     static constexpr int N0 = cz_count<L>(0), N1 = cz_count<L>(1), N2 = cz_count<L>(2);
     static constexpr int S0 = WALLS ? 2 : 1, S1 = 2, S2 = 3;
     static constexpr int PLANES = S0 * N0 + S1 * N1 + S2 * N2;
-    double c0[S0][N0][TC];
-    double c1[S1][N1][TC];
-    double c2[S2][N2][TC];
+    W c0[S0][N0][TC];
+    W c1[S1][N1][TC];
+    W c2[S2][N2][TC];
     // ring slot of one plane in each class (or of the three planes phase B
     // reads); computed once per plane, wave-uniform
     struct Pos {
@@ -66,7 +67,7 @@ struct PushRings {
         return Pos{(rel + 1) % S0, rel % S1, (rel - 1) % S2};
     }
     template <int q>
-    __device__ __forceinline__ double& at(const Pos& ps, int cell)
+    __device__ __forceinline__ W& at(const Pos& ps, int cell)
     {
         constexpr int k = L::c(q, 2) + 1, r = cz_rank<L>(q);
         if constexpr (k == 0) return c0[ps.k0][r][cell];

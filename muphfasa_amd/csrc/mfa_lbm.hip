@@ -930,7 +930,11 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
         if (geo.walls)
             return fail(MFA_ERR_UNSUPPORTED,
                         "mfa_step2: 3-D boxes with walls need FP64 storage (use mfa_step)");
-        const int TZ = geo.n[2] < kStep2Planes ? geo.n[2] : kStep2Planes;
+        // FP32 pairs run k_step2p_dev on 64 x 8 columns: one round of blocks
+        // per CU, as the FP64 pass
+        int ncu = 0;
+        if ((rc = cu_count(&ncu))) return rc;
+        const int TZ = step2_planes(((geo.n[0] + 63) / 64) * ((geo.n[1] + 7) / 8), geo.n[2], ncu);
         return step2_dev(fin, fout, geo, lat, omega, force, TZ, (hipStream_t)stream);
     }
     // in-plane byte offsets are 32-bit in k_step2p

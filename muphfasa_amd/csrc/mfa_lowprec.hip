@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "kernels_lowprec.hpp"
+#include "kernels_step2.hpp"
 
 namespace mfa {
 
@@ -139,127 +140,14 @@ __global__ __launch_bounds__(256) void k_step_dev_x2(const T* __restrict__ fin, 
     });
 }
 
-// Two steps per pass on reduced storage: the temporal blocking of k_step2
-// (mfa_lbm.hip) for stored deviations.  A block owns a TX x TY column of
-// output cells and marches through its z-range.  Phase A pulls and collides
-// the column grown by one cell in x and y one plane ahead; phase B pulls the
-// second step from LDS planes (Step2DevStore) and writes it to HBM.  The
-// intermediate state is rounded to the storage type before it enters LDS,
-// exactly as a single step stores it, so the pair equals two k_step_dev
-// launches bit for bit, and it takes sizeof(T) bytes of LDS per population.
-// Fully periodic 3-D boxes.
-// MODE: how LDS holds the intermediate planes (kernels_common.hpp): 0 the
-// per-c_z rings (CzRings), 3 or 4 a ring of whole planes (PlaneRing).
-template <int TX_, int TY_, int MODE_>
+// Tile of the 2-D two-step pass on reduced storage (k_step2_2d_dev): TX x TY
+// output cells, the tile grown by one cell in LDS.
+template <int TX_, int TY_>
 struct Step2DevTile {
-    static constexpr int TX = TX_, TY = TY_, MODE = MODE_;
+    static constexpr int TX = TX_, TY = TY_;
     static constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
     static constexpr int THREADS = (EC + 63) / 64 * 64;
-    static_assert(MODE == 0 || MODE == 3 || MODE == 4, "plane storage mode");
 };
-
-template <class L, class W, class TL>
-using Step2DevStore = std::conditional_t<TL::MODE == 0, CzRings<L, W, TL::EC>,
-                                         PlaneRing<L, W, TL::EC, (TL::MODE == 0 ? 3 : TL::MODE)>>;
-
-// The tile of a lattice and storage type: MODE where it fits in the
-// 160 KiB of LDS of a CU, the per-c_z rings otherwise.
-template <class L, class T, int TX, int TY, int MODE>
-using Step2DevFit =
-    std::conditional_t<sizeof(Step2DevStore<L, T, Step2DevTile<TX, TY, MODE>>) <= 160 * 1024,
-                       Step2DevTile<TX, TY, MODE>, Step2DevTile<TX, TY, 0>>;
-
-template <class L, bool FORCED, class T, class TL>
-__global__ __launch_bounds__(TL::THREADS) void k_step2_dev(const T* __restrict__ fin, T* __restrict__ fout,
-                                                           const Geo g, const View vi, const View vo,
-                                                           const float omega, const float F0,
-                                                           const float F1, const float F2, const int TZ)
-{
-    using U = typename Word<T>::U;
-    using Store = Step2DevStore<L, U, TL>;
-    __shared__ Store lds;
-    const int tid = threadIdx.x;
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    // one contiguous run of tiles (x fastest) per XCD, as in k_step2
-    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const unsigned per = nb / kXcds, rem = nb % kXcds, k = b % kXcds;
-    const unsigned t = k * per + (k < rem ? k : rem) + b / kXcds;
-    const int bx = (int)(t % gridDim.x), by = (int)((t / gridDim.x) % gridDim.y);
-    const int bz = (int)(t / (gridDim.x * gridDim.y));
-    const int X0 = bx * TL::TX, Y0 = by * TL::TY, Z0 = bz * TZ;
-    const int Z1 = Z0 + TZ < nz ? Z0 + TZ : nz;
-    // phase A: one cell of the grown column (periodic images, any box size).
-    // Its pull sources in the plane are fixed for the whole march: byte
-    // offsets (x' + y' sy) * sizeof(T) for each (c_x, c_y), which the host
-    // has checked fit in 32 bits; the planes z - c_z are wave-uniform bases.
-    const bool inA = tid < TL::EC;
-    const int exA = tid % TL::EX, eyA = tid / TL::EX;
-    const int gx = ((X0 - 1 + exA) % nx + nx) % nx;
-    const int gy = ((Y0 - 1 + eyA) % ny + ny) % ny;
-    unsigned srcA[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            const int sx = (gx - (a - 1) + nx) % nx, sy = (gy - (b - 1) + ny) % ny;
-            srcA[a][b] = (unsigned)(sx + sy * vi.sy) * (unsigned)sizeof(T);
-        }
-    // phase B: one output cell
-    const int ox = tid % TL::TX, oy = tid / TL::TX;
-    const bool inB = tid < TL::TX * TL::TY && X0 + ox < nx && Y0 + oy < ny;
-    const unsigned dstB = inB ? (unsigned)((X0 + ox) + (Y0 + oy) * vo.sy) * (unsigned)sizeof(T) : 0u;
-    float P[L::Q];
-    auto loadA = [&](int z) {
-        if (!inA) return;
-        const T* plane[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) plane[c] = fin + (long long)(((z - (c - 1)) % nz + nz) % nz) * vi.sz;
-        for_q<L>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
-            P[q] = to_f(*at_bytes(plane[cz + 1] + q * vi.sq, srcA[cx + 1][cy + 1]));
-        });
-    };
-    auto storeA = [&](int z) {
-        if (!inA) return;
-        float o[L::Q];
-        collide_dev<L, FORCED>(P, o, omega, F0, F1, F2);
-        for_q<L>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            lds.template at<q>(z, Z0, tid) = f_to_word<T>(o[q]);
-        });
-    };
-    loadA(Z0 - 1);
-    storeA(Z0 - 1);
-    loadA(Z0);
-    storeA(Z0);
-    loadA(Z0 + 1);
-    for (int z = Z0; z < Z1; ++z) {
-        storeA(z + 1);
-        if (z + 2 <= Z1) loadA(z + 2);
-        __syncthreads();
-        if (inB) {
-            float d[L::Q];
-            for_q<L>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
-                d[q] = word_to_f<T>(lds.template at<q>(z - cz, Z0, (ox + 1 - cx) + TL::EX * (oy + 1 - cy)));
-            });
-            float o[L::Q];
-            collide_dev<L, FORCED>(d, o, omega, F0, F1, F2);
-            T* const orow = fout + (long long)z * vo.sz;
-            for_q<L>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                const T sv = from_f<T>(o[q]);
-                T* p = at_bytes(orow + q * vo.sq, dstB);
-                if constexpr (sizeof(T) == 4) __builtin_nontemporal_store(sv, p);
-                else *p = sv;
-            });
-        }
-        if constexpr (!Store::ONE_BARRIER) __syncthreads();
-    }
-}
 
 // Moments of reduced storage, evaluated in FP64 from the dequantised
 // populations w_q + d_q with the reference's arithmetic.
@@ -591,15 +479,121 @@ __global__ __launch_bounds__(TL::THREADS) void k_step2_2d_dev(const T* __restric
     collide_store_dev<L, FORCED>(d, fout, x + (long long)y * vo.sy, vo, omega, F0, F1, 0.f);
 }
 
-// Two steps in one pass (k_step2_dev); the caller has checked a fully
-// periodic 3-D box.  TZ: z-planes per block.  FP32 storage only: with FP16
-// and BF16 words the pass moves half the bytes but runs the same arithmetic,
-// which then bounds it, and it measured slower than two single steps on
-// MI355X (D3Q27 384^3 forced: 1.27 against 1.09 ms per step at its best tile).
-// Tiles (measured, D3Q27 384^3 / D3Q19 256^3 FP32 forced): D3Q27 32 x 8 with
-// a four-plane ring, 1.56 against 2.25 ms per step (147 KiB of LDS; the
-// per-c_z rings, 73 KiB, gave 1.75); D3Q15 and D3Q19 64 x 4 with the per-c_z
-// rings, 0.305 against 0.464 ms (a four-plane ring gave 0.331).
+// Two steps per pass on FP32 storage of a periodic 3-D box in the push
+// layout of k_step2p (kernels_step2.hpp): a block owns a 64 x TY column of
+// output cells; phase A pulls and collides the column grown by one cell and
+// pushes each population, rounded to the storage type exactly as a single
+// step stores it, into the LDS slot of the tile cell that pulls it in the
+// second step (per-c_z rings of 32-bit words); phase B reads its own cell's Q
+// words, collides and writes the second step.  Loads address wave-uniform
+// plane bases plus fixed 32-bit lane offsets.  Per cell the arithmetic is
+// k_step_dev's (collide_dev), so a pass equals two single steps bit for bit.
+template <class L, bool FORCED, class T, class TL>
+__global__ __launch_bounds__(TL::THREADS) __attribute__((amdgpu_waves_per_eu(TL::WAVES_PER_EU)))
+void k_step2p_dev(const T* __restrict__ fin, T* __restrict__ fout, const Geo g, const View vi, const View vo,
+                  const float omega, const float F0, const float F1, const float F2, const int TZ)
+{
+    using U = typename Word<T>::U;
+    using Rings = PushRings<L, TL::SC, false, U>;
+    using Pos = typename Rings::Pos;
+    __shared__ Rings lds;
+    __shared__ U dummy[TL::SX];
+    const int tid = threadIdx.x;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    // one contiguous run of tiles (x fastest) per XCD, as in k_step2p
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned per = nb / kXcds, rem = nb % kXcds, xcd = b % kXcds;
+    const unsigned t = xcd * per + (xcd < rem ? xcd : rem) + b / kXcds;
+    const int X0 = (int)(t % gridDim.x) * TL::TX, Y0 = (int)((t / gridDim.x) % gridDim.y) * TL::TY;
+    const int Z0 = (int)(t / (gridDim.x * gridDim.y)) * TZ;
+    const int Z1 = Z0 + TZ < nz ? Z0 + TZ : nz;
+    // phase A: one cell of the grown column; its in-plane pull sources as
+    // byte offsets (x' + y' sy) * sizeof(T), fixed for the march (the host
+    // checks that a plane's offsets fit in 32 bits)
+    int ex, ey;
+    grown_cell<TL>(tid, ex, ey);
+    const bool inA = tid < TL::EC;
+    const int gx = ((X0 - 1 + ex) % nx + nx) % nx, gy = ((Y0 - 1 + ey) % ny + ny) % ny;
+    unsigned srcA[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int sx = (gx - (a - 1) + nx) % nx, sy = (gy - (c - 1) + ny) % ny;
+            srcA[a][c] = (unsigned)(sx + (long long)sy * vi.sy) * (unsigned)sizeof(T);
+        }
+    // phase B: one output cell
+    const int ox = tid % TL::TX, oy = tid / TL::TX;
+    const bool inB = tid < TL::TC && X0 + ox < nx && Y0 + oy < ny;
+    const int own = ox + 1 + TL::SX * oy;
+    const unsigned dstB =
+        inB ? (unsigned)((X0 + ox) + (long long)(Y0 + oy) * vo.sy) * (unsigned)sizeof(T) : 0u;
+    auto loadA = [&](float (&P)[L::Q], int p) {
+        if (!inA) return;
+        const int pw = (p % nz + nz) % nz;
+        const T* plane[3];
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci) plane[ci] = fin + (long long)(((pw - (ci - 1)) % nz + nz) % nz) * vi.sz;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
+            P[q] = to_f(*at_bytes(plane[cz + 1] + q * vi.sq, srcA[cx + 1][cy + 1]));
+        });
+    };
+    auto storeA = [&](const float (&P)[L::Q], int p) {
+        const Pos pw = Rings::plane(p, Z0);
+        if (!inA) return;
+        float o[L::Q];
+        collide_dev<L, FORCED>(P, o, omega, F0, F1, F2);
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int dx = ex - 1 + L::c(q, 0), dy = ey - 1 + L::c(q, 1);
+            U* dst = (dx >= 0 && dx < TL::TX && dy >= 0 && dy < TL::TY)
+                         ? &lds.template at<q>(pw, dx + 1 + TL::SX * dy)
+                         : &dummy[ex];
+            *dst = f_to_word<T>(o[q]);
+        });
+    };
+    auto phaseB = [&](int z) {
+        const Pos pr = Rings::pulled_by(z, Z0);
+        if (!inB) return;
+        float d[L::Q];
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            d[q] = word_to_f<T>(lds.template at<q>(pr, own));
+        });
+        float o[L::Q];
+        collide_dev<L, FORCED>(d, o, omega, F0, F1, F2);
+        T* const orow = fout + (long long)z * vo.sz;
+        for_q<L>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const T sv = from_f<T>(o[q]);
+            T* p = at_bytes(orow + q * vo.sq, dstB);
+            if constexpr (sizeof(T) == 4) __builtin_nontemporal_store(sv, p);
+            else *p = sv;
+        });
+    };
+    float P[L::Q];
+    loadA(P, Z0 - 1);
+    for (int j = Z0 - 2; j < Z1; ++j) {
+        storeA(P, j + 1);
+        if (j + 2 <= Z1) loadA(P, j + 2);
+        __syncthreads();
+        if (j >= Z0) phaseB(j);
+        __syncthreads();
+    }
+}
+
+// Two steps in one pass; the caller has checked a fully periodic 3-D box and
+// chosen TZ, the z-planes per block.  FP32 storage only: with FP16 and BF16
+// words the pass moves half the bytes but runs the same arithmetic, which
+// then bounds it, and it measured slower than two single steps on MI355X
+// (D3Q27 384^3 forced: 1.27 against 1.09 ms per step at its best tile).
+// k_step2p_dev on 64 x 8 columns (D3Q27: 114 KiB of LDS; D3Q19: 80 KiB):
+// D3Q27 384^3 forced 1.32 ms per step and D3Q19 256^3 0.284 ms, against
+// 1.56 and 0.305 ms for the grown-column kernel it replaced (D3Q27 32 x 8
+// with a ring of four whole planes, D3Q19 64 x 4 with per-c_z rings).
 int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, double omega,
               const double* force, int TZ, hipStream_t st)
 {
@@ -609,7 +603,7 @@ int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doub
     if (lat == MFA_D2Q9) {
         using L = Lattice<MFA_D2Q9>;
         using T = float;
-        using TL = Step2DevTile<64, 4, 3>;
+        using TL = Step2DevTile<64, 4>;
         const dim3 grid((g.n[0] + TL::TX - 1) / TL::TX, (g.n[1] + TL::TY - 1) / TL::TY);
         const float om = (float)omega;
         const float F0 = force ? (float)force[0] : 0.f, F1 = force ? (float)force[1] : 0.f;
@@ -635,7 +629,7 @@ int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doub
             return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: 3-D lattices only");
         } else {
             using T = float;
-            using TL = std::conditional_t<L::Q == 27, Step2DevFit<L, T, 32, 8, 4>, Step2DevFit<L, T, 64, 4, 0>>;
+            using TL = PushTile<8>;
             const dim3 grid((g.n[0] + TL::TX - 1) / TL::TX, (g.n[1] + TL::TY - 1) / TL::TY,
                             (g.n[2] + TZ - 1) / TZ);
             const float om = (float)omega;
@@ -644,10 +638,10 @@ int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doub
             const T* a = (const T*)fin->data;
             T* b = (T*)fout->data;
             if (force)
-                hipLaunchKernelGGL((k_step2_dev<L, true, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g,
+                hipLaunchKernelGGL((k_step2p_dev<L, true, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g,
                                    vi, vo, om, F0, F1, F2, TZ);
             else
-                hipLaunchKernelGGL((k_step2_dev<L, false, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g,
+                hipLaunchKernelGGL((k_step2p_dev<L, false, T, TL>), grid, dim3(TL::THREADS), 0, st, a, b, g,
                                    vi, vo, om, F0, F1, F2, TZ);
             MFA_HIP_CHECK(hipGetLastError());
             return (int)MFA_OK;

@@ -8,7 +8,7 @@ launches of the workload's step kernel:
                two steps per launch (k_batch_step2)
   tgv27        D3Q27 Taylor-Green vortex 384^3, FP64, two steps per launch
                (k_step2p)
-  tgv27_f32    the same stored as FP32 deviations (k_step2_dev)
+  tgv27_f32    the same stored as FP32 deviations (k_step2p_dev)
   level64      D3Q19 256^3 FP64 forced, 64 boxes of 64^3 (k_level_step and
                the ghost exchange)
 
@@ -70,7 +70,7 @@ def _tgv(st, launches, N=384):
         abi.step2(fa, fb, g, omega)
         fa, fb = fb, fa
     torch.cuda.synchronize()
-    return {"kernel": "k_step2p" if st == "f64" else "k_step2_dev", "cells": N ** 3, "q": 27,
+    return {"kernel": "k_step2p" if st == "f64" else "k_step2p_dev", "cells": N ** 3, "q": 27,
             "elem": elem, "steps_per_launch": 2, "copy_bytes": N ** 3 * 27 * elem,
             "workload": f"D3Q27 TGV {N}^3, Re 800, {'FP64' if st == 'f64' else 'FP32 deviations'}"}
 
