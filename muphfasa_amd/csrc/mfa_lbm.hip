@@ -926,6 +926,14 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
         MFA_HIP_CHECK(hipGetLastError());
         return (int)MFA_OK;
     }
+    // in-plane byte offsets are 32-bit in the 3-D pair kernels (k_step2p,
+    // k_step2p_dev); checked before any device query
+    {
+        const long long es = fin->dtype == MFA_F64 ? 8 : (fin->dtype == MFA_F32 ? 4 : 2);
+        for (const mfa_field* f : {fin, (const mfa_field*)fout})
+            if (((long long)(f->n[0] - 1) + (long long)(f->n[1] - 1) * f->sy + 1) * es > 0xffffffffLL)
+                return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: a z-plane spans more than 4 GiB (use mfa_step)");
+    }
     if (fin->dtype != MFA_F64) {
         if (geo.walls)
             return fail(MFA_ERR_UNSUPPORTED,
@@ -937,10 +945,6 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
         const int TZ = step2_planes(((geo.n[0] + 63) / 64) * ((geo.n[1] + 7) / 8), geo.n[2], ncu);
         return step2_dev(fin, fout, geo, lat, omega, force, TZ, (hipStream_t)stream);
     }
-    // in-plane byte offsets are 32-bit in k_step2p
-    for (const mfa_field* f : {fin, (const mfa_field*)fout})
-        if (((long long)(f->n[0] - 1) + (long long)(f->n[1] - 1) * f->sy + 1) * 8 > 0xffffffffLL)
-            return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: a z-plane spans more than 4 GiB (use mfa_step)");
     const View vi = view_of(*fin), vo = view_of(*fout);
     const double F0 = force ? force[0] : 0.0, F1 = force ? force[1] : 0.0, F2 = force ? force[2] : 0.0;
     int ncu = 0;
