@@ -103,10 +103,15 @@ __device__ __forceinline__ void grown_cell(int tid, int& ex, int& ey)
         const int t = tid - T::TC;
         ex = t % T::TX + 1;
         ey = t < T::TX ? 0 : T::TY + 1;
-    } else {
+    } else if (tid < T::EC) {
         const int t = tid - T::TC - 2 * T::TX;
         ex = (t & 1) ? T::TX + 1 : 0;
         ey = t >> 1;
+    } else {
+        // a lane beyond the grown column: the corner cell, whose rows its
+        // wavefront loads anyway (its results go nowhere)
+        ex = 0;
+        ey = 0;
     }
 }
 
@@ -267,7 +272,7 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
     const bool inB = tid < T::TC && X0 + ox < nx && Y0 + oy < ny;
     const int own = ox + 1 + T::SX * oy;    // slot of the output cell
     const unsigned dstB =
-        inB ? (unsigned)((X0 + ox) + (long long)(Y0 + oy) * vo.sy) * (unsigned)sizeof(double) : 0u;
+        inB ? (unsigned)((X0 + ox) + (long long)(Y0 + oy) * vo.sy) * (unsigned)sizeof(double) : kDropOffset;
 
     auto has_plane = [&](int p) { return pz || (p >= 0 && p < nz); };
     // edge flags of a cell of the domain at (x, y, z)
@@ -282,8 +287,14 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
         }
         return edge;
     };
+    // Without walls every lane of phase A loads and collides: the source
+    // offsets of a lane beyond the grown column are wrapped into the box like
+    // any other, and push() sends its values to the dummy row.  A branch
+    // around the loads or their use would give the compiler a path on which
+    // a load is still outstanding at the next iteration's waits.
     auto loadA = [&](double (&P)[L::Q], int p) {
-        if (!inA || !has_plane(p)) return;
+        if constexpr (WALLS)
+            if (!inA || !has_plane(p)) return;
         const int pw = (p % nz + nz) % nz;
         const double* plane[3];
 #pragma unroll
@@ -316,14 +327,15 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
     auto push = [&](auto qc, const Pos& ps, double out) {
         constexpr int q = decltype(qc)::value;
         const int dx = ex - 1 + L::c(q, 0), dy = ey - 1 + L::c(q, 1);
-        double* dst = (dx >= 0 && dx < T::TX && dy >= 0 && dy < T::TY)
+        double* dst = (inA && dx >= 0 && dx < T::TX && dy >= 0 && dy < T::TY)
                           ? &lds.template at<q>(ps, dx + 1 + T::SX * dy)
                           : &dummy[ex];
         *dst = out;
     };
     auto storeA = [&](const double (&P)[L::Q], int p) {
         const Pos pw = Rings::plane(p, Z0);
-        if (!inA || !has_plane(p)) return;
+        if constexpr (WALLS)
+            if (!inA || !has_plane(p)) return;
         if constexpr (!WALLS) {
             collide<L, FORCED>(P, omega, F0, F1, F2, [&](auto qc, double out) { push(qc, pw, out); });
         } else {
@@ -342,9 +354,11 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
             }
         }
     };
+    // Phase B runs on the waves of the tile's own rows (tid < TC) for every
+    // lane: a lane whose cell lies outside the box reads its (unused) slots
+    // and its stores are dropped, so no path skips the stores.
     auto phaseB = [&](int z) {
         const Pos pr = Rings::pulled_by(z, Z0);
-        if (!inB) return;
         double f[L::Q];
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -353,19 +367,65 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
         double* const orow = fout + (long long)z * vo.sz;
         collide<L, FORCED>(f, omega, F0, F1, F2, [&](auto qc, double out) {
             constexpr int q = decltype(qc)::value;
-            __builtin_nontemporal_store(out, at_bytes(orow + q * vo.sq, dstB));
+            store_nt_or_drop(orow + q * vo.sq, dstB, out);
         });
     };
 
-    // iterations Z0-2 and Z0-1 compute the two planes below the block's range
+    // Iteration j computes phase A of plane j+1 (from loads issued in the
+    // iteration before), issues the loads of plane j+2, and after a barrier
+    // phase B of plane j; a second barrier closes it.  The iterations before
+    // Z0 (no phase B) and the first with phase B are peeled off the loop, and
+    // the rim waves (tid >= TC: rows and columns outside the tile) run the
+    // same schedule without phase B in a loop of their own: then on every
+    // path into the loop the phase-B stores lie between a load and its use,
+    // so the compiler's wait for the load leaves those stores in flight.
     double P[L::Q];
     loadA(P, Z0 - 1);
-    for (int j = Z0 - 2; j < Z1; ++j) {
-        storeA(P, j + 1);
-        if (j + 2 <= Z1) loadA(P, j + 2);
+#pragma unroll 1
+    for (int p = Z0 - 1; p <= Z0; ++p) {
+        storeA(P, p);
+        loadA(P, p + 1);
         __syncthreads();
-        if (j >= Z0) phaseB(j);
         __syncthreads();
+    }
+    // TC is a multiple of 64, so the test is uniform over a wavefront; taken
+    // from the first lane, the compiler branches on it instead of running
+    // both sides under lane masks one after the other
+    if (__builtin_amdgcn_readfirstlane(tid) < T::TC) {
+        storeA(P, Z0 + 1);
+        if (Z0 + 2 <= Z1) loadA(P, Z0 + 2);
+        __syncthreads();
+        phaseB(Z0);
+        __syncthreads();
+#pragma unroll 1
+        for (int j = Z0 + 1; j < Z1 - 1; ++j) {
+            storeA(P, j + 1);
+            loadA(P, j + 2);
+            __syncthreads();
+            phaseB(j);
+            __syncthreads();
+        }
+        if (Z1 - 1 > Z0) {
+            storeA(P, Z1);
+            __syncthreads();
+            phaseB(Z1 - 1);
+        }
+    } else {
+        storeA(P, Z0 + 1);
+        if (Z0 + 2 <= Z1) loadA(P, Z0 + 2);
+        __syncthreads();
+        __syncthreads();
+#pragma unroll 1
+        for (int j = Z0 + 1; j < Z1 - 1; ++j) {
+            storeA(P, j + 1);
+            loadA(P, j + 2);
+            __syncthreads();
+            __syncthreads();
+        }
+        if (Z1 - 1 > Z0) {
+            storeA(P, Z1);
+            __syncthreads();
+        }
     }
 }
 

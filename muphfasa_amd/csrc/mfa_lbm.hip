@@ -927,11 +927,11 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
         return (int)MFA_OK;
     }
     // in-plane byte offsets are 32-bit in the 3-D pair kernels (k_step2p,
-    // k_step2p_dev); checked before any device query
+    // k_step2p_dev) and end below kDropOffset; checked before any device query
     {
         const long long es = fin->dtype == MFA_F64 ? 8 : (fin->dtype == MFA_F32 ? 4 : 2);
         for (const mfa_field* f : {fin, (const mfa_field*)fout})
-            if (((long long)(f->n[0] - 1) + (long long)(f->n[1] - 1) * f->sy + 1) * es > 0xffffffffLL)
+            if (((long long)(f->n[0] - 1) + (long long)(f->n[1] - 1) * f->sy + 1) * es > (long long)kDropOffset)
                 return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: a z-plane spans more than 4 GiB (use mfa_step)");
     }
     if (fin->dtype != MFA_F64) {
