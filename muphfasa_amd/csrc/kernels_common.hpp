@@ -189,14 +189,14 @@ __device__ __forceinline__ W* at_bytes(W* base, unsigned off)
     return (W*)((char*)base + off);
 }
 
-// Non-temporal store of one 8-byte word at a 32-bit byte offset from a
+// Non-temporal store of one 8- or 4-byte word at a 32-bit byte offset from a
 // wave-uniform base, through a buffer resource whose range ends at
 // kDropOffset: a lane that has nothing to write passes kDropOffset and the
 // hardware drops its store (out-of-range buffer writes are discarded).  A
 // loop of such stores has no branch around them, so every path from a load
 // to its first use passes the stores issued after the load, and the
 // compiler's wait for the load need not also wait for them.  Valid offsets
-// end at or below kDropOffset - 8.
+// end at or below kDropOffset - sizeof(word).
 constexpr unsigned kDropOffset = 0xFFFFFFF0u;
 __device__ __forceinline__ void store_nt_or_drop(double* base, unsigned off, double v)
 {
@@ -207,6 +207,12 @@ __device__ __forceinline__ void store_nt_or_drop(double* base, unsigned off, dou
         __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)kDropOffset, 0x00020000);
     // cache policy 2: non-temporal, as __builtin_nontemporal_store
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)off, 0, 2);
+}
+__device__ __forceinline__ void store_nt_or_drop(float* base, unsigned off, float v)
+{
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)kDropOffset, 0x00020000);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 2);
 }
 
 // Element offsets of the pull-source rows of one cell whose y and z are the

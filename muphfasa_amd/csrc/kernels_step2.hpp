@@ -22,9 +22,10 @@
 // outside the domain.  Per cell the arithmetic is that of k_step (pull,
 // collide, wall values), so a pass equals two single steps bit for bit.
 //
-// Schedule.  Iteration j computes phase A of plane j+1 (from loads issued in
-// the iteration before), issues the loads of plane j+2, and after a barrier
-// computes phase B of plane j; a second barrier closes the iteration.
+// Schedule (pair_schedule).  Iteration j computes phase A of plane j+1 (from
+// loads issued in the iteration before), issues the loads of plane j+2, and
+// after a barrier computes phase B of plane j; a second barrier closes the
+// iteration.
 #pragma once
 
 #include "kernels_common.hpp"
@@ -171,6 +172,71 @@ __device__ __forceinline__ void wall_fill(const double (&s)[L::Q], const G& g, c
         }
         emit(qc, v);
     });
+}
+
+// The march of a pair kernel over planes Z0-1 .. Z1 (phase A) and Z0 .. Z1-1
+// (phase B).  Iteration j computes phase A of plane j+1 (from loads issued
+// in the iteration before), issues the loads of plane j+2, and after a
+// barrier phase B of plane j; a second barrier closes it.  The iterations
+// before Z0 (no phase B) and the first with phase B are peeled off the loop,
+// and the rim waves (tid >= TC: rows and columns outside the tile) run the
+// same schedule without phase B in a loop of their own: then on every path
+// into the loop the phase-B stores lie between a load and its use, so the
+// compiler's wait for the load leaves those stores in flight.  loadA(P, p)
+// and storeA(P, p) are phase A's loads and its collide-and-push of plane p,
+// phaseB(z) phase B of plane z; phaseB must not branch around its stores.
+template <int TC, class Regs, class LoadA, class StoreA, class PhaseB>
+__device__ __forceinline__ void pair_schedule(int tid, int Z0, int Z1, Regs& P, LoadA&& loadA,
+                                              StoreA&& storeA, PhaseB&& phaseB)
+{
+    static_assert(TC % 64 == 0, "the tile/rim split must fall between wavefronts");
+    loadA(P, Z0 - 1);
+#pragma unroll 1
+    for (int p = Z0 - 1; p <= Z0; ++p) {
+        storeA(P, p);
+        loadA(P, p + 1);
+        __syncthreads();
+        __syncthreads();
+    }
+    // uniform over a wavefront; taken from the first lane, the compiler
+    // branches on it instead of running both sides under lane masks one
+    // after the other
+    if (__builtin_amdgcn_readfirstlane(tid) < TC) {
+        storeA(P, Z0 + 1);
+        if (Z0 + 2 <= Z1) loadA(P, Z0 + 2);
+        __syncthreads();
+        phaseB(Z0);
+        __syncthreads();
+#pragma unroll 1
+        for (int j = Z0 + 1; j < Z1 - 1; ++j) {
+            storeA(P, j + 1);
+            loadA(P, j + 2);
+            __syncthreads();
+            phaseB(j);
+            __syncthreads();
+        }
+        if (Z1 - 1 > Z0) {
+            storeA(P, Z1);
+            __syncthreads();
+            phaseB(Z1 - 1);
+        }
+    } else {
+        storeA(P, Z0 + 1);
+        if (Z0 + 2 <= Z1) loadA(P, Z0 + 2);
+        __syncthreads();
+        __syncthreads();
+#pragma unroll 1
+        for (int j = Z0 + 1; j < Z1 - 1; ++j) {
+            storeA(P, j + 1);
+            loadA(P, j + 2);
+            __syncthreads();
+            __syncthreads();
+        }
+        if (Z1 - 1 > Z0) {
+            storeA(P, Z1);
+            __syncthreads();
+        }
+    }
 }
 
 template <class L, bool FORCED, bool WALLS, class T>
@@ -371,62 +437,8 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
         });
     };
 
-    // Iteration j computes phase A of plane j+1 (from loads issued in the
-    // iteration before), issues the loads of plane j+2, and after a barrier
-    // phase B of plane j; a second barrier closes it.  The iterations before
-    // Z0 (no phase B) and the first with phase B are peeled off the loop, and
-    // the rim waves (tid >= TC: rows and columns outside the tile) run the
-    // same schedule without phase B in a loop of their own: then on every
-    // path into the loop the phase-B stores lie between a load and its use,
-    // so the compiler's wait for the load leaves those stores in flight.
     double P[L::Q];
-    loadA(P, Z0 - 1);
-#pragma unroll 1
-    for (int p = Z0 - 1; p <= Z0; ++p) {
-        storeA(P, p);
-        loadA(P, p + 1);
-        __syncthreads();
-        __syncthreads();
-    }
-    // TC is a multiple of 64, so the test is uniform over a wavefront; taken
-    // from the first lane, the compiler branches on it instead of running
-    // both sides under lane masks one after the other
-    if (__builtin_amdgcn_readfirstlane(tid) < T::TC) {
-        storeA(P, Z0 + 1);
-        if (Z0 + 2 <= Z1) loadA(P, Z0 + 2);
-        __syncthreads();
-        phaseB(Z0);
-        __syncthreads();
-#pragma unroll 1
-        for (int j = Z0 + 1; j < Z1 - 1; ++j) {
-            storeA(P, j + 1);
-            loadA(P, j + 2);
-            __syncthreads();
-            phaseB(j);
-            __syncthreads();
-        }
-        if (Z1 - 1 > Z0) {
-            storeA(P, Z1);
-            __syncthreads();
-            phaseB(Z1 - 1);
-        }
-    } else {
-        storeA(P, Z0 + 1);
-        if (Z0 + 2 <= Z1) loadA(P, Z0 + 2);
-        __syncthreads();
-        __syncthreads();
-#pragma unroll 1
-        for (int j = Z0 + 1; j < Z1 - 1; ++j) {
-            storeA(P, j + 1);
-            loadA(P, j + 2);
-            __syncthreads();
-            __syncthreads();
-        }
-        if (Z1 - 1 > Z0) {
-            storeA(P, Z1);
-            __syncthreads();
-        }
-    }
+    pair_schedule<T::TC>(tid, Z0, Z1, P, loadA, storeA, phaseB);
 }
 
 // ------------------------------------------------------------ wall shell --

@@ -486,13 +486,15 @@ __global__ __launch_bounds__(TL::THREADS) void k_step2_2d_dev(const T* __restric
 // step stores it, into the LDS slot of the tile cell that pulls it in the
 // second step (per-c_z rings of 32-bit words); phase B reads its own cell's Q
 // words, collides and writes the second step.  Loads address wave-uniform
-// plane bases plus fixed 32-bit lane offsets.  Per cell the arithmetic is
+// plane bases plus fixed 32-bit lane offsets; the march is pair_schedule's,
+// with the same branch-free phases as k_step2p.  Per cell the arithmetic is
 // k_step_dev's (collide_dev), so a pass equals two single steps bit for bit.
 template <class L, bool FORCED, class T, class TL>
 __global__ __launch_bounds__(TL::THREADS) __attribute__((amdgpu_waves_per_eu(TL::WAVES_PER_EU)))
 void k_step2p_dev(const T* __restrict__ fin, T* __restrict__ fout, const Geo g, const View vi, const View vo,
                   const float omega, const float F0, const float F1, const float F2, const int TZ)
 {
+    static_assert(sizeof(T) == 4, "FP32 storage only");
     using U = typename Word<T>::U;
     using Rings = PushRings<L, TL::SC, false, U>;
     using Pos = typename Rings::Pos;
@@ -528,9 +530,10 @@ void k_step2p_dev(const T* __restrict__ fin, T* __restrict__ fout, const Geo g, 
     const bool inB = tid < TL::TC && X0 + ox < nx && Y0 + oy < ny;
     const int own = ox + 1 + TL::SX * oy;
     const unsigned dstB =
-        inB ? (unsigned)((X0 + ox) + (long long)(Y0 + oy) * vo.sy) * (unsigned)sizeof(T) : 0u;
+        inB ? (unsigned)((X0 + ox) + (long long)(Y0 + oy) * vo.sy) * (unsigned)sizeof(T) : kDropOffset;
+    // every lane loads and collides (a lane beyond the grown column takes
+    // the corner cell and pushes to the dummy row), as in k_step2p
     auto loadA = [&](float (&P)[L::Q], int p) {
-        if (!inA) return;
         const int pw = (p % nz + nz) % nz;
         const T* plane[3];
 #pragma unroll
@@ -543,13 +546,12 @@ void k_step2p_dev(const T* __restrict__ fin, T* __restrict__ fout, const Geo g, 
     };
     auto storeA = [&](const float (&P)[L::Q], int p) {
         const Pos pw = Rings::plane(p, Z0);
-        if (!inA) return;
         float o[L::Q];
         collide_dev<L, FORCED>(P, o, omega, F0, F1, F2);
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const int dx = ex - 1 + L::c(q, 0), dy = ey - 1 + L::c(q, 1);
-            U* dst = (dx >= 0 && dx < TL::TX && dy >= 0 && dy < TL::TY)
+            U* dst = (inA && dx >= 0 && dx < TL::TX && dy >= 0 && dy < TL::TY)
                          ? &lds.template at<q>(pw, dx + 1 + TL::SX * dy)
                          : &dummy[ex];
             *dst = f_to_word<T>(o[q]);
@@ -557,7 +559,6 @@ void k_step2p_dev(const T* __restrict__ fin, T* __restrict__ fout, const Geo g, 
     };
     auto phaseB = [&](int z) {
         const Pos pr = Rings::pulled_by(z, Z0);
-        if (!inB) return;
         float d[L::Q];
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -568,21 +569,11 @@ void k_step2p_dev(const T* __restrict__ fin, T* __restrict__ fout, const Geo g, 
         T* const orow = fout + (long long)z * vo.sz;
         for_q<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const T sv = from_f<T>(o[q]);
-            T* p = at_bytes(orow + q * vo.sq, dstB);
-            if constexpr (sizeof(T) == 4) __builtin_nontemporal_store(sv, p);
-            else *p = sv;
+            store_nt_or_drop(orow + q * vo.sq, dstB, from_f<T>(o[q]));
         });
     };
     float P[L::Q];
-    loadA(P, Z0 - 1);
-    for (int j = Z0 - 2; j < Z1; ++j) {
-        storeA(P, j + 1);
-        if (j + 2 <= Z1) loadA(P, j + 2);
-        __syncthreads();
-        if (j >= Z0) phaseB(j);
-        __syncthreads();
-    }
+    pair_schedule<TL::TC>(tid, Z0, Z1, P, loadA, storeA, phaseB);
 }
 
 // Two steps in one pass; the caller has checked a fully periodic 3-D box and
@@ -621,7 +612,7 @@ int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, doub
     }
     // in-plane byte offsets are 32-bit in the kernel
     for (const mfa_field* f : {fin, (const mfa_field*)fout})
-        if (((long long)(f->n[0] - 1) + (long long)(f->n[1] - 1) * f->sy + 1) * 4 > 0xffffffffLL)
+        if (((long long)(f->n[0] - 1) + (long long)(f->n[1] - 1) * f->sy + 1) * 4 > (long long)kDropOffset)
             return fail(MFA_ERR_UNSUPPORTED, "mfa_step2: a z-plane spans more than 4 GiB (use mfa_step)");
     return with_lattice_dev(lat, [&](auto Lc) {
         using L = decltype(Lc);
