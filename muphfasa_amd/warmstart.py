@@ -307,6 +307,7 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
             summary[f"{mode}_steps_ratio_of_sums"] = (
                 float(cc["steps"][both].sum() / max(rm["steps"][both].sum(), 1)) if both.any() else None)
             summary[f"{mode}_time_ratio_median"] = float(np.median(tratio)) if both.any() else None
+            summary[f"{mode}_time_ratio_ge3_frac"] = float(np.mean(tratio >= 3.0)) if both.any() else None
             summary[f"{mode}_wall_ratio"] = round(cc["wall_s"] / rm["wall_s"], 3)
             summary[f"{mode}_lu_ratio"] = round(cc["lattice_updates"] / max(rm["lattice_updates"], 1), 3)
             summary[f"{mode}_inference_s"] = round(rm["inference_s"], 3)
@@ -327,6 +328,8 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
                     if sel.any() else None,
                     "cold_time_median_s": float(np.median(cc["t_case"][sel])) if sel.any() else None,
                     "start_time_median_s": float(np.median(rm["t_case"][sel])) if sel.any() else None,
+                    "pred_err_l2_median": float(np.nanmedian(per_case["pred_err_l2"][inb]))
+                    if "pred_err_l2" in per_case and inb.any() else None,
                 })
             bands[mode] = rows
         summary["bands"] = bands
