@@ -293,6 +293,72 @@ def test_step2_equals_two_oracle_steps(dev, lat, si, forced, layout, bc):
     assert np.array_equal(got, O.valid(L, expect))
 
 
+@pytest.mark.parametrize("lat", ["D3Q19", "D3Q27"])
+@pytest.mark.parametrize("layout", ["component", "row"])
+@pytest.mark.parametrize("dt,bc", [("f64", "periodic"), ("f64", "lid"), ("f32", "periodic")])
+def test_step2_writes_only_the_box(dev, lat, layout, dt, bc):
+    """mfa_step2 on a box held inside a larger buffer (a ghost layer plus
+    extra padding on every side; the box cuts the 64-cell columns in x and
+    the tile rows in y): the box gets the bits of two steps (the oracle's for
+    FP64, two mfa_step calls' for FP32 storage) and every other word of the
+    output buffer keeps its sentinel.  The pair kernels drop the stores of
+    lanes outside the box in the hardware (an offset beyond the store's
+    range) instead of branching around them; this pins that."""
+    import torch
+    L = O.Lattice(lat)
+    lid, did = abi.LATTICE[lat], abi.DTYPE[dt]
+    n = [70, 13, 9]
+    print(f"case step2_box lattice={lat} layout={layout} dtype={dt} bc={bc}", flush=True)
+    g, ref, per = bc_tables(L, bc)
+    F = [2e-5, -1e-5, 3e-6]
+    omega = 1.0 / 0.8
+    f0 = random_state(L, n, seed=zlib.crc32(f"step2_box/{lat}".encode()))
+    e0, e1, e2 = n[0] + 2 + 5, n[1] + 2 + 3, n[2] + 2 + 1
+    tdt = torch.float64 if dt == "f64" else torch.float32
+    if layout == "component":
+        shape = (L.Q, e2, e1, e0)
+        box = (slice(None), slice(1, 1 + n[2]), slice(1, 1 + n[1]), slice(1, 1 + n[0]))
+    else:
+        shape = (e2, e1, L.Q, e0)
+        box = (slice(1, 1 + n[2]), slice(1, 1 + n[1]), slice(None), slice(1, 1 + n[0]))
+    sentinel = -12345.5
+    outside = torch.ones(shape, dtype=torch.bool, device=dev)
+    outside[box] = False
+
+    def padded():
+        return torch.full(shape, sentinel, dtype=tdt, device=dev)
+
+    def fld(t):
+        return abi.field(t, n, 1, 3, did, layout=layout)
+
+    def in_layout(comp):                       # (Q, z, y, x) -> the buffer's order
+        return comp if layout == "component" else comp.permute(1, 2, 0, 3)
+
+    src = to_dev(np.ascontiguousarray(O.valid(L, f0)), dev)
+    fin = padded()
+    if dt == "f64":
+        fin[box] = in_layout(src)
+    else:
+        dev32 = torch.empty((L.Q, n[2], n[1], n[0]), dtype=tdt, device=dev)
+        abi.convert(abi.field(src, n, 0, 3), abi.field(dev32, n, 0, 3, did), lid)
+        fin[box] = in_layout(dev32)
+    fout = padded()
+    abi.step2(fld(fin), fld(fout), g, omega, F)
+    if dt == "f64":
+        _, mid = _oracle_step(L, f0, n, per, ref, omega, F)
+        _, expect = _oracle_step(L, mid, n, per, ref, omega, F)
+        want = in_layout(to_dev(np.ascontiguousarray(O.valid(L, expect)), dev))
+    else:
+        mid, two = padded(), padded()
+        abi.step(fld(fin), fld(mid), g, omega, F)
+        abi.step(fld(mid), fld(two), g, omega, F)
+        want = two[box]
+    torch.cuda.synchronize()
+    word = torch.int64 if dt == "f64" else torch.int32
+    assert torch.equal(fout[box].contiguous().view(word), want.contiguous().view(word))
+    assert bool((fout[outside] == sentinel).all())
+
+
 STEP2_2D_SIZES = [[13, 7], [64, 3], [130, 9], [1, 5], [70, 33]]
 
 
