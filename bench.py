@@ -28,6 +28,13 @@ initial state.  The CPU baseline is the oracle (the reference's algorithm in C,
 boxes of max_grid_size 64 with their own ghost layers, OpenMP over boxes),
 built with -march=native for this host and timed on every core of the
 process's affinity set, on a bounded number of steps of the same workload.
+
+The metric's second half, time to the 1e-8 steady state from the learned
+warm start against the plain start, is measured after the timed region on a
+bounded held-out sample of the warm-start workload (32 cavities of 256^2 by
+default, at most 600 000 steps each, about a minute; --no-warm-start skips
+it) and reported as
+`time_to_steady` beside the MLUPS line (see warm_start_sample).
 """
 from __future__ import annotations
 
@@ -66,7 +73,58 @@ def parse():
     p.add_argument("--layout", default="row", choices=["row", "component"],
                    help="population order in HBM: row-interleaved (the engine's own "
                         "allocation) or the MultiFab's component-major planes")
+    p.add_argument("--no-warm-start", action="store_true",
+                   help="skip the time-to-steady sample of the learned warm start")
+    p.add_argument("--warm-cases", type=int, default=32,
+                   help="held-out cavities in the time-to-steady sample")
+    p.add_argument("--warm-max-steps", type=int, default=600000)
+    p.add_argument("--warm-weights", default=os.path.join(ROOT, "muphfasa_amd", "weights",
+                                                          "warmstart_256.npz"))
     return p.parse_args()
+
+
+def warm_start_sample(n_cases, seed, max_steps, weights):
+    """The metric's second half, time to the 1e-8 steady criterion from the
+    learned warm start against the plain (rest) start, on a bounded held-out
+    sample: n_cases 256^2 D2Q9 cavities with 0-3 obstacles and Re log-uniform
+    in [100, 3200] (the warm-start workload's distribution, from a seed the
+    network was not trained on), FP64 batched engine, the criterion checked
+    every 1 000 steps.  Starts: cold (rest), warm_neq (the network's
+    prediction: equilibrium plus its Chapman-Enskog part and the two-state
+    average, mfa_ensemble_init_start) and exact_neq (the same recipe on the
+    cold run's own converged fields: what a perfect prediction would give).
+    Per case, time = steps x its share of the batch's measured step rate +
+    its share of the inference; ratios are medians over the cases where both
+    starts converged within max_steps."""
+    import torch
+    from muphfasa_amd import surrogate as S
+    from muphfasa_amd import warmstart as WS
+
+    net = S.WarmStartNet(seed=seed)
+    net.load(weights)
+    t0 = time.perf_counter()
+    summ, _ = WS.evaluate(net, n_eval=n_cases, seed=seed, tol=1e-8, max_steps=max_steps,
+                          modes=("cold", "warm_neq", "exact_neq"))
+    wall = time.perf_counter() - t0
+    del net
+    torch.cuda.empty_cache()
+
+    def r3(x):
+        return round(x, 3) if x is not None else None
+
+    return {"unit": "x (median per-case time to 1e-8, plain / warm)",
+            "warm_vs_plain": r3(summ.get("warm_neq_time_ratio_median")),
+            "warm_vs_plain_steps": r3(summ.get("warm_neq_steps_ratio_median")),
+            "exact_fields_vs_plain": r3(summ.get("exact_neq_time_ratio_median")),
+            "cases": n_cases, "cold_converged": summ.get("cold_converged"),
+            "both_converged": summ.get("warm_neq_both_converged"),
+            "network_error_median_rel_l2": r3(summ.get("pred_err_l2_median")),
+            "converged_field_diff_max_over_U": summ.get("warm_neq_field_diff_max"),
+            "inference_s": summ.get("warm_neq_inference_s"),
+            "sample": f"{n_cases} held-out 256^2 cavities (seed {seed}), 0-3 obstacles, Re "
+                      f"log-uniform in [100, 3200], FP64, cap {max_steps} steps; "
+                      f"weights {os.path.relpath(weights, ROOT)}; {wall:.1f} s",
+            "wall_s": round(wall, 1)}
 
 
 def _native_oracle():
@@ -245,10 +303,17 @@ def main():
     prof = traffic_from_profiles(n, args.kernel)
     traffic = prof.get("hbm_bytes_per_launch") if prof else None
 
+    del fa_t, fb_t, cur, nxt, fa, fb
+    torch.cuda.empty_cache()
+    warm = None
+    if not args.no_warm_start:
+        try:
+            warm = warm_start_sample(args.warm_cases, args.seed + 1, args.warm_max_steps,
+                                     args.warm_weights)
+        except Exception as e:  # the MLUPS line is still reported
+            warm = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.no_cpu_baseline:
-        del fa_t, fb_t, cur, nxt, fa, fb
-        torch.cuda.empty_cache()
         cpu = cpu_baseline(n, args.cpu_seconds, args.seed)
 
     line = {
@@ -300,6 +365,7 @@ def main():
         "settle_launches": args.settle,
         "wall_s_timed": round(wall, 4),
         "cpu_baseline": cpu,
+        "time_to_steady": warm,
     }
     print(json.dumps(line), flush=True)
 

@@ -227,8 +227,9 @@ def evaluate(net, n_eval=256, seed=777, tol=1e-8, max_steps=200000, check=1000, 
     torch.cuda.synchronize()
     del ew
     if any(m.startswith("warm") for m in modes):
-        # and of the network kernels, so that the timed inference is steady
-        predict(net, masks_dev[:16], re_dev[:16])
+        # and of the network kernels and the field conversion after them, so
+        # that the timed inference is steady
+        fields_on_device(predict(net, masks_dev[:16], re_dev[:16]), masks_dev[:16])
         torch.cuda.synchronize()
     res = {}
     vel_pred = None
