@@ -132,7 +132,11 @@ int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g,
  * MFA_F32 storage: D2Q9 boxes with or without walls and fully periodic 3-D
  * boxes (the intermediate state rounded to FP32 as a single step stores it).
  * Anything else returns MFA_ERR_UNSUPPORTED and the caller uses mfa_step:
- * 3-D boxes with walls in FP32, and FP16 / BF16 storage, where the pass
+ * 3-D boxes with walls in FP32, 3-D boxes whose z-plane (all populations of
+ * one z, from the first cell to the last) spans more than 4 GiB less 16
+ * bytes (the pass addresses a plane through 32-bit offsets, and stores of
+ * lanes outside the box go to an offset beyond that range, where the
+ * hardware drops them), and FP16 / BF16 storage, where the pass
  * measured slower than two single steps (the words halve the bytes but not
  * the arithmetic, which then bounds the pass).  No counterpart in the
  * reference. */
