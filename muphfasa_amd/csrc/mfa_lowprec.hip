@@ -582,7 +582,7 @@ void k_step2p_dev(const T* __restrict__ fin, T* __restrict__ fout, const Geo g, 
 // then bounds it, and it measured slower than two single steps on MI355X
 // (D3Q27 384^3 forced: 1.27 against 1.09 ms per step at its best tile).
 // k_step2p_dev on 64 x 8 columns (D3Q27: 114 KiB of LDS; D3Q19: 80 KiB):
-// D3Q27 384^3 forced 1.32 ms per step and D3Q19 256^3 0.284 ms, against
+// D3Q27 384^3 forced 1.33 ms per step and D3Q19 256^3 0.268 ms, against
 // 1.56 and 0.305 ms for the grown-column kernel it replaced (D3Q27 32 x 8
 // with a ring of four whole planes, D3Q19 64 x 4 with per-c_z rings).
 int step2_dev(const mfa_field* fin, mfa_field* fout, const Geo& g, int lat, double omega,
