@@ -1,7 +1,8 @@
 """Host (C++) surface on the CPU: header/exports, the inputs parser and the
 UnitConverter chain against the reference's own numbers
 (tests/unit/test_units.cpp, the lid2d and uniform_force inputs, the
-configurations' tau/nu), and the abort conditions of InputParser/Units."""
+configurations' tau/nu), and the abort conditions of InputParser/Units; and
+the benchmark's refusal of more than one GPU."""
 import os
 import re
 
@@ -284,3 +285,16 @@ def test_restart_refuses_malformed_reference_checkpoints(tmp_path):
     assert r.returncode == 0 and "resumes at step 21" in r.stdout, r.stdout
     r = _inputinfo(inp, *args, f"amr.restart={bad}")
     assert r.returncode == 1 and "grown by the ghost width" in r.stdout, r.stdout
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus 2 exits with status 2 and a JSON line
+    that says so, before it touches a device."""
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]

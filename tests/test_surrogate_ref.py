@@ -74,3 +74,20 @@ def test_warmstart_architecture_size_and_shapes():
     assert out.shape == (1, 32, 32, 8)
     for name, (a, b) in S.SKIPS.items():
         assert act[name].shape == act[b].shape
+
+
+def test_shipped_weights_match_the_network():
+    """The trained weights shipped with the package (bench.py's time-to-steady
+    sample loads them) hold, for every layer of the network, an FP32 weight
+    matrix of cout x 9*cin and a bias of cout, all finite, and nothing else."""
+    import os
+    from muphfasa_amd import surrogate as S
+    path = os.path.join(os.path.dirname(S.__file__), "weights", "warmstart_256.npz")
+    z = np.load(path)
+    names = [n for n, *_ in S.LAYERS]
+    assert set(z.files) == {f"{n}.w" for n in names} | {f"{n}.b" for n in names}
+    for n, cin, cout, *_ in S.LAYERS:
+        w, b = z[f"{n}.w"], z[f"{n}.b"]
+        assert w.shape == (cout, 9 * cin) and w.dtype == np.float32, n
+        assert b.shape == (cout,) and b.dtype == np.float32, n
+        assert np.isfinite(w).all() and np.isfinite(b).all(), n
