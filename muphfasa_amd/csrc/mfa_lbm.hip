@@ -1014,9 +1014,10 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
     };
     // Columns (D3Q19 256^3 forced, ms per step): periodic 64 x 8, 0.513,
     // against 0.554 for 64 x 4 and 0.577 for the grown-column layout of
-    // 64 x 4; D3Q15 64 x 8, 0.379; D3Q27 (27 rings of planes) 64 x 4, 2.81
-    // at 384^3.  With walls class 0 needs a second ring slot and the wall
-    // values hold the cell's populations in registers: 64 x 4.
+    // 64 x 4; D3Q15 64 x 8, 0.379; D3Q27 (27 rings of planes, 143 KB of
+    // LDS) 64 x 5, 1.8 % faster than 64 x 4 (2.68 ms) at 384^3.  With walls
+    // class 0 needs a second ring slot and the wall values hold the cell's
+    // populations in registers: 64 x 4.
     using Yes = std::true_type;
     using No = std::false_type;
     bool shell = geo.walls != 0;
@@ -1025,7 +1026,7 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
     if (shell) {
         if (lat == MFA_D3Q15) return run_shell(Lattice<MFA_D3Q15>{}, PushTile<8>{});
         if (lat == MFA_D3Q19) return run_shell(Lattice<MFA_D3Q19>{}, PushTile<8>{});
-        return run_shell(Lattice<MFA_D3Q27>{}, PushTile<4>{});
+        return run_shell(Lattice<MFA_D3Q27>{}, PushTile<5>{});
     }
     if (geo.walls) {
         if (lat == MFA_D3Q15) return run(Lattice<MFA_D3Q15>{}, PushTile<4>{}, Yes{});
@@ -1034,7 +1035,7 @@ int mfa_step2(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, doubl
     }
     if (lat == MFA_D3Q15) return run(Lattice<MFA_D3Q15>{}, PushTile<8>{}, No{});
     if (lat == MFA_D3Q19) return run(Lattice<MFA_D3Q19>{}, PushTile<8>{}, No{});
-    return run(Lattice<MFA_D3Q27>{}, PushTile<4>{}, No{});
+    return run(Lattice<MFA_D3Q27>{}, PushTile<5>{}, No{});
 }
 
 int mfa_step(const mfa_field* fin, mfa_field* fout, const mfa_geom_bc* g, double omega,
