@@ -165,6 +165,49 @@ def test_step2_equals_two_steps_at_256cubed(dev):
     assert torch.equal(tb, tc)
 
 
+@pytest.mark.parametrize("dt,side", [("f64", 5300), ("f32", 7500)])
+def test_step2_at_the_largest_plane_it_accepts(dev, dt, side):
+    """mfa_step2 on a D3Q19 box (row order) whose z-plane spans just under the
+    4 GiB less 16 bytes it accepts: side x side x 3 cells, 4.27e9 bytes per
+    plane in FP64 (5300^2) and in FP32 storage (7500^2), rows and columns cut
+    by the tiles.  The pair's 32-bit byte offsets reach the top of their range
+    and stay below the offset of the dropped stores; the pass equals two
+    mfa_step calls bit for bit."""
+    import torch
+    n = [side, side, 3]
+    span = (n[0] + (n[1] - 1) * 19 * n[0]) * (8 if dt == "f64" else 4)
+    assert 0xF0000000 < span <= 0xFFFFFFF0
+    print(f"case step2 largest plane dtype={dt} n={n} span={span}", flush=True)
+    did = abi.DTYPE[dt]
+    tdt = torch.float64 if dt == "f64" else torch.float32
+    F = [1e-6, 0.0, 0.0]
+    omega = 1.0 / 0.8
+    g = abi.resolve_bc(2, [abi.BC["Periodic"]] * 6)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    shape = abi.row_shape(19, n, 0, 3)
+    ta = torch.empty(shape, dtype=tdt, device=dev)
+    fa = abi.field(ta, n, 0, 3, did, layout="row")
+    rho = torch.ones((1, n[2], n[1], n[0]), dtype=torch.float64, device=dev)
+    vel = (torch.rand((3, n[2], n[1], n[0]), dtype=torch.float64, device=dev, generator=gen) * 2 - 1) * 1e-2
+    abi.init_equilibrium(abi.field(rho, n, 0, 3), abi.field(vel, n, 0, 3), fa, 2, F)
+    del rho, vel
+    tb = torch.empty_like(ta)
+    tc = torch.empty_like(ta)
+    fb = abi.field(tb, n, 0, 3, did, layout="row")
+    fc = abi.field(tc, n, 0, 3, did, layout="row")
+    abi.step(fa, fb, g, omega, F)
+    abi.step(fb, fc, g, omega, F)
+    tb.fill_(float("nan"))
+    abi.step2(fa, fb, g, omega, F)
+    torch.cuda.synchronize()
+    word = torch.int64 if dt == "f64" else torch.int32
+    same = torch.equal(tb.view(word), tc.view(word))
+    del ta, tb, tc
+    torch.cuda.empty_cache()
+    assert same
+
+
 @pytest.mark.parametrize("lat", ["D3Q15", "D3Q19", "D3Q27"])
 @pytest.mark.parametrize("bc", ["lid", "xper_walls", "inflow_pressure"])
 def test_walled_step2_equals_two_steps_at_config_scale(dev, lat, bc):
