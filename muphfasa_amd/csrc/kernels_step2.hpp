@@ -32,6 +32,21 @@
 
 namespace mfa {
 
+// fn(q) for every population, those with c_z = -1 first, then c_z = 0, then
+// c_z = +1 (each class in population order).
+template <class L, class Fn>
+__device__ __forceinline__ void for_cz(Fn&& fn)
+{
+    auto cls = [&](auto czc) {
+        for_q<L>([&](auto qc) {
+            if constexpr (L::c(decltype(qc)::value, 2) == decltype(czc)::value) fn(qc);
+        });
+    };
+    cls(std::integral_constant<int, -1>{});
+    cls(std::integral_constant<int, 0>{});
+    cls(std::integral_constant<int, 1>{});
+}
+
 // Rings of planes by c_z class k = c_z + 1.  Phase B of plane z reads class 0
 // from source plane z+1 (written in the same iteration, before the barrier),
 // class 1 from plane z (one iteration earlier) and class 2 from plane z-1
@@ -370,7 +385,10 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
             else if (sz >= nz) sz = pz ? 0 : nz - 1;
             plane[ci] = fin + (long long)sz * vi.sz;
         }
-        for_q<L>([&](auto qc) {
+        // issued plane by plane (z-1, z, z+1): the requests of one source
+        // plane leave together (D3Q19 256^3: 0.6 % faster than population
+        // order; the FP32 pair measured 1 % slower so, and keeps it)
+        for_cz<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
             P[q] = *at_bytes(plane[cz + 1] + q * vi.sq, srcA[cx + 1][cy + 1]);
