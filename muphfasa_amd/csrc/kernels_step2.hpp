@@ -387,7 +387,8 @@ void k_step2p(const double* __restrict__ fin, double* __restrict__ fout, const G
         }
         // issued plane by plane (z-1, z, z+1): the requests of one source
         // plane leave together (D3Q19 256^3: 0.6 % faster than population
-        // order; the FP32 pair measured 1 % slower so, and keeps it)
+        // order; k_step2p_dev keeps population order, which measured 1 %
+        // faster for FP32 words)
         for_cz<L>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
