@@ -9,6 +9,9 @@ launches of the workload's step kernel:
   tgv27        D3Q27 Taylor-Green vortex 384^3, FP64, two steps per launch
                (k_step2p)
   tgv27_f32    the same stored as FP32 deviations (k_step2p_dev)
+  tgv27_f16    the same stored as FP16 deviations, one step per launch
+               (k_step_dev)
+  tgv27_bf16   the same stored as BF16 deviations, one step per launch
   level64      D3Q19 256^3 FP64 forced, 64 boxes of 64^3 (k_level_step and
                the ghost exchange)
 
@@ -54,7 +57,8 @@ def _tgv(st, launches, N=384):
     from . import tgv
     n = [N, N, N]
     tdt = getattr(torch, tgv.TORCH_DT[st])
-    elem = 8 if st == "f64" else 4
+    elem = {"f64": 8, "f32": 4}.get(st, 2)
+    pair = st in ("f64", "f32")             # 16-bit words take single steps
     _copy_probe(N ** 3 * 27 * elem)
     a = torch.empty(abi.row_shape(27, n, 0, 3), dtype=tdt, device="cuda:0")
     b = torch.empty_like(a)
@@ -67,12 +71,14 @@ def _tgv(st, launches, N=384):
     omega = 1.0 / (0.5 + 3.0 * 0.05 * L / 800.0)
     g = abi.resolve_bc(3, [abi.BC["Periodic"]] * 6)
     for _ in range(launches):
-        abi.step2(fa, fb, g, omega)
+        (abi.step2 if pair else abi.step)(fa, fb, g, omega)
         fa, fb = fb, fa
     torch.cuda.synchronize()
-    return {"kernel": "k_step2p" if st == "f64" else "k_step2p_dev", "cells": N ** 3, "q": 27,
-            "elem": elem, "steps_per_launch": 2, "copy_bytes": N ** 3 * 27 * elem,
-            "workload": f"D3Q27 TGV {N}^3, Re 800, {'FP64' if st == 'f64' else 'FP32 deviations'}"}
+    names = {"f64": "FP64", "f32": "FP32 deviations", "f16dev": "FP16 deviations",
+             "bf16dev": "BF16 deviations"}
+    return {"kernel": "k_step2p" if st == "f64" else ("k_step2p_dev" if pair else "k_step_dev"),
+            "cells": N ** 3, "q": 27, "elem": elem, "steps_per_launch": 2 if pair else 1,
+            "copy_bytes": N ** 3 * 27 * elem, "workload": f"D3Q27 TGV {N}^3, Re 800, {names[st]}"}
 
 
 def level64(launches):
@@ -88,7 +94,8 @@ def level64(launches):
 
 
 WORKLOADS = {"ensemble512": ensemble512, "tgv27": lambda k: _tgv("f64", k),
-             "tgv27_f32": lambda k: _tgv("f32", k), "level64": level64}
+             "tgv27_f32": lambda k: _tgv("f32", k), "tgv27_f16": lambda k: _tgv("f16dev", k),
+             "tgv27_bf16": lambda k: _tgv("bf16dev", k), "level64": level64}
 
 
 def main():
